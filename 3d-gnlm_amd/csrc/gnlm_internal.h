@@ -1,0 +1,34 @@
+// gnlm_internal.h -- launchers shared between the kernel files and the C-ABI
+// layer (gnlm_api.cpp).  Host-side only; no torch types anywhere.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "gnlm_device.h"
+#include "../../include/gnlm.h"
+
+namespace gnlm {
+
+// Exponent scale of the weight: w = exp(-(S/P)/h^2) = exp2(S * k).
+double weight_scale(int P, float h);
+
+// ---- tiled 3D forward (nlm3d_fwd.hip) --------------------------------------
+// Returns true when the tiled kernel handles (r_p, r_s): r_p in 0..3 and the
+// LDS image of a tile with its (r_p + r_s) halo fits in 160 KiB.
+bool fwd3d_tiled_supported(int r_p, int r_s);
+hipError_t fwd3d_tiled_launch(const InDesc &in, float *out, float *Zout, int r_p, int r_s, float k,
+                              hipStream_t s);
+
+// ---- tiled 2D forward (nlm2d_fwd.hip) --------------------------------------
+bool fwd2d_tiled_supported(int r_p, int r_s);
+hipError_t fwd2d_tiled_launch(const InDesc &in, float *out, float *Zout, int r_p, int r_s, float k,
+                              hipStream_t s);
+
+// ---- generic per-offset forward (nlm_generic.hip) --------------------------
+// Any radii; 2D when rpz = rsz = 0 and nz = 1.  Workspace is allocated inside.
+hipError_t fwd_generic_launch(const InDesc &in, float *out, float *Zout, int rp, int rs, int rpz, int rsz,
+                              float k, hipStream_t s);
+
+// ---- misc (gnlm_misc.hip) --------------------------------------------------
+hipError_t u8_to_f32_launch(const uint8_t *in, float *out, long long n, hipStream_t s);
+
+}  // namespace gnlm

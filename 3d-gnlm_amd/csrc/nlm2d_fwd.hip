@@ -1,0 +1,220 @@
+// nlm2d_fwd.hip -- tiled 2D NLM forward for gfx950.
+//
+// The 2D variant of the reference ("2D NLM/11x11/main.cu:36-63": patch SSD
+// over an nb x nb patch for every offset of the sr x sr window, weight
+// exp(-(S/nb^2)/PARAMETER^2), weighted mean of the candidate centres), with
+// the patch radius generic instead of hard-wired to nb = 11.
+//
+// One workgroup (256 threads) per 64 x 32 output tile:
+//   * the tile's input with its r_p + r_s halo is staged into LDS with
+//     reflect-101 applied on the fly;
+//   * per offset, phase 1 forms x-direction patch-row sums (s_x) for the
+//     tile's 32 + 2 r_p rows in segments of 16 outputs; each thread keeps its
+//     segment's centre samples in registers for the whole offset loop and
+//     reads only the shifted samples from LDS.  s_x is stored column-major
+//     with an odd pitch so both its writes and the column reads are
+//     conflict-free;
+//   * phase 2: a thread owns 8 consecutive rows of one column, loads the
+//     8 + 2 r_p s_x values it needs once, sums each output's window directly
+//     in registers, and accumulates Z and Y in registers.
+// Sums follow the canonical order (x sums fused with fma, then y sums left to
+// right), identical to the CPU oracle and the generic path.
+#include "gnlm_internal.h"
+
+namespace gnlm {
+namespace {
+
+constexpr int NT = 256;
+
+template <int RP>
+struct G2 {
+    static constexpr int TX = 64, TY = 32;
+    static constexpr int SEG = 16;                  // outputs per phase-1 segment
+    static constexpr int NSEG = TX / SEG;           // segments per row
+    static constexpr int EY = TY + 2 * RP;          // phase-1 rows
+    static constexpr int EYP = EY | 1;              // odd column pitch of s_x
+    static constexpr int NR = EY * NSEG;            // phase-1 work items
+    static constexpr int ROWS = (NR + NT - 1) / NT;
+    static constexpr int ES = SEG + 2 * RP;         // samples per segment
+    static constexpr int YR = 8;                    // rows per thread in phase 2
+    static constexpr int SXN = TX * EYP;
+};
+
+struct Fwd2Args {
+    InDesc in;
+    float *out;
+    float *Zout;
+    int rs;
+    float k;
+    int ntx, nty;
+};
+
+__host__ __device__ constexpr int lds2_x(int TX, int R) { return (TX + 2 * R) | 1; }
+
+template <int RP>
+size_t fwd2_lds_bytes(int rs)
+{
+    using G = G2<RP>;
+    const int R = RP + rs;
+    return ((size_t)lds2_x(G::TX, R) * (G::TY + 2 * R) + 2 * (size_t)G::SXN) * sizeof(float);
+}
+
+template <int RP>
+__global__ __launch_bounds__(NT) void nlm2d_fwd_tiled(Fwd2Args a)
+{
+    using G = G2<RP>;
+    extern __shared__ float lds[];
+    const int R = RP + a.rs;
+    const int LXr = G::TX + 2 * R;
+    const int LX = lds2_x(G::TX, R);
+    const int LY = G::TY + 2 * R;
+    float *U = lds;
+    float *SXB = lds + LX * LY;
+
+    const int t = xcd_tile(blockIdx.x, a.ntx * a.nty);
+    const int x0 = (t % a.ntx) * G::TX;
+    const int y0 = (t / a.ntx) * G::TY;
+    const int tid = threadIdx.x;
+
+    for (int i = tid; i < LX * LY; i += NT) {
+        const int lx = i % LX, ly = i / LX;
+        float v = 0.0f;
+        if (lx < LXr) v = load_in(a.in, x0 - R + lx, y0 - R + ly, 0);
+        U[i] = v;
+    }
+    __syncthreads();
+
+    // phase-1 items: (row ry, segment sg), ry fastest
+    float cen[G::ROWS][G::ES];
+    int base[G::ROWS];
+#pragma unroll
+    for (int q = 0; q < G::ROWS; ++q) {
+        const int r = tid + q * NT;
+        const int ry = r % G::EY, sg = r / G::EY;
+        base[q] = (R - RP + ry) * LX + (R - RP) + sg * G::SEG;
+        if (r < G::NR) {
+#pragma unroll
+            for (int i = 0; i < G::ES; ++i) cen[q][i] = U[base[q] + i];
+        }
+    }
+
+    // phase-2 ownership: column cx, rows cy0 .. cy0+YR-1
+    const int cx = tid % G::TX;
+    const int cy0 = (tid / G::TX) * G::YR;
+    float Zacc[G::YR], Yacc[G::YR];
+#pragma unroll
+    for (int m = 0; m < G::YR; ++m) {
+        Zacc[m] = 0.0f;
+        Yacc[m] = 0.0f;
+    }
+    const int candbase = (R + cy0) * LX + (R + cx);
+
+    int it = 0;
+    for (int dy = -a.rs; dy <= a.rs; ++dy)
+        for (int dx = -a.rs; dx <= a.rs; ++dx, ++it) {
+            float *SX = SXB + (it & 1) * G::SXN;
+            const int doff = dy * LX + dx;
+#pragma unroll
+            for (int q = 0; q < G::ROWS; ++q) {
+                const int r = tid + q * NT;
+                if (r < G::NR) {
+                    const int ry = r % G::EY, sg = r / G::EY;
+                    const float *sh = U + base[q] + doff;
+                    float e[G::ES];
+#pragma unroll
+                    for (int i = 0; i < G::ES; ++i) e[i] = cen[q][i] - sh[i];
+                    float *dst = SX + (sg * G::SEG) * G::EYP + ry;
+#pragma unroll
+                    for (int j = 0; j < G::SEG; ++j) {
+                        float acc = 0.0f;
+#pragma unroll
+                        for (int i = 0; i <= 2 * RP; ++i) acc = fmaf(e[j + i], e[j + i], acc);
+                        dst[j * G::EYP] = acc;
+                    }
+                }
+            }
+            __syncthreads();
+
+            float col[G::YR + 2 * RP];
+            const float *src = SX + cx * G::EYP + cy0;
+#pragma unroll
+            for (int j = 0; j < G::YR + 2 * RP; ++j) col[j] = src[j];
+            const float *cand = U + candbase + doff;
+#pragma unroll
+            for (int m = 0; m < G::YR; ++m) {
+                float S = 0.0f;
+#pragma unroll
+                for (int j = 0; j <= 2 * RP; ++j) S = S + col[m + j];
+                const float w = exp2_fast(S * a.k);
+                Zacc[m] = Zacc[m] + w;
+                Yacc[m] = fmaf(w, cand[m * LX], Yacc[m]);
+            }
+        }
+
+    const int gx = x0 + cx;
+    if (gx < a.in.nx) {
+#pragma unroll
+        for (int m = 0; m < G::YR; ++m) {
+            const int gy = y0 + cy0 + m;
+            if (gy < a.in.ny) {
+                const long long o = gx + (long long)a.in.nx * gy;
+                a.out[o] = Yacc[m] / Zacc[m];
+                if (a.Zout) a.Zout[o] = Zacc[m];
+            }
+        }
+    }
+}
+
+template <int RP>
+hipError_t launch2_rp(const InDesc &in, float *out, float *Zout, int rs, float k, hipStream_t s)
+{
+    using G = G2<RP>;
+    Fwd2Args a;
+    a.in = in;
+    a.out = out;
+    a.Zout = Zout;
+    a.rs = rs;
+    a.k = k;
+    a.ntx = (in.nx + G::TX - 1) / G::TX;
+    a.nty = (in.ny + G::TY - 1) / G::TY;
+    const size_t lds = fwd2_lds_bytes<RP>(rs);
+    hipError_t e = hipFuncSetAttribute((const void *)nlm2d_fwd_tiled<RP>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(nlm2d_fwd_tiled<RP>, dim3(a.ntx * a.nty), dim3(NT), lds, s, a);
+    return hipGetLastError();
+}
+
+constexpr size_t kLdsLimit = 160 * 1024;
+
+}  // namespace
+
+bool fwd2d_tiled_supported(int r_p, int r_s)
+{
+    if (r_s < 0) return false;
+    switch (r_p) {
+    case 0: return fwd2_lds_bytes<0>(r_s) <= kLdsLimit;
+    case 1: return fwd2_lds_bytes<1>(r_s) <= kLdsLimit;
+    case 2: return fwd2_lds_bytes<2>(r_s) <= kLdsLimit;
+    case 3: return fwd2_lds_bytes<3>(r_s) <= kLdsLimit;
+    case 4: return fwd2_lds_bytes<4>(r_s) <= kLdsLimit;
+    case 5: return fwd2_lds_bytes<5>(r_s) <= kLdsLimit;
+    default: return false;
+    }
+}
+
+hipError_t fwd2d_tiled_launch(const InDesc &in, float *out, float *Zout, int r_p, int r_s, float k,
+                              hipStream_t s)
+{
+    switch (r_p) {
+    case 0: return launch2_rp<0>(in, out, Zout, r_s, k, s);
+    case 1: return launch2_rp<1>(in, out, Zout, r_s, k, s);
+    case 2: return launch2_rp<2>(in, out, Zout, r_s, k, s);
+    case 3: return launch2_rp<3>(in, out, Zout, r_s, k, s);
+    case 4: return launch2_rp<4>(in, out, Zout, r_s, k, s);
+    case 5: return launch2_rp<5>(in, out, Zout, r_s, k, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace gnlm

@@ -1,0 +1,166 @@
+"""Python front end of libgnlm.so (the MI355X NLM library), over ctypes.
+
+The library's C-ABI (include/gnlm.h) takes plain device pointers and a
+hipStream_t; this module passes torch tensors' data pointers and torch's
+current stream.  torch is only the device-memory and stream plumbing here:
+every computation runs in the HIP kernels of libgnlm.so, and nothing falls
+back to a CPU or torch implementation -- if the library is missing the
+calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgnlm.so")
+
+OK, EINVAL, ENOMEM, EUNSUPPORTED, EHIP, ESTATE = range(6)
+PATH_AUTO, PATH_TILED, PATH_GENERIC = 0, 1, 2
+
+_lib = None
+
+
+class GnlmError(RuntimeError):
+    def __init__(self, status, where=""):
+        self.status = status
+        msg = lib().gnlm_status_string(status).decode() if _lib is not None else str(status)
+        super().__init__(f"{where}: {msg} (status {status})")
+
+
+def lib():
+    """Load libgnlm.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"libgnlm.so not built at {LIB_PATH}; run __graft_entry__.build()")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i, f, ll = ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_longlong
+    L.gnlm_status_string.restype = ctypes.c_char_p
+    L.gnlm_status_string.argtypes = [i]
+    L.gnlm_version.restype = i
+    L.gnlm3d_forward.argtypes = [vp, vp, vp, i, i, i, i, i, f, vp]
+    L.gnlm3d_forward_ex.argtypes = [vp, i, ll, ll, vp, vp, i, i, i, i, i, f, i, vp]
+    L.gnlm2d_forward.argtypes = [vp, vp, i, i, i, i, f, vp]
+    L.gnlm2d_forward_ex.argtypes = [vp, i, ll, vp, vp, i, i, i, i, f, i, vp]
+    L.gnlm_u8_to_f32.argtypes = [vp, vp, ll, vp]
+    L.gnlm_nl_means_3d_u8.argtypes = [vp, vp, i, i, i, i, i, f]
+    L.gnlm_nl_means_2d_u8.argtypes = [vp, vp, i, i, i, i, i, i, f]
+    L.gnlm_phantom3d.argtypes = [vp, vp, i, i, i, i, f, f, f, f, ctypes.c_ulonglong]
+    L.gnlm_phantom2d.argtypes = [vp, vp, i, i, i, f, f, f, f, f, f, ctypes.c_ulonglong]
+    L.gnlm_hash_f32.restype = ctypes.c_ulonglong
+    L.gnlm_hash_f32.argtypes = [vp, ll]
+    for name in ("gnlm3d_forward", "gnlm3d_forward_ex", "gnlm2d_forward", "gnlm2d_forward_ex",
+                 "gnlm_u8_to_f32", "gnlm_nl_means_3d_u8", "gnlm_nl_means_2d_u8", "gnlm_phantom3d",
+                 "gnlm_phantom2d"):
+        getattr(L, name).restype = i
+    _extend(L)
+    _lib = L
+    return L
+
+
+def _extend(L):
+    """Argument types of the backward / batch / tuning entry points."""
+    vp, i, f = ctypes.c_void_p, ctypes.c_int, ctypes.c_float
+    if hasattr(L, "gnlm3d_backward"):
+        L.gnlm3d_backward.argtypes = [vp, vp, vp, vp, i, i, i, i, i, f, vp, vp, vp]
+        L.gnlm3d_backward.restype = i
+
+
+def check(status, where=""):
+    if status != OK:
+        raise GnlmError(status, where)
+
+
+def _stream(t):
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream(t.device).cuda_stream)
+
+
+def _dev_f32(t, name):
+    import torch
+    if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float32 and t.is_contiguous()):
+        raise TypeError(f"{name} must be a contiguous float32 CUDA tensor")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def nlm3d(u, r_p, r_s, h, want_Z=False, path=PATH_AUTO, out=None):
+    """3D NLM of u[z, y, x] (float32 CUDA tensor), reflect-101 borders."""
+    import torch
+    nz, ny, nx = u.shape
+    y = torch.empty_like(u) if out is None else out
+    Z = torch.empty_like(u) if want_Z else None
+    st = lib().gnlm3d_forward_ex(_dev_f32(u, "u"), 0, 0, 0, _dev_f32(y, "out"),
+                                 _dev_f32(Z, "Z") if Z is not None else None,
+                                 nx, ny, nz, int(r_p), int(r_s), float(h), int(path), _stream(u))
+    check(st, "gnlm3d_forward_ex")
+    return (y, Z) if want_Z else y
+
+
+def nlm2d(u, r_p, r_s, h, want_Z=False, path=PATH_AUTO):
+    """2D NLM of u[y, x] (float32 CUDA tensor), reflect-101 borders."""
+    import torch
+    ny, nx = u.shape
+    y = torch.empty_like(u)
+    Z = torch.empty_like(u) if want_Z else None
+    st = lib().gnlm2d_forward_ex(_dev_f32(u, "u"), 0, 0, _dev_f32(y, "out"),
+                                 _dev_f32(Z, "Z") if Z is not None else None,
+                                 nx, ny, int(r_p), int(r_s), float(h), int(path), _stream(u))
+    check(st, "gnlm2d_forward_ex")
+    return (y, Z) if want_Z else y
+
+
+def nlm3d_padded(up, pad, nx, ny, nz, r_p, r_s, h, path=PATH_AUTO):
+    """3D NLM of a pre-padded volume up[(nz+2pad), (ny+2pad), (nx+2pad)]."""
+    import torch
+    y = torch.empty((nz, ny, nx), dtype=torch.float32, device=up.device)
+    st = lib().gnlm3d_forward_ex(_dev_f32(up, "up"), int(pad), 0, 0, _dev_f32(y, "out"), None,
+                                 nx, ny, nz, int(r_p), int(r_s), float(h), int(path), _stream(up))
+    check(st, "gnlm3d_forward_ex")
+    return y
+
+
+def nl_means_3d_u8(padded_u8, X, Y, depth, SS, NS, h=10.0):
+    """Host-pointer drop-in backing: padded uint8 volume in, X*Y*depth floats out."""
+    a = np.ascontiguousarray(padded_u8, dtype=np.uint8)
+    out = np.empty((depth, Y, X), np.float32)
+    st = lib().gnlm_nl_means_3d_u8(a.ctypes.data, out.ctypes.data, X, Y, depth, SS, NS, float(h))
+    check(st, "gnlm_nl_means_3d_u8")
+    return out
+
+
+def nl_means_2d_u8(padded_u8, W, H, sr, nb, X, Y, h=10.0):
+    a = np.ascontiguousarray(padded_u8, dtype=np.uint8)
+    out = np.zeros((Y, Y), np.float32)
+    st = lib().gnlm_nl_means_2d_u8(a.ctypes.data, out.ctypes.data, W, H, sr, nb, X, Y, float(h))
+    check(st, "gnlm_nl_means_2d_u8")
+    return out
+
+
+def phantom3d(nx, ny, nz, n_ellipsoids=12, axis_lo=4.0, axis_hi=20.0, background=0.05, sigma=0.05, seed=0):
+    """Seeded ellipsoid phantom -> (noisy, clean) float32 arrays [nz, ny, nx]."""
+    noisy = np.empty((nz, ny, nx), np.float32)
+    clean = np.empty((nz, ny, nx), np.float32)
+    st = lib().gnlm_phantom3d(noisy.ctypes.data, clean.ctypes.data, nx, ny, nz, n_ellipsoids,
+                              axis_lo, axis_hi, background, sigma, seed)
+    if st != 0:
+        raise ValueError("gnlm_phantom3d: bad arguments")
+    return noisy, clean
+
+
+def phantom2d(nx, ny, n_shapes=40, size_lo=0.008, size_hi=0.125, sine_amp=0.2, period_lo=32.0,
+              period_hi=256.0, sigma=0.08, seed=0):
+    """Seeded 2D rectangles/discs + sinusoid phantom -> (noisy, clean) [ny, nx]."""
+    noisy = np.empty((ny, nx), np.float32)
+    clean = np.empty((ny, nx), np.float32)
+    st = lib().gnlm_phantom2d(noisy.ctypes.data, clean.ctypes.data, nx, ny, n_shapes, size_lo, size_hi,
+                              sine_amp, period_lo, period_hi, sigma, seed)
+    if st != 0:
+        raise ValueError("gnlm_phantom2d: bad arguments")
+    return noisy, clean
+
+
+def hash_f32(a):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return int(lib().gnlm_hash_f32(a.ctypes.data, a.size))

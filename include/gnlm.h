@@ -1,0 +1,115 @@
+/*
+ * gnlm.h -- C-ABI of the MI355X (gfx950) non-local-means library libgnlm.so.
+ *
+ * This is the drop-in boundary for the 3D-GNLM NLM path.  The reference's
+ * host launchers are C++ static members that own all device memory for one
+ * synchronous call:
+ *   3D: NLMeansProcessor::NL_Means(unsigned char*, float*)   3DNLM/main.h:6,
+ *       body 3DNLM/main.cu:7607-7680 (geometry from 3DNLM/config.h:4-9)
+ *   2D: NLMeansProcessor::NL_Means(unsigned char*, float*, int W, int H,
+ *       int sr_size, int nb_size, int IMAGE_SIZE_X, int IMAGE_SIZE_Y)
+ *       "2D NLM/11x11/main.h:6", body "2D NLM/11x11/main.cu:472-539".
+ * The C++ drop-in with those exact signatures is include/gnlm_main.h; it is
+ * built on the host-pointer entry points at the end of this file.
+ *
+ * Conventions of the device-pointer entry points:
+ *   - all pointers are device pointers owned by the caller; nothing is freed;
+ *   - volumes are x-fastest, [nz][ny][nx] FP32, the reference's output layout
+ *     (3DNLM/main.cu:125);
+ *   - the input is unpadded; samples outside the volume use reflect-101, the
+ *     padding the reference's driver builds on the host (3DNLM/nlm.cpp:36-41);
+ *   - r_p / r_s are patch / search RADII: NS = 2 r_p + 1, SS = 2 r_s + 1;
+ *   - h is the filtering parameter: w = exp(-(S / NS^d) / h^2) with S the
+ *     patch sum of squared differences (3DNLM/main.cu:107-111);
+ *   - every call is stream-ordered on `stream` (a hipStream_t, NULL = the
+ *     default stream) and returns a gnlm_status; nothing throws across the ABI;
+ *   - calls on distinct streams may run concurrently from different threads.
+ * The reference swallows errors (3DNLM/cuda_utility.h:9-16 prints and
+ * returns a cudaError_t that its callers ignore); this ABI returns them.
+ */
+#ifndef GNLM_H
+#define GNLM_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum gnlm_status {
+    GNLM_OK = 0,
+    GNLM_EINVAL = 1,        /* bad dimension, radius, h or NULL pointer      */
+    GNLM_ENOMEM = 2,        /* device or host allocation failed              */
+    GNLM_EUNSUPPORTED = 3,  /* parameter combination not implemented         */
+    GNLM_EHIP = 4,          /* HIP runtime error (launch or copy)            */
+    GNLM_ESTATE = 5         /* object used in the wrong state                */
+} gnlm_status;
+
+/* Text for a status code (static storage).
+ * Replaces the cudaGetErrorString printing of 3DNLM/cuda_utility.h:9-16. */
+const char *gnlm_status_string(int status);
+
+/* Library version, major*10000 + minor*100 + patch. */
+int gnlm_version(void);
+
+/* Forward-path selection for the *_ex entry points. */
+enum { GNLM_PATH_AUTO = 0, GNLM_PATH_TILED = 1, GNLM_PATH_GENERIC = 2 };
+
+/* 3D forward.  d_Z (nullable) receives the normaliser Z(x) = sum_d w_d(x),
+ * which the backward pass reuses.
+ * Replaces the kernel launches of 3DNLM/main.cu:7622-7667. */
+int gnlm3d_forward(const float *d_in, float *d_out, float *d_Z, int nx, int ny, int nz,
+                   int r_p, int r_s, float h, void *stream);
+
+/* 3D forward with an explicit input layout and path.
+ * in_pad == 0: d_in is the unpadded volume (reflect-101 on device);
+ * in_pad  > 0: d_in is padded by in_pad >= r_p + r_s voxels per side, with
+ *              row stride in_sy and plane stride in_sz (elements; 0 selects
+ *              the dense values nx+2*in_pad and in_sy*(ny+2*in_pad)) --
+ *              the layout the reference's launcher receives
+ *              (3DNLM/main.cu:7613-7620).
+ * path: GNLM_PATH_AUTO / _TILED / _GENERIC (tiled returns EUNSUPPORTED for
+ * radii it does not handle). */
+int gnlm3d_forward_ex(const float *d_in, int in_pad, long long in_sy, long long in_sz, float *d_out,
+                      float *d_Z, int nx, int ny, int nz, int r_p, int r_s, float h, int path,
+                      void *stream);
+
+/* 2D forward on an nx * ny image (x fastest).
+ * Replaces the kernel launches of "2D NLM/11x11/main.cu:486-525". */
+int gnlm2d_forward(const float *d_in, float *d_out, int nx, int ny, int r_p, int r_s, float h,
+                   void *stream);
+
+/* 2D forward with explicit input layout and path (see gnlm3d_forward_ex;
+ * in_sy is the row stride of a padded input). */
+int gnlm2d_forward_ex(const float *d_in, int in_pad, long long in_sy, float *d_out, float *d_Z,
+                      int nx, int ny, int r_p, int r_s, float h, int path, void *stream);
+
+/* uint8 -> float32 element conversion (n elements), for 8-bit inputs such as
+ * the reference's (3DNLM/main.cu:7620 copies unsigned char volumes). */
+int gnlm_u8_to_f32(const unsigned char *d_in, float *d_out, long long n, void *stream);
+
+/* ---- host-pointer, synchronous entry points (the drop-in's backing) ----
+ * They allocate device memory for one call, copy in, run, copy out and free,
+ * as the reference's launchers do (3DNLM/main.cu:7609-7676).
+ *
+ * 3D: h_in is the padded uint8 volume of (X+SS+NS-2)*(Y+SS+NS-2)*(depth+SS+NS-2)
+ * bytes, x fastest (3DNLM/nlm.cpp:48-61); h_out receives X*Y*depth floats.
+ * SS and NS are the window SIZES (odd).  h is on the 0-255 scale
+ * (the reference hard-wires PARAMETER = 10, 3DNLM/main.cu:13). */
+int gnlm_nl_means_3d_u8(const unsigned char *h_in, float *h_out, int X, int Y, int depth, int SS,
+                        int NS, float h);
+
+/* 2D: h_in is the padded uint8 image whose row stride is W (the reference
+ * kernels index it with stride W, "2D NLM/11x11/main.cu:36"), padded by
+ * (sr + nb - 2)/2 per side; h_out receives rows of IMAGE_SIZE_Y floats
+ * (the reference's output stride, "2D NLM/11x11/main.cu:64") of which the
+ * first IMAGE_SIZE_X are written, for IMAGE_SIZE_Y rows.  H is the padded
+ * row count. */
+int gnlm_nl_means_2d_u8(const unsigned char *h_in, float *h_out, int W, int H, int sr_size,
+                        int nb_size, int image_size_x, int image_size_y, float h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GNLM_H */

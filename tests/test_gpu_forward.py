@@ -1,0 +1,201 @@
+"""GPU parity of the forward kernels against the CPU oracle.
+
+Tolerances (FP32):
+  * GPU vs the oracle's FP32 path (same canonical summation order, so the
+    patch distances agree bit for bit; the only difference is exp2 on the GPU
+    against expf on the CPU): max relative error <= 1e-5 (the target stated
+    for the path), typically ~1e-7.
+  * GPU vs FP64: max relative error <= 1e-5.
+  * tiled vs generic path, padded vs reflect input: bit-identical.
+"""
+import numpy as np
+import pytest
+import torch
+
+import gnlm
+import oracle
+from nlm_metrics import max_rel, report
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def _run3(dev, u, rp, rs, h, path=gnlm.PATH_AUTO):
+    t = torch.from_numpy(np.ascontiguousarray(u, np.float32)).to(dev)
+    y = gnlm.nlm3d(t, rp, rs, h, path=path)
+    torch.cuda.synchronize()
+    return y.cpu().numpy()
+
+
+def _run2(dev, u, rp, rs, h, path=gnlm.PATH_AUTO):
+    t = torch.from_numpy(np.ascontiguousarray(u, np.float32)).to(dev)
+    y = gnlm.nlm2d(t, rp, rs, h, path=path)
+    torch.cuda.synchronize()
+    return y.cpu().numpy()
+
+
+CASES3 = [
+    # (nz, ny, nx), rp, rs, h
+    ((16, 16, 16), 1, 2, 0.3),
+    ((17, 19, 23), 2, 3, 0.25),
+    ((9, 33, 40), 1, 5, 0.2),
+    ((5, 6, 7), 1, 3, 0.3),       # dims smaller than the halo: repeated reflection
+    ((12, 12, 12), 0, 2, 0.3),    # 1-voxel patch
+    ((12, 13, 14), 2, 0, 0.3),    # search window = self only
+    ((10, 18, 21), 3, 2, 0.4),
+    ((1, 20, 20), 1, 2, 0.3),     # single plane
+]
+
+
+@pytest.mark.parametrize("shape,rp,rs,h", CASES3)
+def test_3d_matches_oracle(dev, shape, rp, rs, h):
+    rng = np.random.default_rng(hash((shape, rp, rs)) & 0xFFFF)
+    u = rng.random(shape, dtype=np.float32)
+    print(f"case 3d shape={shape} rp={rp} rs={rs} h={h}", flush=True)
+    y = _run3(dev, u, rp, rs, h)
+    ref32 = oracle.nlm3d_brute(u, rp, rs, h)
+    ref64 = oracle.nlm3d_brute(u.astype(np.float64), rp, rs, h)
+    assert max_rel(y, ref32) <= TOL, report(y, ref32)
+    assert max_rel(y, ref64) <= TOL, report(y, ref64)
+
+
+@pytest.mark.parametrize("shape,rp,rs,h", CASES3[:4] + CASES3[6:7])
+def test_3d_tiled_equals_generic(dev, shape, rp, rs, h):
+    rng = np.random.default_rng(7)
+    u = rng.random(shape, dtype=np.float32)
+    a = _run3(dev, u, rp, rs, h, gnlm.PATH_TILED)
+    b = _run3(dev, u, rp, rs, h, gnlm.PATH_GENERIC)
+    assert np.array_equal(a, b), report(a, b)
+
+
+def test_3d_large_patch_generic(dev):
+    # the reference's default geometry class (NS = 17 -> r_p = 8), small volume
+    u, _ = gnlm.phantom3d(20, 18, 12, n_ellipsoids=3, axis_lo=3, axis_hi=7, sigma=0.05, seed=3)
+    y = _run3(dev, u, 8, 2, 0.2)
+    ref = oracle.nlm3d_separable_f32(u, 8, 2, 0.2)
+    assert max_rel(y, ref) <= TOL, report(y, ref)
+
+
+def test_3d_config1(dev):
+    """96x96x48 ellipsoid phantom, patch 3^3, search 7^3, h = 0.05."""
+    u, _ = gnlm.phantom3d(96, 96, 48, n_ellipsoids=12, axis_lo=4, axis_hi=20, background=0.05,
+                          sigma=0.05, seed=20261204)
+    y = _run3(dev, u, 1, 3, 0.05)
+    ref32 = oracle.nlm3d_separable_f32(u, 1, 3, 0.05)
+    ref64 = oracle.nlm3d_separable_f64(u, 1, 3, 0.05)
+    print("cfg1 vs fp32", report(y, ref32), "vs fp64", report(y, ref64))
+    assert max_rel(y, ref32) <= TOL
+    assert max_rel(y, ref64) <= TOL
+
+
+def test_3d_padded_equals_reflect(dev):
+    rng = np.random.default_rng(11)
+    u = rng.random((13, 17, 19), dtype=np.float32)
+    rp, rs = 2, 3
+    R = rp + rs
+    up = np.pad(u, R, mode="reflect")  # numpy 'reflect' is reflect-101
+    a = _run3(dev, u, rp, rs, 0.3)
+    t = torch.from_numpy(up).to(dev)
+    b = gnlm.nlm3d_padded(t, R, 19, 17, 13, rp, rs, 0.3).cpu().numpy()
+    assert np.array_equal(a, b)
+
+
+def test_3d_deterministic(dev):
+    u, _ = gnlm.phantom3d(40, 36, 34, n_ellipsoids=5, axis_lo=3, axis_hi=9, sigma=0.05, seed=5)
+    a = _run3(dev, u, 2, 5, 0.05)
+    b = _run3(dev, u, 2, 5, 0.05)
+    assert np.array_equal(a, b)
+
+
+@pytest.mark.slow
+def test_3d_config3_slabs(dev):
+    """256^3, patch 5^3, search 11^3: full GPU run, slabs checked against the
+    CPU FP32 path and FP64; whole-volume properties checked everywhere."""
+    u, _ = gnlm.phantom3d(256, 256, 256, n_ellipsoids=64, axis_lo=5, axis_hi=40, background=0.05,
+                          sigma=0.05, seed=20261206)
+    y = _run3(dev, u, 2, 5, 0.05)
+    assert np.isfinite(y).all()
+    assert y.min() >= u.min() - 1e-6 and y.max() <= u.max() + 1e-6  # convex combination
+    for z0 in (0, 128, 253):
+        ref32 = oracle.nlm3d_separable_f32(u, 2, 5, 0.05, z0=z0, z1=z0 + 3)
+        ref64 = oracle.nlm3d_separable_f64(u, 2, 5, 0.05, z0=z0, z1=z0 + 3)
+        print("cfg3 slab", z0, "vs fp32", report(y[z0:z0 + 3], ref32), "vs fp64", report(y[z0:z0 + 3], ref64))
+        assert max_rel(y[z0:z0 + 3], ref32) <= TOL
+        assert max_rel(y[z0:z0 + 3], ref64) <= TOL
+
+
+CASES2 = [
+    ((64, 64), 1, 2, 0.2),
+    ((37, 101), 2, 5, 0.1),
+    ((70, 130), 5, 10, 0.3),   # the reference's own 2D geometry (nb = 11, sr = 21)
+    ((5, 9), 2, 3, 0.3),
+    ((40, 50), 0, 3, 0.2),
+]
+
+
+@pytest.mark.parametrize("shape,rp,rs,h", CASES2)
+def test_2d_matches_oracle(dev, shape, rp, rs, h):
+    rng = np.random.default_rng(sum(shape) + rp + rs)
+    u = rng.random(shape, dtype=np.float32)
+    print(f"case 2d shape={shape} rp={rp} rs={rs} h={h}", flush=True)
+    y = _run2(dev, u, rp, rs, h)
+    ref32 = oracle.nlm2d_brute(u, rp, rs, h)
+    ref64 = oracle.nlm2d_brute(u.astype(np.float64), rp, rs, h)
+    assert max_rel(y, ref32) <= TOL, report(y, ref32)
+    assert max_rel(y, ref64) <= TOL, report(y, ref64)
+
+
+@pytest.mark.parametrize("shape,rp,rs,h", CASES2[:3])
+def test_2d_tiled_equals_generic(dev, shape, rp, rs, h):
+    rng = np.random.default_rng(3)
+    u = rng.random(shape, dtype=np.float32)
+    a = _run2(dev, u, rp, rs, h, gnlm.PATH_TILED)
+    b = _run2(dev, u, rp, rs, h, gnlm.PATH_GENERIC)
+    assert np.array_equal(a, b), report(a, b)
+
+
+def test_2d_config2(dev):
+    """2048^2 phantom, patch 5x5, search 11x11, h = 0.08."""
+    u, _ = gnlm.phantom2d(2048, 2048, sigma=0.08, seed=20261205)
+    y = _run2(dev, u, 2, 5, 0.08)
+    ref = oracle.nlm2d_separable_f32(u, 2, 5, 0.08)
+    print("cfg2 vs fp32", report(y, ref))
+    assert max_rel(y, ref) <= TOL
+
+
+def test_z_invariant_3d_equals_2d(dev):
+    """A volume constant along z: 3D NLM (NS, SS, h) equals 2D NLM (NS, SS, h),
+    since S_3D = NS * S_2D and P_3D = NS * P_2D (the reference's own driver
+    builds exactly such volumes, 3DNLM/nlm.cpp:55-61)."""
+    rng = np.random.default_rng(9)
+    img = rng.random((30, 34), dtype=np.float32)
+    vol = np.repeat(img[None], 9, axis=0)
+    y3 = _run3(dev, vol, 1, 2, 0.3)
+    y2 = _run2(dev, img, 1, 2, 0.3)
+    for z in range(9):
+        assert max_rel(y3[z], y2) <= 2e-6
+
+
+def test_dropin_3d_u8(dev):
+    """The reference's drop-in input: padded uint8, h = PARAMETER = 10."""
+    rng = np.random.default_rng(4)
+    X, Y, D, SS, NS = 20, 18, 10, 5, 3
+    pad = (SS + NS - 2) // 2
+    img = rng.integers(0, 256, (D, Y, X), dtype=np.uint8)
+    padded = np.pad(img, pad, mode="reflect")
+    out = gnlm.nl_means_3d_u8(padded, X, Y, D, SS, NS, 10.0)
+    ref = oracle.nlm3d_brute(img.astype(np.float64), (NS - 1) // 2, (SS - 1) // 2, 10.0)
+    assert max_rel(out, ref) <= TOL, report(out, ref)
+
+
+def test_dropin_2d_u8(dev):
+    rng = np.random.default_rng(5)
+    n, sr, nb = 48, 7, 5
+    pad = (sr + nb - 2) // 2
+    img = rng.integers(0, 256, (n, n), dtype=np.uint8)
+    padded = np.pad(img, pad, mode="reflect")
+    W = H = n + 2 * pad
+    out = gnlm.nl_means_2d_u8(padded, W, H, sr, nb, n, n, 10.0)
+    ref = oracle.nlm2d_brute(img.astype(np.float64), (nb - 1) // 2, (sr - 1) // 2, 10.0)
+    assert max_rel(out, ref) <= TOL, report(out, ref)
