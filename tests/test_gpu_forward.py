@@ -1,11 +1,11 @@
 """GPU parity of the forward kernels against the CPU oracle.
 
-Tolerances (FP32):
-  * GPU vs the oracle's FP32 path (same canonical summation order, so the
-    patch distances agree bit for bit; the only difference is exp2 on the GPU
-    against expf on the CPU): max relative error <= 1e-5 (the target stated
-    for the path), typically ~1e-7.
-  * GPU vs FP64: max relative error <= 1e-5.
+Tolerances (FP32), stated per comparison:
+  * uniform random inputs (outputs bounded away from 0): max relative error
+    |a-b|/max(|b|,1e-6) <= 1e-5 against both the oracle's FP32 path and FP64.
+  * phantom workloads (background 0.05 + noise, so a few denoised voxels sit
+    near 0 and the plain relative error is ill-conditioned there): relative
+    error floored at 1e-3 of the output range, <= 1e-5, against both.
   * tiled vs generic path, padded vs reflect input: bit-identical.
 """
 import numpy as np
@@ -14,7 +14,7 @@ import torch
 
 import gnlm
 import oracle
-from nlm_metrics import max_rel, report
+from nlm_metrics import max_rel, max_rel_fl, report
 
 pytestmark = pytest.mark.gpu
 
@@ -74,7 +74,7 @@ def test_3d_large_patch_generic(dev):
     u, _ = gnlm.phantom3d(20, 18, 12, n_ellipsoids=3, axis_lo=3, axis_hi=7, sigma=0.05, seed=3)
     y = _run3(dev, u, 8, 2, 0.2)
     ref = oracle.nlm3d_separable_f32(u, 8, 2, 0.2)
-    assert max_rel(y, ref) <= TOL, report(y, ref)
+    assert max_rel_fl(y, ref) <= TOL, report(y, ref)
 
 
 def test_3d_config1(dev):
@@ -85,8 +85,8 @@ def test_3d_config1(dev):
     ref32 = oracle.nlm3d_separable_f32(u, 1, 3, 0.05)
     ref64 = oracle.nlm3d_separable_f64(u, 1, 3, 0.05)
     print("cfg1 vs fp32", report(y, ref32), "vs fp64", report(y, ref64))
-    assert max_rel(y, ref32) <= TOL
-    assert max_rel(y, ref64) <= TOL
+    assert max_rel_fl(y, ref32) <= TOL
+    assert max_rel_fl(y, ref64) <= TOL
 
 
 def test_3d_padded_equals_reflect(dev):
@@ -121,8 +121,8 @@ def test_3d_config3_slabs(dev):
         ref32 = oracle.nlm3d_separable_f32(u, 2, 5, 0.05, z0=z0, z1=z0 + 3)
         ref64 = oracle.nlm3d_separable_f64(u, 2, 5, 0.05, z0=z0, z1=z0 + 3)
         print("cfg3 slab", z0, "vs fp32", report(y[z0:z0 + 3], ref32), "vs fp64", report(y[z0:z0 + 3], ref64))
-        assert max_rel(y[z0:z0 + 3], ref32) <= TOL
-        assert max_rel(y[z0:z0 + 3], ref64) <= TOL
+        assert max_rel_fl(y[z0:z0 + 3], ref32) <= TOL
+        assert max_rel_fl(y[z0:z0 + 3], ref64) <= TOL
 
 
 CASES2 = [
@@ -161,7 +161,7 @@ def test_2d_config2(dev):
     y = _run2(dev, u, 2, 5, 0.08)
     ref = oracle.nlm2d_separable_f32(u, 2, 5, 0.08)
     print("cfg2 vs fp32", report(y, ref))
-    assert max_rel(y, ref) <= TOL
+    assert max_rel_fl(y, ref) <= TOL
 
 
 def test_z_invariant_3d_equals_2d(dev):
