@@ -111,6 +111,16 @@ int gnlm2d_forward(const float *d_in, float *d_out, int nx, int ny, int r_p, int
     return gnlm2d_forward_ex(d_in, 0, 0, d_out, nullptr, nx, ny, r_p, r_s, h, GNLM_PATH_AUTO, stream);
 }
 
+int gnlm3d_backward(const float *d_in, const float *d_out, const float *d_Z, const float *d_gout, int nx, int ny,
+                    int nz, int r_p, int r_s, float h, float *d_gin, float *d_gh, void *stream)
+{
+    if (!d_in || !d_out || !d_Z || !d_gout || !d_gin) return GNLM_EINVAL;
+    if (nx < 1 || ny < 1 || nz < 1 || r_p < 0 || r_s < 0 || bad_h(h)) return GNLM_EINVAL;
+    if (!bwd3d_supported(r_p, r_s)) return GNLM_EUNSUPPORTED;
+    return hip_status(bwd3d_launch(d_in, d_out, d_Z, d_gout, nx, ny, nz, r_p, r_s, h, d_gin, d_gh,
+                                   (hipStream_t)stream));
+}
+
 int gnlm_u8_to_f32(const unsigned char *d_in, float *d_out, long long n, void *stream)
 {
     if (!d_in || !d_out || n < 0) return GNLM_EINVAL;

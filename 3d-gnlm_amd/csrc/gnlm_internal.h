@@ -28,6 +28,11 @@ hipError_t fwd2d_tiled_launch(const InDesc &in, float *out, float *Zout, int r_p
 hipError_t fwd_generic_launch(const InDesc &in, float *out, float *Zout, int rp, int rs, int rpz, int rsz,
                               float k, hipStream_t s);
 
+// ---- adjoint (nlm3d_bwd.hip) -------------------------------------------------
+bool bwd3d_supported(int r_p, int r_s);
+hipError_t bwd3d_launch(const float *u, const float *y, const float *Z, const float *g, int nx, int ny, int nz,
+                        int rp, int rs, float h, float *gu, float *gh, hipStream_t s);
+
 // ---- misc (gnlm_misc.hip) --------------------------------------------------
 hipError_t u8_to_f32_launch(const uint8_t *in, float *out, long long n, hipStream_t s);
 

@@ -111,6 +111,19 @@ def nlm2d(u, r_p, r_s, h, want_Z=False, path=PATH_AUTO):
     return (y, Z) if want_Z else y
 
 
+def nlm3d_backward(u, y, Z, gy, r_p, r_s, h):
+    """Adjoint of nlm3d: returns (dL/du, dL/dh) as (tensor like u, 0-dim tensor)."""
+    import torch
+    nz, ny, nx = u.shape
+    gu = torch.empty_like(u)
+    gh = torch.empty((), dtype=torch.float32, device=u.device)
+    st = lib().gnlm3d_backward(_dev_f32(u, "u"), _dev_f32(y, "y"), _dev_f32(Z, "Z"), _dev_f32(gy, "gy"),
+                               nx, ny, nz, int(r_p), int(r_s), float(h), _dev_f32(gu, "gu"),
+                               ctypes.c_void_p(gh.data_ptr()), _stream(u))
+    check(st, "gnlm3d_backward")
+    return gu, gh
+
+
 def nlm3d_padded(up, pad, nx, ny, nz, r_p, r_s, h, path=PATH_AUTO):
     """3D NLM of a pre-padded volume up[(nz+2pad), (ny+2pad), (nx+2pad)]."""
     import torch

@@ -84,6 +84,17 @@ int gnlm2d_forward(const float *d_in, float *d_out, int nx, int ny, int r_p, int
 int gnlm2d_forward_ex(const float *d_in, int in_pad, long long in_sy, float *d_out, float *d_Z,
                       int nx, int ny, int r_p, int r_s, float h, int path, void *stream);
 
+/* 3D adjoint (no counterpart in the reference, which has no backward pass;
+ * it differentiates the forward above).  Given the forward's input d_in,
+ * output d_out and normaliser d_Z (from gnlm3d_forward) and d_gout = dL/dy,
+ * writes d_gin = dL/d(input) (reflect-101 copies folded back onto their
+ * source voxels) and, if d_gh is not NULL, one float dL/dh.  Patch distances
+ * are recomputed; nothing of size voxels x offsets is stored.  Results are
+ * deterministic.  Returns GNLM_EUNSUPPORTED when the tile's LDS image for
+ * (r_p, r_s) exceeds 160 KiB (r_p = 1 supports r_s <= 4). */
+int gnlm3d_backward(const float *d_in, const float *d_out, const float *d_Z, const float *d_gout, int nx,
+                    int ny, int nz, int r_p, int r_s, float h, float *d_gin, float *d_gh, void *stream);
+
 /* uint8 -> float32 element conversion (n elements), for 8-bit inputs such as
  * the reference's (3DNLM/main.cu:7620 copies unsigned char volumes). */
 int gnlm_u8_to_f32(const unsigned char *d_in, float *d_out, long long n, void *stream);

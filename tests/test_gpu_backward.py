@@ -1,0 +1,111 @@
+"""GPU adjoint: dL/d(input) and dL/dh.
+
+Tolerances:
+  * vs the oracle's FP64 chain-rule adjoint on the same input and dL/dy:
+    max |gpu - ref| <= 1e-4 * max |ref| for dL/du, and relative 1e-4 for dL/dh.
+  * vs FP64 central finite differences (eps = 1e-4) of the MSE loss against
+    the clean phantom on a 24^3 crop (the differentiable workload's check):
+    |gpu - fd| <= 1e-4 * max(|fd|, max|grad|) at sampled voxels and for h.
+"""
+import numpy as np
+import pytest
+import torch
+
+import gnlm
+import oracle
+from nlm_autograd import nlm3d
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu_grad(dev, u, g, rp, rs, h):
+    ut = torch.from_numpy(np.ascontiguousarray(u, np.float32)).to(dev)
+    y, Z = gnlm.nlm3d(ut, rp, rs, h, want_Z=True)
+    gt = torch.from_numpy(np.ascontiguousarray(g, np.float32)).to(dev)
+    gu, gh = gnlm.nlm3d_backward(ut, y, Z, gt, rp, rs, h)
+    torch.cuda.synchronize()
+    return gu.cpu().numpy().astype(np.float64), float(gh.item())
+
+
+@pytest.mark.parametrize("shape,rp,rs,h", [
+    ((7, 8, 9), 1, 2, 0.3),
+    ((10, 12, 11), 1, 3, 0.25),
+    ((6, 9, 8), 0, 2, 0.3),
+    ((12, 12, 12), 1, 4, 0.2),
+    ((3, 17, 18), 1, 2, 0.3),   # thin volume: reflect-101 folds many positions
+])
+def test_backward_matches_oracle(dev, shape, rp, rs, h):
+    rng = np.random.default_rng(sum(shape) * 7 + rs)
+    u = rng.random(shape).astype(np.float32)
+    g = rng.standard_normal(shape).astype(np.float32)
+    print(f"case bwd shape={shape} rp={rp} rs={rs}", flush=True)
+    gu, gh = _gpu_grad(dev, u, g, rp, rs, h)
+    ref_gu, ref_gh = oracle.nlm3d_grad_f64(u.astype(np.float64), g.astype(np.float64), rp, rs, h)
+    scale = np.abs(ref_gu).max()
+    assert np.abs(gu - ref_gu).max() <= 1e-4 * scale, (np.abs(gu - ref_gu).max(), scale)
+    assert abs(gh - ref_gh) <= 1e-4 * max(abs(ref_gh), 1e-8), (gh, ref_gh)
+
+
+def test_backward_deterministic(dev):
+    rng = np.random.default_rng(1)
+    u = rng.random((20, 22, 24)).astype(np.float32)
+    g = rng.standard_normal(u.shape).astype(np.float32)
+    a = _gpu_grad(dev, u, g, 1, 3, 0.2)
+    b = _gpu_grad(dev, u, g, 1, 3, 0.2)
+    assert np.array_equal(a[0], b[0]) and a[1] == b[1]
+
+
+def _crop24():
+    noisy, clean = gnlm.phantom3d(128, 128, 128, n_ellipsoids=24, axis_lo=5, axis_hi=30, background=0.05,
+                                  sigma=0.05, seed=20261207)
+    s = slice(52, 76)
+    return np.ascontiguousarray(noisy[s, s, s]), np.ascontiguousarray(clean[s, s, s])
+
+
+def test_backward_vs_finite_differences_crop24(dev):
+    """24^3 crop, patch 3^3, search 9^3, h = 0.1: MSE loss vs the clean phantom."""
+    u, clean = _crop24()
+    rp, rs, h = 1, 4, 0.1
+    ut = torch.from_numpy(u).to(dev).requires_grad_(True)
+    ht = torch.tensor(h, dtype=torch.float32, device=dev, requires_grad=True)
+    ct = torch.from_numpy(clean).to(dev)
+    y = nlm3d(ut, ht, rp, rs)
+    loss = ((y - ct) ** 2).mean()
+    loss.backward()
+    gu = ut.grad.cpu().numpy().astype(np.float64)
+    gh = float(ht.grad.item())
+    u64, c64 = u.astype(np.float64), clean.astype(np.float64)
+    eps = 1e-4
+    rng = np.random.default_rng(3)
+    idx = [tuple(rng.integers(0, 24, 3)) for _ in range(6)] + [(0, 0, 0), (23, 23, 23), (0, 12, 23)]
+    scale = np.abs(gu).max()
+    worst = 0.0
+    for i in idx:
+        up = u64.copy(); up[i] += eps
+        um = u64.copy(); um[i] -= eps
+        fd = (oracle.nlm3d_mse_f64(up, c64, rp, rs, h) - oracle.nlm3d_mse_f64(um, c64, rp, rs, h)) / (2 * eps)
+        err = abs(fd - gu[i]) / max(abs(fd), scale)
+        worst = max(worst, err)
+        assert err <= 1e-4, (i, fd, gu[i])
+    fdh = (oracle.nlm3d_mse_f64(u64, c64, rp, rs, h + eps) - oracle.nlm3d_mse_f64(u64, c64, rp, rs, h - eps)) / (2 * eps)
+    print(f"crop24: worst voxel rel err {worst:.2e}, dL/dh gpu {gh:.6e} fd {fdh:.6e}")
+    assert abs(gh - fdh) <= 1e-4 * abs(fdh)
+
+
+def test_fit_h_decreases_loss(dev):
+    """A few gradient steps on h (log-parameterised) lower the MSE vs clean."""
+    u, clean = _crop24()
+    ut = torch.from_numpy(u).to(dev)
+    ct = torch.from_numpy(clean).to(dev)
+    logh = torch.tensor(np.log(0.1), dtype=torch.float32, device=dev, requires_grad=True)
+    opt = torch.optim.Adam([logh], lr=0.05)
+    losses = []
+    for _ in range(15):
+        opt.zero_grad()
+        y = nlm3d(ut, torch.exp(logh), 1, 4)
+        loss = ((y - ct) ** 2).mean()
+        loss.backward()
+        opt.step()
+        losses.append(loss.item())
+    print("fit-h losses", losses[0], losses[-1], "h", float(torch.exp(logh)))
+    assert losses[-1] < losses[0]
