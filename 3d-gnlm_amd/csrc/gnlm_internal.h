@@ -11,12 +11,25 @@ namespace gnlm {
 // Exponent scale of the weight: w = exp(-(S/P)/h^2) = exp2(S * k).
 double weight_scale(int P, float h);
 
+constexpr size_t kLdsLimitBytes = 160 * 1024;  // LDS per workgroup on gfx950
+
 // ---- tiled 3D forward (nlm3d_fwd.hip) --------------------------------------
-// Returns true when the tiled kernel handles (r_p, r_s): r_p in 0..3 and the
-// LDS image of a tile with its (r_p + r_s) halo fits in 160 KiB.
+// Tile variants are enumerated in nlm3d_tile.h (Fwd3Variant).  A variant
+// handles (r_p, r_s) when r_p is in 0..3 and the LDS image of its tile with
+// the (r_p + r_s) halo fits in 160 KiB.
+size_t fwd3d_lds_bytes(int r_p, int r_s, int variant);
+bool fwd3d_variant_supported(int r_p, int r_s, int variant);
 bool fwd3d_tiled_supported(int r_p, int r_s);
+hipError_t fwd3d_tiled_launch_variant(const InDesc &in, float *out, float *Zout, int r_p, int r_s, float k,
+                                      int variant, hipStream_t s);
+// Launch with the variant the autotuner chose for (r_p, r_s) (or the default).
 hipError_t fwd3d_tiled_launch(const InDesc &in, float *out, float *Zout, int r_p, int r_s, float k,
                               hipStream_t s);
+
+// ---- autotuner (gnlm_tune.cpp) ---------------------------------------------
+// Variant for (r_p, r_s): the tuned one if gnlm_autotune ran, otherwise the
+// first supported variant.  Thread-safe.
+int fwd3d_pick_variant(int r_p, int r_s);
 
 // ---- tiled 2D forward (nlm2d_fwd.hip) --------------------------------------
 bool fwd2d_tiled_supported(int r_p, int r_s);

@@ -1,0 +1,136 @@
+// gnlm_tune.cpp -- autotuner for the tiled 3D forward.
+//
+// For a (patch, search) radius pair it times every tile variant that fits in
+// LDS on a synthetic probe volume on the caller's device and stream, keeps the
+// fastest, and records the choice in a process-wide table guarded by a mutex.
+// The per-volume launch and the batched persistent launch both read that
+// table, so a volume is always processed with the same tile shape; the tile
+// arithmetic is also independent of the tile shape (direct sums in a fixed
+// order), so every variant produces identical bits.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <utility>
+#include <vector>
+#include "gnlm_internal.h"
+#include "nlm3d_tile.h"
+
+namespace gnlm {
+namespace {
+
+std::mutex g_mu;
+std::map<std::pair<int, int>, int> g_table;  // (r_p, r_s) -> variant
+
+int first_supported(int r_p, int r_s)
+{
+    for (int v = 0; v < kNumFwd3Variants; ++v)
+        if (fwd3d_variant_supported(r_p, r_s, v)) return v;
+    return -1;
+}
+
+__global__ void probe_fill(float *p, long long n, unsigned seed)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        unsigned x = (unsigned)i * 2654435761u ^ seed;
+        x ^= x >> 15;
+        x *= 2246822519u;
+        x ^= x >> 13;
+        p[i] = (float)(x & 0xffffff) * (1.0f / 16777216.0f);
+    }
+}
+
+}  // namespace
+
+int fwd3d_pick_variant(int r_p, int r_s)
+{
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        auto it = g_table.find({r_p, r_s});
+        if (it != g_table.end()) return it->second;
+    }
+    return first_supported(r_p, r_s);
+}
+
+}  // namespace gnlm
+
+using namespace gnlm;
+
+extern "C" {
+
+int gnlm_autotune(int r_p, int r_s, int probe_n, void *stream, int *variant_out, float *ms_out)
+{
+    if (r_p < 0 || r_s < 0) return GNLM_EINVAL;
+    if (!fwd3d_tiled_supported(r_p, r_s)) return GNLM_EUNSUPPORTED;
+    if (probe_n <= 0) probe_n = 64;
+    if (probe_n > 512) return GNLM_EINVAL;
+    hipStream_t s = (hipStream_t)stream;
+    const long long n = (long long)probe_n * probe_n * probe_n;
+    float *buf = nullptr;
+    if (hipMalloc((void **)&buf, sizeof(float) * 2 * (size_t)n) != hipSuccess) return GNLM_ENOMEM;
+    hipLaunchKernelGGL(probe_fill, dim3(1024), dim3(256), 0, s, buf, n, 12345u);
+    InDesc in{buf, probe_n, probe_n, probe_n, 0, 0, 0};
+    const float k = (float)weight_scale((2 * r_p + 1) * (2 * r_p + 1) * (2 * r_p + 1), 0.1f);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int st = GNLM_OK;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) st = GNLM_EHIP;
+    int best = -1;
+    float best_ms = 0.0f;
+    for (int v = 0; v < kNumFwd3Variants && st == GNLM_OK; ++v) {
+        float ms = -1.0f;
+        if (fwd3d_variant_supported(r_p, r_s, v)) {
+            // one warm-up, then the median of three timed runs
+            hipError_t e = fwd3d_tiled_launch_variant(in, buf + n, nullptr, r_p, r_s, k, v, s);
+            std::vector<float> t;
+            for (int r = 0; r < 3 && e == hipSuccess; ++r) {
+                (void)hipEventRecord(e0, s);
+                e = fwd3d_tiled_launch_variant(in, buf + n, nullptr, r_p, r_s, k, v, s);
+                (void)hipEventRecord(e1, s);
+                if (e == hipSuccess) e = hipEventSynchronize(e1);
+                float m = 0.0f;
+                if (e == hipSuccess) e = hipEventElapsedTime(&m, e0, e1);
+                t.push_back(m);
+            }
+            if (e != hipSuccess) {
+                st = GNLM_EHIP;
+                break;
+            }
+            std::sort(t.begin(), t.end());
+            ms = t[t.size() / 2];
+            if (best < 0 || ms < best_ms) {
+                best = v;
+                best_ms = ms;
+            }
+        }
+        if (ms_out) ms_out[v] = ms;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(buf);
+    if (st != GNLM_OK) return st;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        g_table[{r_p, r_s}] = best;
+    }
+    if (variant_out) *variant_out = best;
+    return GNLM_OK;
+}
+
+int gnlm_tuned_variant(int r_p, int r_s)
+{
+    if (!fwd3d_tiled_supported(r_p, r_s)) return -1;
+    return fwd3d_pick_variant(r_p, r_s);
+}
+
+int gnlm_set_variant(int r_p, int r_s, int variant)
+{
+    if (!fwd3d_variant_supported(r_p, r_s, variant)) return GNLM_EUNSUPPORTED;
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_table[{r_p, r_s}] = variant;
+    return GNLM_OK;
+}
+
+int gnlm_num_variants(void) { return kNumFwd3Variants; }
+
+}  // extern "C"

@@ -1,0 +1,249 @@
+// nlm3d_batch.hip -- batched 3D NLM: many volumes of uneven size, each with
+// its own h and radii, in ONE persistent launch captured in a HIP graph.
+//
+// No counterpart in the reference (3DNLM/main.cu:7607-7680 processes one
+// compile-time volume per call).  Design:
+//   * at plan creation every volume is cut into tiles of the variant the
+//     autotuner picked for its (r_p, r_s); the tiles of all volumes form one
+//     work list ordered longest-first by cost (tile voxels x offsets), and the
+//     list and the per-volume descriptors are uploaded once;
+//   * the persistent kernel runs as many workgroups as fit on the device at
+//     once; each repeatedly takes the next tile index from a device counter
+//     (one atomicAdd by one thread, broadcast through LDS) and processes the
+//     tile with the same device function the per-volume launch uses, so the
+//     outputs are bit-identical to per-volume launches.  Every workgroup exits
+//     as soon as the counter passes the end of the list;
+//   * the graph holds [counter reset kernel -> persistent kernel]; replaying
+//     it re-runs the whole batch with one launch (no hipMemset in the graph:
+//     the counter is cleared by a kernel).
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
+#include <new>
+#include <vector>
+#include "gnlm_internal.h"
+#include "nlm3d_tile.h"
+
+namespace gnlm {
+namespace {
+
+struct BatchVol {
+    InDesc in;
+    float *out;
+    float *Zout;
+    int rp, rs;
+    float k;
+    int variant;
+    int ntx, nty;
+};
+
+struct BatchTile {
+    int vol;
+    int tx, ty, tz;
+};
+
+__global__ void reset_counter(int *c) { *c = 0; }
+
+template <int RP>
+__device__ __forceinline__ void run_tile_rp(const BatchVol &v, const BatchTile &t, float *lds)
+{
+    Fwd3Tile a;
+    a.in = v.in;
+    a.out = v.out;
+    a.Zout = v.Zout;
+    a.rs = v.rs;
+    a.k = v.k;
+    if (v.variant == kV16x16x8) {
+        a.x0 = t.tx * 16;
+        a.y0 = t.ty * 16;
+        a.z0 = t.tz * 8;
+        fwd3_tile<Fwd3Cfg<RP, 16, 16, 8>>(a, lds);
+    } else {
+        a.x0 = t.tx * 16;
+        a.y0 = t.ty * 16;
+        a.z0 = t.tz * 16;
+        fwd3_tile<Fwd3Cfg<RP, 16, 16, 16>>(a, lds);
+    }
+}
+
+__global__ __launch_bounds__(kFwdThreads) void nlm3d_fwd_persistent(const BatchVol *__restrict__ vols,
+                                                                     const BatchTile *__restrict__ tiles,
+                                                                     int ntiles, int *counter)
+{
+    extern __shared__ float lds[];
+    __shared__ int next;
+    for (;;) {
+        if (threadIdx.x == 0) next = atomicAdd(counter, 1);
+        __syncthreads();
+        const int t = next;
+        __syncthreads();  // all threads have read `next` and finished the previous tile's LDS use
+        if (t >= ntiles) break;
+        const BatchTile tl = tiles[t];
+        const BatchVol v = vols[tl.vol];
+        switch (v.rp) {
+        case 0: run_tile_rp<0>(v, tl, lds); break;
+        case 1: run_tile_rp<1>(v, tl, lds); break;
+        case 2: run_tile_rp<2>(v, tl, lds); break;
+        default: run_tile_rp<3>(v, tl, lds); break;
+        }
+    }
+}
+
+}  // namespace
+}  // namespace gnlm
+
+using namespace gnlm;
+
+struct gnlm_batch_plan {
+    int nvol = 0, ntiles = 0, nblocks = 0;
+    size_t lds = 0;
+    BatchVol *d_vols = nullptr;
+    BatchTile *d_tiles = nullptr;
+    int *d_counter = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+void plan_free(gnlm_batch_plan *p)
+{
+    if (!p) return;
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    if (p->d_vols) (void)hipFree(p->d_vols);
+    if (p->d_tiles) (void)hipFree(p->d_tiles);
+    if (p->d_counter) (void)hipFree(p->d_counter);
+    delete p;
+}
+
+int st_of(hipError_t e)
+{
+    if (e == hipSuccess) return GNLM_OK;
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return GNLM_ENOMEM;
+    return GNLM_EHIP;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gnlm_batch_plan_create(const gnlm_batch_volume *vols, int n, gnlm_batch_plan **out)
+{
+    if (!out) return GNLM_EINVAL;
+    *out = nullptr;
+    if (!vols || n < 1) return GNLM_EINVAL;
+    std::vector<BatchVol> hv((size_t)n);
+    std::vector<BatchTile> ht;
+    std::vector<double> cost;
+    size_t lds = 0;
+    for (int i = 0; i < n; ++i) {
+        const gnlm_batch_volume &v = vols[i];
+        if (!v.d_in || !v.d_out || v.nx < 1 || v.ny < 1 || v.nz < 1 || v.r_p < 0 || v.r_s < 0 ||
+            !(v.h > 0.0f) || !std::isfinite(v.h))
+            return GNLM_EINVAL;
+        if (v.r_p > 3 || !fwd3d_tiled_supported(v.r_p, v.r_s)) return GNLM_EUNSUPPORTED;
+        BatchVol &b = hv[(size_t)i];
+        b.in = InDesc{v.d_in, v.nx, v.ny, v.nz, 0, 0, 0};
+        b.out = v.d_out;
+        b.Zout = v.d_Z;
+        b.rp = v.r_p;
+        b.rs = v.r_s;
+        const int P = (2 * v.r_p + 1) * (2 * v.r_p + 1) * (2 * v.r_p + 1);
+        b.k = (float)weight_scale(P, v.h);
+        b.variant = fwd3d_pick_variant(v.r_p, v.r_s);
+        const int tz = fwd3_tz(b.variant);
+        b.ntx = (v.nx + 15) / 16;
+        b.nty = (v.ny + 15) / 16;
+        const int ntz = (v.nz + tz - 1) / tz;
+        lds = std::max(lds, fwd3d_lds_bytes(v.r_p, v.r_s, b.variant));
+        const double c = 256.0 * tz * std::pow(2.0 * v.r_s + 1.0, 3);
+        for (int z = 0; z < ntz; ++z)
+            for (int y = 0; y < b.nty; ++y)
+                for (int x = 0; x < b.ntx; ++x) {
+                    ht.push_back(BatchTile{i, x, y, z});
+                    cost.push_back(c);
+                }
+    }
+    // longest first; ties keep volume/tile order (stable)
+    std::vector<size_t> order(ht.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
+    std::vector<BatchTile> sorted(ht.size());
+    for (size_t i = 0; i < order.size(); ++i) sorted[i] = ht[order[i]];
+
+    gnlm_batch_plan *p = new (std::nothrow) gnlm_batch_plan();
+    if (!p) return GNLM_ENOMEM;
+    p->nvol = n;
+    p->ntiles = (int)sorted.size();
+    p->lds = lds;
+    hipError_t e = hipFuncSetAttribute((const void *)nlm3d_fwd_persistent,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    int dev = 0, ncu = 0, occ = 0;
+    if (e == hipSuccess) e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void *)nlm3d_fwd_persistent, kFwdThreads, lds);
+    if (e == hipSuccess && occ < 1) e = hipErrorInvalidConfiguration;
+    if (e == hipSuccess) p->nblocks = std::min(ncu * occ, p->ntiles);
+    if (e == hipSuccess) e = hipMalloc((void **)&p->d_vols, sizeof(BatchVol) * hv.size());
+    if (e == hipSuccess) e = hipMalloc((void **)&p->d_tiles, sizeof(BatchTile) * sorted.size());
+    if (e == hipSuccess) e = hipMalloc((void **)&p->d_counter, sizeof(int));
+    if (e == hipSuccess) e = hipMemcpy(p->d_vols, hv.data(), sizeof(BatchVol) * hv.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(p->d_tiles, sorted.data(), sizeof(BatchTile) * sorted.size(), hipMemcpyHostToDevice);
+    // capture [reset counter -> persistent kernel] on a private stream
+    hipStream_t cs = nullptr;
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, cs, p->d_counter);
+        hipLaunchKernelGGL(nlm3d_fwd_persistent, dim3(p->nblocks), dim3(kFwdThreads), lds, cs,
+                           (const BatchVol *)p->d_vols, (const BatchTile *)p->d_tiles, p->ntiles, p->d_counter);
+        hipError_t e2 = hipStreamEndCapture(cs, &p->graph);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = e2;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
+    if (cs) (void)hipStreamDestroy(cs);
+    if (e != hipSuccess) {
+        plan_free(p);
+        return st_of(e);
+    }
+    *out = p;
+    return GNLM_OK;
+}
+
+int gnlm_batch_run(gnlm_batch_plan *plan, void *stream)
+{
+    if (!plan || !plan->exec) return GNLM_ESTATE;
+    return st_of(hipGraphLaunch(plan->exec, (hipStream_t)stream));
+}
+
+int gnlm_batch_run_eager(gnlm_batch_plan *plan, void *stream)
+{
+    if (!plan || !plan->d_counter) return GNLM_ESTATE;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, s, plan->d_counter);
+    hipLaunchKernelGGL(nlm3d_fwd_persistent, dim3(plan->nblocks), dim3(kFwdThreads), plan->lds, s,
+                       (const BatchVol *)plan->d_vols, (const BatchTile *)plan->d_tiles, plan->ntiles,
+                       plan->d_counter);
+    return st_of(hipGetLastError());
+}
+
+int gnlm_batch_plan_info(const gnlm_batch_plan *plan, int *ntiles, int *nblocks)
+{
+    if (!plan) return GNLM_EINVAL;
+    if (ntiles) *ntiles = plan->ntiles;
+    if (nblocks) *nblocks = plan->nblocks;
+    return GNLM_OK;
+}
+
+int gnlm_batch_plan_destroy(gnlm_batch_plan *plan)
+{
+    if (!plan) return GNLM_EINVAL;
+    plan_free(plan);
+    return GNLM_OK;
+}
+
+}  // extern "C"

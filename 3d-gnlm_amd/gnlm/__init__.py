@@ -66,6 +66,80 @@ def _extend(L):
     if hasattr(L, "gnlm3d_backward"):
         L.gnlm3d_backward.argtypes = [vp, vp, vp, vp, i, i, i, i, i, f, vp, vp, vp]
         L.gnlm3d_backward.restype = i
+    L.gnlm_batch_plan_create.argtypes = [ctypes.POINTER(BatchVolume), i, ctypes.POINTER(vp)]
+    L.gnlm_batch_run.argtypes = [vp, vp]
+    L.gnlm_batch_run_eager.argtypes = [vp, vp]
+    L.gnlm_batch_plan_info.argtypes = [vp, ctypes.POINTER(i), ctypes.POINTER(i)]
+    L.gnlm_batch_plan_destroy.argtypes = [vp]
+    L.gnlm_autotune.argtypes = [i, i, i, vp, ctypes.POINTER(i), ctypes.POINTER(f)]
+    L.gnlm_tuned_variant.argtypes = [i, i]
+    L.gnlm_set_variant.argtypes = [i, i, i]
+    for name in ("gnlm_batch_plan_create", "gnlm_batch_run", "gnlm_batch_run_eager", "gnlm_batch_plan_info",
+                 "gnlm_batch_plan_destroy", "gnlm_autotune", "gnlm_tuned_variant", "gnlm_set_variant",
+                 "gnlm_num_variants"):
+        getattr(L, name).restype = i
+
+
+class BatchVolume(ctypes.Structure):
+    """Mirror of gnlm_batch_volume (include/gnlm.h)."""
+    _fields_ = [("d_in", ctypes.c_void_p), ("d_out", ctypes.c_void_p), ("d_Z", ctypes.c_void_p),
+                ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+                ("r_p", ctypes.c_int), ("r_s", ctypes.c_int), ("h", ctypes.c_float)]
+
+
+class BatchPlan:
+    """Persistent, graph-captured forward over many volumes.
+
+    vols: list of (u, out, r_p, r_s, h) with u/out float32 CUDA tensors
+    [nz, ny, nx].  The tensors must stay alive while the plan is used."""
+
+    def __init__(self, vols):
+        arr = (BatchVolume * len(vols))()
+        self._keep = []
+        for i, (u, out, rp, rs, h) in enumerate(vols):
+            nz, ny, nx = u.shape
+            arr[i] = BatchVolume(_dev_f32(u, "u").value, _dev_f32(out, "out").value, None, nx, ny, nz,
+                                 int(rp), int(rs), float(h))
+            self._keep += [u, out]
+        self._plan = ctypes.c_void_p()
+        check(lib().gnlm_batch_plan_create(arr, len(vols), ctypes.byref(self._plan)), "gnlm_batch_plan_create")
+        self._dev = vols[0][0]
+
+    def run(self, eager=False):
+        fn = lib().gnlm_batch_run_eager if eager else lib().gnlm_batch_run
+        check(fn(self._plan, _stream(self._dev)), "gnlm_batch_run")
+
+    def info(self):
+        nt, nb = ctypes.c_int(), ctypes.c_int()
+        check(lib().gnlm_batch_plan_info(self._plan, ctypes.byref(nt), ctypes.byref(nb)), "info")
+        return nt.value, nb.value
+
+    def close(self):
+        if self._plan:
+            lib().gnlm_batch_plan_destroy(self._plan)
+            self._plan = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def autotune(r_p, r_s, probe_n=64, device_tensor=None):
+    """Time the tile variants for (r_p, r_s); returns (chosen variant, [ms per variant])."""
+    import torch
+    L = lib()
+    nv = L.gnlm_num_variants()
+    ms = (ctypes.c_float * nv)()
+    v = ctypes.c_int(-1)
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    check(L.gnlm_autotune(int(r_p), int(r_s), int(probe_n), stream, ctypes.byref(v), ms), "gnlm_autotune")
+    return v.value, list(ms)
+
+
+def set_variant(r_p, r_s, variant):
+    check(lib().gnlm_set_variant(int(r_p), int(r_s), int(variant)), "gnlm_set_variant")
 
 
 def check(status, where=""):

@@ -95,6 +95,49 @@ int gnlm2d_forward_ex(const float *d_in, int in_pad, long long in_sy, float *d_o
 int gnlm3d_backward(const float *d_in, const float *d_out, const float *d_Z, const float *d_gout, int nx,
                     int ny, int nz, int r_p, int r_s, float h, float *d_gin, float *d_gh, void *stream);
 
+/* ---- batched persistent forward --------------------------------------
+ * No counterpart in the reference (its launcher processes one compile-time
+ * volume per call, 3DNLM/main.cu:7607-7680).  Many volumes of uneven size,
+ * each with its own h and radii, run in ONE persistent launch replayed from a
+ * HIP graph; workgroups pull tiles from a device work queue (longest first).
+ * Outputs are bit-identical to per-volume gnlm3d_forward calls.
+ * The plan keeps the descriptors (device pointers included): buffers must
+ * stay allocated while the plan is used.  r_p <= 3 and the tiled path's
+ * radius limits apply (GNLM_EUNSUPPORTED otherwise). */
+typedef struct gnlm_batch_volume {
+    const float *d_in;  /* unpadded input, x fastest        */
+    float *d_out;       /* output, same layout              */
+    float *d_Z;         /* nullable: normaliser per voxel   */
+    int nx, ny, nz;
+    int r_p, r_s;
+    float h;
+} gnlm_batch_volume;
+
+typedef struct gnlm_batch_plan gnlm_batch_plan;
+
+/* Builds the tile list, uploads it, and captures the graph (on the current
+ * device).  Tile shapes come from the autotuner table (gnlm_autotune). */
+int gnlm_batch_plan_create(const gnlm_batch_volume *vols, int n, gnlm_batch_plan **plan);
+/* Replays the captured graph on `stream` (one graph launch). */
+int gnlm_batch_run(gnlm_batch_plan *plan, void *stream);
+/* Same work as gnlm_batch_run, launched without the graph. */
+int gnlm_batch_run_eager(gnlm_batch_plan *plan, void *stream);
+int gnlm_batch_plan_info(const gnlm_batch_plan *plan, int *ntiles, int *nblocks);
+int gnlm_batch_plan_destroy(gnlm_batch_plan *plan);
+
+/* ---- autotuner --------------------------------------------------------
+ * No counterpart in the reference (tile sizes are #defines there,
+ * 3DNLM/main.cu:15-21).  Times every tile variant of the tiled 3D forward
+ * that fits for (r_p, r_s) on a probe_n^3 synthetic volume (0 = 64) on
+ * `stream`, records the fastest in a process-wide, mutex-guarded table used
+ * by gnlm3d_forward and the batch plans, and returns it in *variant (nullable);
+ * ms[v] (nullable, gnlm_num_variants() entries) receives each variant's time
+ * (-1 if it does not fit).  All variants produce identical bits. */
+int gnlm_autotune(int r_p, int r_s, int probe_n, void *stream, int *variant, float *ms);
+int gnlm_tuned_variant(int r_p, int r_s);            /* -1 if unsupported */
+int gnlm_set_variant(int r_p, int r_s, int variant);  /* manual override    */
+int gnlm_num_variants(void);
+
 /* uint8 -> float32 element conversion (n elements), for 8-bit inputs such as
  * the reference's (3DNLM/main.cu:7620 copies unsigned char volumes). */
 int gnlm_u8_to_f32(const unsigned char *d_in, float *d_out, long long n, void *stream);

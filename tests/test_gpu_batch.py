@@ -1,0 +1,84 @@
+"""Batched persistent forward and the autotuner.
+
+Bar: bit-identical outputs between the graph-replayed persistent launch, its
+eager launch, and per-volume gnlm3d_forward calls; every tile variant gives
+identical bits."""
+import numpy as np
+import pytest
+import torch
+
+import gnlm
+import oracle
+from nlm_metrics import max_rel, report
+
+pytestmark = pytest.mark.gpu
+
+
+def _volumes(dev, n, seed, dims=(16, 24, 32, 40, 48)):
+    rng = np.random.default_rng(seed)
+    vols = []
+    for i in range(n):
+        nx, ny, nz = (int(rng.choice(dims)) for _ in range(3))
+        sigma = float(rng.uniform(0.02, 0.10))
+        u, _ = gnlm.phantom3d(nx, ny, nz, n_ellipsoids=4, axis_lo=3, axis_hi=10, sigma=sigma, seed=seed * 100 + i)
+        rp = int(rng.choice([1, 2]))
+        rs = int(rng.choice([3, 5]))
+        vols.append((torch.from_numpy(u).to(dev), rp, rs, sigma))
+    return vols
+
+
+def test_batch_graph_equals_per_volume(dev):
+    vols = _volumes(dev, 10, 1)
+    outs = [torch.empty_like(u) for u, *_ in vols]
+    plan = gnlm.BatchPlan([(u, o, rp, rs, h) for (u, rp, rs, h), o in zip(vols, outs)])
+    ntiles, nblocks = plan.info()
+    print(f"batch: {len(vols)} volumes, {ntiles} tiles, {nblocks} workgroups", flush=True)
+    refs = [gnlm.nlm3d(u, rp, rs, h) for u, rp, rs, h in vols]
+    torch.cuda.synchronize()
+    for rep in range(3):
+        for o in outs:
+            o.fill_(-1.0)           # eager work between replays
+        torch.cuda.synchronize()
+        plan.run()
+        torch.cuda.synchronize()
+        for o, r in zip(outs, refs):
+            assert torch.equal(o, r), f"replay {rep} differs"
+    for o in outs:
+        o.zero_()
+    plan.run(eager=True)
+    torch.cuda.synchronize()
+    for o, r in zip(outs, refs):
+        assert torch.equal(o, r)
+    plan.close()
+
+
+def test_batch_matches_oracle(dev):
+    vols = _volumes(dev, 4, 2, dims=(16, 24))
+    outs = [torch.empty_like(u) for u, *_ in vols]
+    plan = gnlm.BatchPlan([(u, o, rp, rs, h) for (u, rp, rs, h), o in zip(vols, outs)])
+    plan.run()
+    torch.cuda.synchronize()
+    for (u, rp, rs, h), o in zip(vols, outs):
+        ref = oracle.nlm3d_separable_f64(u.cpu().numpy(), rp, rs, h)
+        got = o.cpu().numpy()
+        assert max_rel(got, ref, floor=1e-3 * (ref.max() - ref.min())) <= 1e-5, report(got, ref)
+    plan.close()
+
+
+@pytest.mark.parametrize("rp,rs", [(1, 3), (2, 5), (1, 5)])
+def test_autotune_and_variants_identical(dev, rp, rs):
+    v, ms = gnlm.autotune(rp, rs, probe_n=48)
+    print(f"autotune rp={rp} rs={rs}: variant {v}, ms {ms}")
+    assert v >= 0 and ms[v] > 0
+    rng = np.random.default_rng(rp * 10 + rs)
+    u = torch.from_numpy(rng.random((21, 35, 37), dtype=np.float32)).to(dev)
+    outs = []
+    for var in range(len(ms)):
+        if ms[var] < 0:
+            continue
+        gnlm.set_variant(rp, rs, var)
+        outs.append(gnlm.nlm3d(u, rp, rs, 0.2))
+    torch.cuda.synchronize()
+    gnlm.set_variant(rp, rs, v)
+    for o in outs[1:]:
+        assert torch.equal(o, outs[0])
