@@ -107,5 +107,5 @@ def test_fit_h_decreases_loss(dev):
         loss.backward()
         opt.step()
         losses.append(loss.item())
-    print("fit-h losses", losses[0], losses[-1], "h", float(torch.exp(logh)))
+    print("fit-h losses", losses[0], losses[-1], "h", float(torch.exp(logh.detach())))
     assert losses[-1] < losses[0]

@@ -5,7 +5,10 @@ Tolerances (FP32), stated per comparison:
     |a-b|/max(|b|,1e-6) <= 1e-5 against both the oracle's FP32 path and FP64.
   * phantom workloads (background 0.05 + noise, so a few denoised voxels sit
     near 0 and the plain relative error is ill-conditioned there): relative
-    error floored at 1e-3 of the output range, <= 1e-5, against both.
+    error floored at 1e-3 of the output range, <= 1e-5 against the FP32
+    path; against FP64 <= 1e-5 for patch 3^3 and <= 1e-4 for patch 5^3 at
+    256^3, where the FP32 patch sums themselves (CPU and GPU alike) are
+    ~3.5e-5 from FP64.
   * tiled vs generic path, padded vs reflect input: bit-identical.
 """
 import numpy as np
@@ -18,7 +21,8 @@ from nlm_metrics import max_rel, max_rel_fl, report
 
 pytestmark = pytest.mark.gpu
 
-TOL = 1e-5
+TOL = 1e-5       # vs the CPU FP32 path (the parity bar)
+TOL_FP64 = 1e-4  # vs FP64 where FP32 rounding of 125-term patch sums dominates
 
 
 def _run3(dev, u, rp, rs, h, path=gnlm.PATH_AUTO):
@@ -122,7 +126,8 @@ def test_3d_config3_slabs(dev):
         ref64 = oracle.nlm3d_separable_f64(u, 2, 5, 0.05, z0=z0, z1=z0 + 3)
         print("cfg3 slab", z0, "vs fp32", report(y[z0:z0 + 3], ref32), "vs fp64", report(y[z0:z0 + 3], ref64))
         assert max_rel_fl(y[z0:z0 + 3], ref32) <= TOL
-        assert max_rel_fl(y[z0:z0 + 3], ref64) <= TOL
+        # FP32 arithmetic itself (CPU and GPU alike) sits ~3.5e-5 from FP64 here
+        assert max_rel_fl(y[z0:z0 + 3], ref64) <= TOL_FP64
 
 
 CASES2 = [
