@@ -4,9 +4,10 @@
 // LDS on a synthetic probe volume on the caller's device and stream, keeps the
 // fastest, and records the choice in a process-wide table guarded by a mutex.
 // The per-volume launch and the batched persistent launch both read that
-// table, so a volume is always processed with the same tile shape; the tile
-// arithmetic is also independent of the tile shape (direct sums in a fixed
-// order), so every variant produces identical bits.
+// table, so a volume is always processed with the same tile shape, and the
+// per-volume and batched launches of a volume give identical bits.  The
+// direct-sum variants are also bit-identical to each other; the slab variant
+// (sliding sums) agrees with them to rounding.
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <map>
@@ -22,8 +23,11 @@ namespace {
 std::mutex g_mu;
 std::map<std::pair<int, int>, int> g_table;  // (r_p, r_s) -> variant
 
+// Default when (r_p, r_s) has not been tuned: the slab variant where it
+// fits (fastest on the measured configurations), else the direct-sum tiles.
 int first_supported(int r_p, int r_s)
 {
+    if (fwd3d_variant_supported(r_p, r_s, kVSlab)) return kVSlab;
     for (int v = 0; v < kNumFwd3Variants; ++v)
         if (fwd3d_variant_supported(r_p, r_s, v)) return v;
     return -1;
@@ -132,5 +136,11 @@ int gnlm_set_variant(int r_p, int r_s, int variant)
 }
 
 int gnlm_num_variants(void) { return kNumFwd3Variants; }
+
+void gnlm_reset_tuning(void)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_table.clear();
+}
 
 }  // extern "C"

@@ -33,6 +33,7 @@ struct BatchVol {
     float *Zout;
     int rp, rs;
     float k;
+    float sigma;
     int variant;
     int ntx, nty;
 };
@@ -53,7 +54,14 @@ __device__ __forceinline__ void run_tile_rp(const BatchVol &v, const BatchTile &
     a.Zout = v.Zout;
     a.rs = v.rs;
     a.k = v.k;
-    if (v.variant == kV16x16x8) {
+    a.sigma = v.sigma;
+    if (v.variant == kVSlab) {
+        using C = SlabCfg<RP>;
+        a.x0 = t.tx * C::TX;
+        a.y0 = t.ty * C::TY;
+        a.z0 = t.tz * C::TZ;
+        fwd3_slab_tile<RP>(a, lds, v.sigma);
+    } else if (v.variant == kV16x16x8) {
         a.x0 = t.tx * 16;
         a.y0 = t.ty * 16;
         a.z0 = t.tz * 8;
@@ -151,13 +159,14 @@ int gnlm_batch_plan_create(const gnlm_batch_volume *vols, int n, gnlm_batch_plan
         b.rs = v.r_s;
         const int P = (2 * v.r_p + 1) * (2 * v.r_p + 1) * (2 * v.r_p + 1);
         b.k = (float)weight_scale(P, v.h);
+        b.sigma = std::sqrt(-b.k);
         b.variant = fwd3d_pick_variant(v.r_p, v.r_s);
-        const int tz = fwd3_tz(b.variant);
-        b.ntx = (v.nx + 15) / 16;
-        b.nty = (v.ny + 15) / 16;
+        const int tx = fwd3_tx(b.variant, v.r_p), ty = fwd3_ty(b.variant, v.r_p), tz = fwd3_tz(b.variant, v.r_p);
+        b.ntx = (v.nx + tx - 1) / tx;
+        b.nty = (v.ny + ty - 1) / ty;
         const int ntz = (v.nz + tz - 1) / tz;
         lds = std::max(lds, fwd3d_lds_bytes(v.r_p, v.r_s, b.variant));
-        const double c = 256.0 * tz * std::pow(2.0 * v.r_s + 1.0, 3);
+        const double c = (double)tx * ty * tz * std::pow(2.0 * v.r_s + 1.0, 3);
         for (int z = 0; z < ntz; ++z)
             for (int y = 0; y < b.nty; ++y)
                 for (int x = 0; x < b.ntx; ++x) {

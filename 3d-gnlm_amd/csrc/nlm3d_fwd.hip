@@ -1,5 +1,6 @@
 // nlm3d_fwd.hip -- per-volume tiled 3D NLM forward (one launch per volume).
 // The tile computation itself lives in nlm3d_tile.h.
+#include <cmath>
 #include "gnlm_internal.h"
 #include "nlm3d_tile.h"
 
@@ -15,6 +16,48 @@ struct Fwd3Args {
     int ntx, nty, ntz;
 };
 
+template <int RP>
+__global__ __launch_bounds__(kFwdThreads) void nlm3d_fwd_slab(Fwd3Args a, float sigma)
+{
+    extern __shared__ float lds[];
+    using C = SlabCfg<RP>;
+    const int nblocks = a.ntx * a.nty * a.ntz;
+    const int t = xcd_tile(blockIdx.x, nblocks);
+    Fwd3Tile tl;
+    tl.in = a.in;
+    tl.out = a.out;
+    tl.Zout = a.Zout;
+    tl.rs = a.rs;
+    tl.k = a.k;
+    tl.sigma = sigma;
+    tl.x0 = (t % a.ntx) * C::TX;
+    tl.y0 = ((t / a.ntx) % a.nty) * C::TY;
+    tl.z0 = (t / (a.ntx * a.nty)) * C::TZ;
+    fwd3_slab_tile<RP>(tl, lds, sigma);
+}
+
+template <int RP>
+hipError_t launch_slab(const InDesc &in, float *out, float *Zout, int rs, float k, hipStream_t s)
+{
+    using C = SlabCfg<RP>;
+    Fwd3Args a;
+    a.in = in;
+    a.out = out;
+    a.Zout = Zout;
+    a.rs = rs;
+    a.k = k;
+    a.ntx = (in.nx + C::TX - 1) / C::TX;
+    a.nty = (in.ny + C::TY - 1) / C::TY;
+    a.ntz = (in.nz + C::TZ - 1) / C::TZ;
+    const size_t lds = slab_lds_floats(RP, rs) * sizeof(float);
+    hipError_t e = hipFuncSetAttribute((const void *)nlm3d_fwd_slab<RP>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    const float sigma = std::sqrt(-k);
+    hipLaunchKernelGGL(nlm3d_fwd_slab<RP>, dim3(a.ntx * a.nty * a.ntz), dim3(kFwdThreads), lds, s, a, sigma);
+    return hipGetLastError();
+}
+
 template <class C>
 __global__ __launch_bounds__(kFwdThreads) void nlm3d_fwd_tiled(Fwd3Args a)
 {
@@ -27,6 +70,7 @@ __global__ __launch_bounds__(kFwdThreads) void nlm3d_fwd_tiled(Fwd3Args a)
     tl.Zout = a.Zout;
     tl.rs = a.rs;
     tl.k = a.k;
+    tl.sigma = 0.0f;
     tl.x0 = (t % a.ntx) * C::TX;
     tl.y0 = ((t / a.ntx) % a.nty) * C::TY;
     tl.z0 = (t / (a.ntx * a.nty)) * C::TZ;
@@ -56,6 +100,7 @@ hipError_t launch_cfg(const InDesc &in, float *out, float *Zout, int rs, float k
 template <int RP>
 hipError_t launch_rp(const InDesc &in, float *out, float *Zout, int rs, float k, int variant, hipStream_t s)
 {
+    if (variant == kVSlab) return launch_slab<RP>(in, out, Zout, rs, k, s);
     if (variant == kV16x16x8) return launch_cfg<Fwd3Cfg<RP, 16, 16, 8>>(in, out, Zout, rs, k, s);
     return launch_cfg<Fwd3Cfg<RP, 16, 16, 16>>(in, out, Zout, rs, k, s);
 }
@@ -63,6 +108,7 @@ hipError_t launch_rp(const InDesc &in, float *out, float *Zout, int rs, float k,
 template <int RP>
 size_t lds_rp(int rs, int variant)
 {
+    if (variant == kVSlab) return slab_lds_floats(RP, rs) * sizeof(float);
     if (variant == kV16x16x8) return fwd3_lds_floats<Fwd3Cfg<RP, 16, 16, 8>>(rs) * sizeof(float);
     return fwd3_lds_floats<Fwd3Cfg<RP, 16, 16, 16>>(rs) * sizeof(float);
 }

@@ -63,7 +63,8 @@ struct Fwd3Tile {
     float *out;
     float *Zout;
     int rs;
-    float k;
+    float k;      // exp2 scale (direct-sum variants): w = exp2(S * k)
+    float sigma;  // input pre-scale (slab variant): sigma = sqrt(-k)
     int x0, y0, z0;
 };
 
@@ -186,9 +187,209 @@ __device__ __forceinline__ void fwd3_tile(const Fwd3Tile &a, float *lds)
     }
 }
 
-// Tile variants: (TX, TY, TZ).  TX * TY = 256 always.
-enum Fwd3Variant { kV16x16x16 = 0, kV16x16x8 = 1, kNumFwd3Variants = 2 };
+// ===========================================================================
+// Variant "slab": DPP y sums, sliding x and z sums, per-dz input slabs.
+//
+// Tile 16 x (16 - 2 r_p) x (32 - 2 r_p) voxels, 256 threads.
+//   * The input is pre-scaled by sigma = sqrt(log2(e) / (P h^2)) while it is
+//     staged, so S' = sigma^2 S and w = exp2(-S') needs no multiply; the
+//     candidate values carry the same factor and the final ratio divides it
+//     out (y = Y / (sigma Z)).
+//   * Offsets are visited dz-major.  For each dz only the 32 input planes the
+//     shifted patches touch are staged in LDS (a "slab"), which leaves room
+//     for a 16 x 12 x 28 tile with a 7-voxel halo in 160 KiB.  The centre
+//     samples of each phase-1 row are loaded once into registers.
+//   * phase 1: thread <-> row (y_ext, z_ext), 16 y_ext x 32 z_ext rows, two
+//     per thread, z_ext fastest across lanes (conflict-free LDS rows).  x
+//     sums slide along the row in registers (diff, fma in, fma out).
+//   * phase 2: thread <-> (x, y_ext) with y_ext = lane % 16, so one 16-lane
+//     DPP row holds a whole extended column: the y box sum is formed with
+//     row shifts (3 VALU ops for a 5-wide window) and reads no LDS; its
+//     input column of 32 planes comes in with eight 128-bit LDS reads.
+//   * phase 3: z sums slide over the 32 planes in registers; exp2, Z, Y.
+//   The y_ext lanes of the halo (r_p at each end of a row) carry no output.
+// Sliding sums make rounding depend on where a tile starts, so this variant
+// is deterministic and identical between the per-volume and the batched
+// launch (same tiles), but not bit-identical to the direct-sum variants;
+// its parity with the oracle is checked to tolerance.
+// ===========================================================================
 
-__host__ __device__ constexpr int fwd3_tz(int v) { return v == kV16x16x8 ? 8 : 16; }
+// DPP row shift (within 16-lane rows, zero fill at the row ends).
+template <int CTRL>
+__device__ __forceinline__ float dpp_row(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, true));
+}
+constexpr int kRowShl1 = 0x101, kRowShl2 = 0x102, kRowShl3 = 0x103;
+constexpr int kRowShr1 = 0x111, kRowShr2 = 0x112, kRowShr3 = 0x113;
+
+// Sum of v over the 2RP+1 lanes centred on this one within its 16-lane row.
+// Written so the result does not depend on which way row_shl/row_shr move
+// data: every term and its mirror are both included.
+template <int RP>
+__device__ __forceinline__ float dpp_box(float v)
+{
+    if constexpr (RP == 0) {
+        return v;
+    } else if constexpr (RP == 1) {
+        return (v + dpp_row<kRowShl1>(v)) + dpp_row<kRowShr1>(v);
+    } else if constexpr (RP == 2) {
+        const float p = v + dpp_row<kRowShl1>(v);   // v(i) + v(i+s)
+        const float q = p + dpp_row<kRowShr2>(p);   // + v(i-2s) + v(i-s)
+        return q + dpp_row<kRowShl2>(v);            // + v(i+2s)
+    } else {
+        float s = v;
+        s = s + dpp_row<kRowShl1>(v);
+        s = s + dpp_row<kRowShr1>(v);
+        s = s + dpp_row<kRowShl2>(v);
+        s = s + dpp_row<kRowShr2>(v);
+        s = s + dpp_row<kRowShl3>(v);
+        s = s + dpp_row<kRowShr3>(v);
+        return s;
+    }
+}
+
+template <int RP_>
+struct SlabCfg {
+    static constexpr int RP = RP_;
+    static constexpr int TX = 16;
+    static constexpr int EY = 16;            // one DPP row of lanes
+    static constexpr int TY = EY - 2 * RP;
+    static constexpr int EZ = 32;            // planes per slab / per phase-1 column
+    static constexpr int TZ = EZ - 2 * RP;
+    static constexpr int EX = TX + 2 * RP;   // samples per phase-1 row
+    static constexpr int SYP = 36;           // y pitch of s_x (== 4 mod 32: b128 reads conflict-free)
+    static constexpr int SXP = EY * SYP;     // x pitch of s_x
+    static constexpr int SXN = TX * SXP;     // floats in the s_x buffer
+    static constexpr int ROWS = EY * EZ / kFwdThreads;  // phase-1 rows per thread (2)
+};
+
+__host__ __device__ constexpr size_t slab_lds_floats(int rp, int rs)
+{
+    return (size_t)(((16 + 2 * (rp + rs)) * (16 + 2 * rs)) | 1) * 32 + (size_t)16 * 16 * 36;
+}
+
+template <int RP>
+__device__ __forceinline__ void fwd3_slab_tile(const Fwd3Tile &a, float *lds, float sigma)
+{
+    using C = SlabCfg<RP>;
+    constexpr int NT = kFwdThreads;
+    const int rs = a.rs;
+    const int LX = C::TX + 2 * (RP + rs);
+    const int LY = C::EY + 2 * rs;
+    const int PP = (LX * LY) | 1;            // odd plane pitch: conflict-free phase-1 rows
+    float *SL = lds;                          // slab: 32 planes x LY x LX (pitch PP)
+    float *SX = lds + PP * C::EZ;             // s_x[x][y_ext][z_ext]
+    const int x0 = a.x0, y0 = a.y0, z0 = a.z0;
+    const int tid = threadIdx.x;
+
+    // phase-1 rows: z_ext = tid % 32, y_ext = tid / 32 + 8 q
+    const int r_z = tid % C::EZ;
+    float cen[C::ROWS][C::EX];
+#pragma unroll
+    for (int q = 0; q < C::ROWS; ++q) {
+        const int r_y = tid / C::EZ + 8 * q;
+#pragma unroll
+        for (int i = 0; i < C::EX; ++i)
+            cen[q][i] = sigma * load_in(a.in, x0 - RP + i, y0 - RP + r_y, z0 - RP + r_z);
+    }
+    // phase-2/3 ownership: column x = tid / 16, y_ext = tid % 16
+    const int cx = tid / C::EY;
+    const int cye = tid % C::EY;
+    float Zacc[C::TZ], Yacc[C::TZ];
+#pragma unroll
+    for (int t = 0; t < C::TZ; ++t) {
+        Zacc[t] = 0.0f;
+        Yacc[t] = 0.0f;
+    }
+
+    for (int dz = -rs; dz <= rs; ++dz) {
+        // ---- stage the slab for this dz: planes z0 - RP + dz + [0, 32) ----
+        __syncthreads();  // previous slab no longer read
+        const int xs = x0 - RP - rs, ys = y0 - RP - rs, zs = z0 - RP + dz;
+        for (int i = tid; i < LX * LY * C::EZ; i += NT) {
+            const int lx = i % LX;
+            const int r = i / LX;
+            const int ly = r % LY;
+            const int lz = r / LY;
+            SL[lx + LX * ly + PP * lz] = sigma * load_in(a.in, xs + lx, ys + ly, zs + lz);
+        }
+        __syncthreads();
+
+        for (int dy = -rs; dy <= rs; ++dy)
+            for (int dx = -rs; dx <= rs; ++dx) {
+                // ---- phase 1: sliding x sums of squared differences ----
+#pragma unroll
+                for (int q = 0; q < C::ROWS; ++q) {
+                    const int r_y = tid / C::EZ + 8 * q;
+                    const float *sh = SL + (rs + dx) + LX * (r_y + dy + rs) + PP * r_z;
+                    float e[C::EX];
+#pragma unroll
+                    for (int i = 0; i < C::EX; ++i) e[i] = cen[q][i] - sh[i];
+                    float s = e[0] * e[0];
+#pragma unroll
+                    for (int i = 1; i <= 2 * RP; ++i) s = fmaf(e[i], e[i], s);
+                    float *dst = SX + C::SYP * r_y + r_z;
+                    dst[0] = s;
+#pragma unroll
+                    for (int j = 1; j < C::TX; ++j) {
+                        s = fmaf(e[j + 2 * RP], e[j + 2 * RP], s);
+                        s = fmaf(-e[j - 1], e[j - 1], s);
+                        dst[j * C::SXP] = s;
+                    }
+                }
+                __syncthreads();
+
+                // ---- phase 2: column of s_x (8 x b128), DPP y box ----
+                float sxy[C::EZ];
+                const float4 *col = reinterpret_cast<const float4 *>(SX + cx * C::SXP + cye * C::SYP);
+#pragma unroll
+                for (int m = 0; m < C::EZ / 4; ++m) {
+                    const float4 v = col[m];
+                    sxy[4 * m + 0] = dpp_box<RP>(v.x);
+                    sxy[4 * m + 1] = dpp_box<RP>(v.y);
+                    sxy[4 * m + 2] = dpp_box<RP>(v.z);
+                    sxy[4 * m + 3] = dpp_box<RP>(v.w);
+                }
+
+                // ---- phase 3: sliding z sums, weights, accumulation ----
+                const float *cand = SL + (cx + dx + RP + rs) + LX * (cye + dy + rs) + PP * RP;
+                float S = sxy[0];
+#pragma unroll
+                for (int k = 1; k <= 2 * RP; ++k) S = S + sxy[k];
+#pragma unroll
+                for (int t = 0; t < C::TZ; ++t) {
+                    if (t > 0) S = (S + sxy[t + 2 * RP]) - sxy[t - 1];
+                    const float w = exp2_fast(-S);
+                    Zacc[t] = Zacc[t] + w;
+                    Yacc[t] = fmaf(w, cand[t * PP], Yacc[t]);
+                }
+                __syncthreads();  // s_x is rewritten by the next offset's phase 1
+            }
+    }
+
+    // ---- store (lanes of the y halo carry no output) ----
+    const int gx = x0 + cx, gy = y0 + cye - RP;
+    if (cye >= RP && cye < C::EY - RP && gx < a.in.nx && gy < a.in.ny) {
+        const float inv_sigma = 1.0f / sigma;
+#pragma unroll
+        for (int t = 0; t < C::TZ; ++t) {
+            const int gz = z0 + t;
+            if (gz < a.in.nz) {
+                const long long o = gx + (long long)a.in.nx * (gy + (long long)a.in.ny * gz);
+                a.out[o] = (Yacc[t] / Zacc[t]) * inv_sigma;
+                if (a.Zout) a.Zout[o] = Zacc[t];
+            }
+        }
+    }
+}
+
+// Tile variants.
+enum Fwd3Variant { kV16x16x16 = 0, kV16x16x8 = 1, kVSlab = 2, kNumFwd3Variants = 3 };
+
+// Tile extents of a variant for patch radius rp.
+__host__ __device__ constexpr int fwd3_tx(int v, int rp) { return (void)v, (void)rp, 16; }
+__host__ __device__ constexpr int fwd3_ty(int v, int rp) { return v == kVSlab ? 16 - 2 * rp : 16; }
+__host__ __device__ constexpr int fwd3_tz(int v, int rp) { return v == kVSlab ? 32 - 2 * rp : (v == kV16x16x8 ? 8 : 16); }
 
 }  // namespace gnlm

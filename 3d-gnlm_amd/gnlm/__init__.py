@@ -74,6 +74,8 @@ def _extend(L):
     L.gnlm_autotune.argtypes = [i, i, i, vp, ctypes.POINTER(i), ctypes.POINTER(f)]
     L.gnlm_tuned_variant.argtypes = [i, i]
     L.gnlm_set_variant.argtypes = [i, i, i]
+    L.gnlm_reset_tuning.argtypes = []
+    L.gnlm_reset_tuning.restype = None
     for name in ("gnlm_batch_plan_create", "gnlm_batch_run", "gnlm_batch_run_eager", "gnlm_batch_plan_info",
                  "gnlm_batch_plan_destroy", "gnlm_autotune", "gnlm_tuned_variant", "gnlm_set_variant",
                  "gnlm_num_variants"):
@@ -140,6 +142,13 @@ def autotune(r_p, r_s, probe_n=64, device_tensor=None):
 
 def set_variant(r_p, r_s, variant):
     check(lib().gnlm_set_variant(int(r_p), int(r_s), int(variant)), "gnlm_set_variant")
+
+
+def reset_tuning():
+    lib().gnlm_reset_tuning()
+
+
+VARIANT_16x16x16, VARIANT_16x16x8, VARIANT_SLAB = 0, 1, 2
 
 
 def check(status, where=""):

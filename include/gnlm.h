@@ -132,11 +132,18 @@ int gnlm_batch_plan_destroy(gnlm_batch_plan *plan);
  * `stream`, records the fastest in a process-wide, mutex-guarded table used
  * by gnlm3d_forward and the batch plans, and returns it in *variant (nullable);
  * ms[v] (nullable, gnlm_num_variants() entries) receives each variant's time
- * (-1 if it does not fit).  All variants produce identical bits. */
+ * (-1 if it does not fit).  A volume is always processed with the variant in
+ * the table, so per-volume and batched launches give identical bits. */
 int gnlm_autotune(int r_p, int r_s, int probe_n, void *stream, int *variant, float *ms);
 int gnlm_tuned_variant(int r_p, int r_s);            /* -1 if unsupported */
 int gnlm_set_variant(int r_p, int r_s, int variant);  /* manual override    */
 int gnlm_num_variants(void);
+void gnlm_reset_tuning(void);                        /* forget all choices  */
+/* Variants: 0 = 16x16x16 direct sums, 1 = 16x16x8 direct sums (both bit-
+ * identical to the generic path and the CPU oracle's FP32 order),
+ * 2 = "slab" (16 x (16-2r_p) x (32-2r_p), DPP y sums, sliding x/z sums,
+ * pre-scaled input; agrees with the others to FP32 rounding).  Untuned
+ * default: 2 where it fits, else the first direct variant that fits. */
 
 /* uint8 -> float32 element conversion (n elements), for 8-bit inputs such as
  * the reference's (3DNLM/main.cu:7620 copies unsigned char volumes). */

@@ -66,19 +66,46 @@ def test_batch_matches_oracle(dev):
 
 
 @pytest.mark.parametrize("rp,rs", [(1, 3), (2, 5), (1, 5)])
-def test_autotune_and_variants_identical(dev, rp, rs):
-    v, ms = gnlm.autotune(rp, rs, probe_n=48)
-    print(f"autotune rp={rp} rs={rs}: variant {v}, ms {ms}")
-    assert v >= 0 and ms[v] > 0
-    rng = np.random.default_rng(rp * 10 + rs)
-    u = torch.from_numpy(rng.random((21, 35, 37), dtype=np.float32)).to(dev)
-    outs = []
-    for var in range(len(ms)):
-        if ms[var] < 0:
-            continue
-        gnlm.set_variant(rp, rs, var)
-        outs.append(gnlm.nlm3d(u, rp, rs, 0.2))
-    torch.cuda.synchronize()
-    gnlm.set_variant(rp, rs, v)
-    for o in outs[1:]:
-        assert torch.equal(o, outs[0])
+def test_autotune_variants(dev, rp, rs):
+    """The tuner times every variant that fits and keeps the fastest; the
+    direct-sum variants agree bit for bit, the slab variant to rounding."""
+    try:
+        v, ms = gnlm.autotune(rp, rs, probe_n=48)
+        print(f"autotune rp={rp} rs={rs}: variant {v}, ms {ms}")
+        assert v >= 0 and ms[v] > 0 and ms[v] == min(m for m in ms if m > 0)
+        rng = np.random.default_rng(rp * 10 + rs)
+        u = torch.from_numpy(rng.random((21, 35, 37), dtype=np.float32)).to(dev)
+        outs = {}
+        for var in range(len(ms)):
+            if ms[var] < 0:
+                continue
+            gnlm.set_variant(rp, rs, var)
+            outs[var] = gnlm.nlm3d(u, rp, rs, 0.2).cpu().numpy()
+        direct = [outs[k] for k in (gnlm.VARIANT_16x16x16, gnlm.VARIANT_16x16x8) if k in outs]
+        for o in direct[1:]:
+            assert np.array_equal(o, direct[0])
+        if gnlm.VARIANT_SLAB in outs:
+            assert max_rel(outs[gnlm.VARIANT_SLAB], direct[0]) <= 1e-5
+    finally:
+        gnlm.reset_tuning()
+
+
+def test_batch_slab_and_direct_variants(dev):
+    """Per-volume and batched launches agree bit for bit whichever variant
+    the table selects."""
+    vols = _volumes(dev, 6, 3)
+    try:
+        for var in (gnlm.VARIANT_16x16x16, gnlm.VARIANT_SLAB):
+            for rp in (1, 2):
+                for rs in (3, 5):
+                    gnlm.set_variant(rp, rs, var)
+            outs = [torch.empty_like(u) for u, *_ in vols]
+            plan = gnlm.BatchPlan([(u, o, rp, rs, h) for (u, rp, rs, h), o in zip(vols, outs)])
+            plan.run()
+            refs = [gnlm.nlm3d(u, rp, rs, h) for u, rp, rs, h in vols]
+            torch.cuda.synchronize()
+            for o, r in zip(outs, refs):
+                assert torch.equal(o, r), var
+            plan.close()
+    finally:
+        gnlm.reset_tuning()

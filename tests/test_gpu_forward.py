@@ -65,12 +65,45 @@ def test_3d_matches_oracle(dev, shape, rp, rs, h):
 
 
 @pytest.mark.parametrize("shape,rp,rs,h", CASES3[:4] + CASES3[6:7])
-def test_3d_tiled_equals_generic(dev, shape, rp, rs, h):
+def test_3d_direct_variants_equal_generic(dev, shape, rp, rs, h):
+    """The direct-sum tile variants keep the canonical summation order: they
+    match the generic path (and each other) bit for bit."""
     rng = np.random.default_rng(7)
     u = rng.random(shape, dtype=np.float32)
-    a = _run3(dev, u, rp, rs, h, gnlm.PATH_TILED)
     b = _run3(dev, u, rp, rs, h, gnlm.PATH_GENERIC)
-    assert np.array_equal(a, b), report(a, b)
+    try:
+        for v in (gnlm.VARIANT_16x16x16, gnlm.VARIANT_16x16x8):
+            gnlm.set_variant(rp, rs, v)
+            a = _run3(dev, u, rp, rs, h, gnlm.PATH_TILED)
+            assert np.array_equal(a, b), (v, report(a, b))
+    finally:
+        gnlm.reset_tuning()
+
+
+SLAB_CASES = [
+    ((28, 24, 32), 2, 5, 0.1),    # exactly 2x2x1 slab tiles for r_p = 2
+    ((37, 29, 33), 2, 5, 0.2),    # partial tiles on every axis
+    ((30, 14, 16), 1, 3, 0.3),
+    ((26, 20, 17), 3, 2, 0.4),
+    ((9, 8, 7), 1, 4, 0.3),       # smaller than one tile, reflection folds
+    ((32, 16, 16), 0, 3, 0.3),
+]
+
+
+@pytest.mark.parametrize("shape,rp,rs,h", SLAB_CASES)
+def test_3d_slab_variant_matches_oracle(dev, shape, rp, rs, h):
+    rng = np.random.default_rng(sum(shape) + 31 * rp + rs)
+    u = rng.random(shape, dtype=np.float32)
+    print(f"case slab shape={shape} rp={rp} rs={rs}", flush=True)
+    try:
+        gnlm.set_variant(rp, rs, gnlm.VARIANT_SLAB)
+        y = _run3(dev, u, rp, rs, h, gnlm.PATH_TILED)
+    finally:
+        gnlm.reset_tuning()
+    ref32 = oracle.nlm3d_brute(u, rp, rs, h)
+    ref64 = oracle.nlm3d_brute(u.astype(np.float64), rp, rs, h)
+    assert max_rel(y, ref32) <= TOL, report(y, ref32)
+    assert max_rel(y, ref64) <= TOL, report(y, ref64)
 
 
 def test_3d_large_patch_generic(dev):
