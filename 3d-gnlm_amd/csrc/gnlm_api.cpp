@@ -132,15 +132,15 @@ int gnlm_u8_to_f32(const unsigned char *d_in, float *d_out, long long n, void *s
 // host-pointer entry points (drop-in backing)
 // ---------------------------------------------------------------------------
 
-int gnlm_nl_means_3d_u8(const unsigned char *h_in, float *h_out, int X, int Y, int depth, int SS, int NS,
+int gnlm_nl_means_3d_u8(const unsigned char *h_in, float *h_out, int nx, int ny, int nz, int ss, int ns,
                         float h)
 {
-    if (!h_in || !h_out || X < 1 || Y < 1 || depth < 1) return GNLM_EINVAL;
-    if (SS < 1 || NS < 1 || (SS % 2) == 0 || (NS % 2) == 0) return GNLM_EINVAL;
-    const int r_s = (SS - 1) / 2, r_p = (NS - 1) / 2;
-    const int pad = (SS + NS - 2) / 2;  // = r_s + r_p, 3DNLM/nlm.cpp:33-36
-    const long long W = X + 2LL * pad, H = Y + 2LL * pad, D = depth + 2LL * pad;
-    const long long nin = W * H * D, nout = (long long)X * Y * depth;
+    if (!h_in || !h_out || nx < 1 || ny < 1 || nz < 1) return GNLM_EINVAL;
+    if (ss < 1 || ns < 1 || (ss % 2) == 0 || (ns % 2) == 0) return GNLM_EINVAL;
+    const int r_s = (ss - 1) / 2, r_p = (ns - 1) / 2;
+    const int pad = (ss + ns - 2) / 2;  // = r_s + r_p, 3DNLM/nlm.cpp:33-36
+    const long long W = nx + 2LL * pad, H = ny + 2LL * pad, D = nz + 2LL * pad;
+    const long long nin = W * H * D, nout = (long long)nx * ny * nz;
     // 16-byte alignment for the float image: round the byte buffer up
     const long long nin_al = (nin + 15) / 16 * 16;
     unsigned char *d_u8 = nullptr;
@@ -153,7 +153,7 @@ int gnlm_nl_means_3d_u8(const unsigned char *h_in, float *h_out, int X, int Y, i
     if (e != hipSuccess) st = hip_status(e);
     if (st == GNLM_OK) st = gnlm_u8_to_f32(d_u8, d_f, nin, nullptr);
     if (st == GNLM_OK)
-        st = gnlm3d_forward_ex(d_f, pad, W, W * H, d_o, nullptr, X, Y, depth, r_p, r_s, h, GNLM_PATH_AUTO,
+        st = gnlm3d_forward_ex(d_f, pad, W, W * H, d_o, nullptr, nx, ny, nz, r_p, r_s, h, GNLM_PATH_AUTO,
                                nullptr);
     if (st == GNLM_OK) {
         e = hipMemcpy(h_out, d_o, sizeof(float) * (size_t)nout, hipMemcpyDeviceToHost);

@@ -151,14 +151,16 @@ int gnlm_u8_to_f32(const unsigned char *d_in, float *d_out, long long n, void *s
 
 /* ---- host-pointer, synchronous entry points (the drop-in's backing) ----
  * They allocate device memory for one call, copy in, run, copy out and free,
- * as the reference's launchers do (3DNLM/main.cu:7609-7676).
+ * as the reference's launchers do (3DNLM/main.cu:7609-7676).  (Parameter
+ * names avoid the reference's config.h macros SS, NS, depth, IMAGE_SIZE_*.)
  *
- * 3D: h_in is the padded uint8 volume of (X+SS+NS-2)*(Y+SS+NS-2)*(depth+SS+NS-2)
- * bytes, x fastest (3DNLM/nlm.cpp:48-61); h_out receives X*Y*depth floats.
- * SS and NS are the window SIZES (odd).  h is on the 0-255 scale
- * (the reference hard-wires PARAMETER = 10, 3DNLM/main.cu:13). */
-int gnlm_nl_means_3d_u8(const unsigned char *h_in, float *h_out, int X, int Y, int depth, int SS,
-                        int NS, float h);
+ * 3D: h_in is the padded uint8 volume of (nx+ss+ns-2)*(ny+ss+ns-2)*(nz+ss+ns-2)
+ * bytes, x fastest (3DNLM/nlm.cpp:48-61); h_out receives nx*ny*nz floats.
+ * ss and ns are the search and patch window SIZES (odd; the reference's SS
+ * and NS).  h is on the 0-255 scale (the reference hard-wires
+ * PARAMETER = 10, 3DNLM/main.cu:13). */
+int gnlm_nl_means_3d_u8(const unsigned char *h_in, float *h_out, int nx, int ny, int nz, int ss, int ns,
+                        float h);
 
 /* 2D: h_in is the padded uint8 image whose row stride is W (the reference
  * kernels index it with stride W, "2D NLM/11x11/main.cu:36"), padded by
