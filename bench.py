@@ -52,12 +52,152 @@ def _traffic_from_profiles(name):
         return None
 
 
+def _threads():
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def bench_batch(args):
+    """64 volumes of uneven size (dims from {32, 48, ..., 192} per axis),
+    ellipsoid phantoms with their own sigma ~ U[0.02, 0.10] and h = sigma,
+    r_p in {1, 2}, r_s in {3, 5}: one graph-replayed persistent launch vs 64
+    per-volume launches; outputs must be bit-identical."""
+    import torch
+    import gnlm
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(20261208)
+    dims_set = np.arange(32, 193, 16)
+    vols, outs_a, outs_b, pairs = [], [], [], 0
+    for i in range(64):
+        nx, ny, nz = (int(v) for v in rng.choice(dims_set, 3))
+        sigma = float(rng.uniform(0.02, 0.10))
+        rp, rs = int(rng.choice([1, 2])), int(rng.choice([3, 5]))
+        u, _ = gnlm.phantom3d(nx, ny, nz, n_ellipsoids=12, axis_lo=4, axis_hi=20, background=0.05, sigma=sigma,
+                              seed=20261208 * 100 + i)
+        t = torch.from_numpy(u).to(dev)
+        vols.append((t, rp, rs, sigma))
+        outs_a.append(torch.empty_like(t))
+        outs_b.append(torch.empty_like(t))
+        pairs += nx * ny * nz * (2 * rs + 1) ** 3
+    plan = gnlm.BatchPlan([(u, o, rp, rs, h) for (u, rp, rs, h), o in zip(vols, outs_b)])
+    ntiles, nblocks = plan.info()
+
+    def plain():
+        for (u, rp, rs, h), o in zip(vols, outs_a):
+            gnlm.nlm3d(u, rp, rs, h, out=o)
+
+    def batched():
+        plan.run()
+
+    res = {}
+    for name, fn in (("plain", plain), ("batched", batched)):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+        torch.cuda.synchronize()
+        res[name] = (time.perf_counter() - t0) / args.steps
+    identical = all(torch.equal(a, b) for a, b in zip(outs_a, outs_b))
+    nvox = sum(u.numel() for u, *_ in vols)
+    return {
+        "metric": "batched persistent speedup vs 64 per-volume launches",
+        "value": round(res["plain"] / res["batched"], 4),
+        "unit": "x",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(res["batched"] * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "nlm3d_batch64", "volumes": 64, "voxels": nvox, "pairs": pairs, "tiles": ntiles,
+                   "workgroups": nblocks},
+        "plain_ms": round(res["plain"] * 1e3, 4), "batched_ms": round(res["batched"] * 1e3, 4),
+        "batched_Mvox_s": round(nvox / res["batched"] / 1e6, 2), "bit_identical": identical,
+    }
+
+
+def bench_grad(args):
+    """8 volumes of 128^3 (ellipsoid phantoms, sigma 0.05), patch 3^3, search
+    9^3, per-volume learnable h starting at 0.1: forward vs forward+adjoint of
+    the MSE loss against the clean phantoms, then 50 gradient steps on h."""
+    import torch
+    import gnlm
+    dev = torch.device("cuda:0")
+    rp, rs = 1, 4
+    us, cs = [], []
+    for i in range(8):
+        u, c = gnlm.phantom3d(128, 128, 128, n_ellipsoids=24, axis_lo=5, axis_hi=30, background=0.05, sigma=0.05,
+                              seed=20261207 + i)
+        us.append(torch.from_numpy(u).to(dev))
+        cs.append(torch.from_numpy(c).to(dev))
+    n = us[0].numel()
+
+    def fwd(h):
+        return [gnlm.nlm3d(u, rp, rs, float(hv), want_Z=True) for u, hv in zip(us, h)]
+
+    def fwd_bwd(h):
+        gh = []
+        loss = 0.0
+        for (u, c, hv) in zip(us, cs, h):
+            y, Z = gnlm.nlm3d(u, rp, rs, float(hv), want_Z=True)
+            r = y - c
+            g = (2.0 / n) * r
+            _, dh = gnlm.nlm3d_backward(u, y, Z, g, rp, rs, float(hv))
+            gh.append(dh)
+            loss = loss + (r * r).mean()
+        return loss, gh
+
+    h0 = [0.1] * 8
+    times = {}
+    for name, fn in (("fwd", lambda: fwd(h0)), ("fwd_bwd", lambda: fwd_bwd(h0))):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+        torch.cuda.synchronize()
+        times[name] = (time.perf_counter() - t0) / args.steps
+    # 50 Adam steps on log h per volume
+    logh = torch.full((8,), float(np.log(0.1)), device=dev)
+    m = torch.zeros(8, device=dev)
+    v = torch.zeros(8, device=dev)
+    lr, b1, b2 = 0.05, 0.9, 0.999
+    traj = []
+    t0 = time.perf_counter()
+    for step in range(1, 51):
+        h = torch.exp(logh)
+        loss, gh = fwd_bwd(h.tolist())
+        g = torch.stack(gh) * h  # d/dlogh
+        m = b1 * m + (1 - b1) * g
+        v = b2 * v + (1 - b2) * g * g
+        logh = logh - lr * (m / (1 - b1 ** step)) / (torch.sqrt(v / (1 - b2 ** step)) + 1e-12)
+        if step in (1, 10, 25, 50):
+            traj.append({"step": step, "loss": float(loss), "h_mean": float(h.mean())})
+    fit_s = time.perf_counter() - t0
+    return {
+        "metric": "fwd+bwd / fwd time ratio, 8x128^3 p3 s9",
+        "value": round(times["fwd_bwd"] / times["fwd"], 4),
+        "unit": "x",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(times["fwd_bwd"] * 1e3, 4),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "nlm3d_grad_8x128", "volumes": 8, "dims": [128, 128, 128], "r_p": rp, "r_s": rs,
+                   "h0": 0.1},
+        "fwd_ms": round(times["fwd"] * 1e3, 4), "fwd_bwd_ms": round(times["fwd_bwd"] * 1e3, 4),
+        "fit_h": {"steps": 50, "seconds": round(fit_s, 3), "trajectory": traj,
+                  "h_final": [round(float(x), 5) for x in torch.exp(logh).tolist()]},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="nlm3d_256_p5_s11", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="nlm3d_256_p5_s11",
+                    choices=sorted(WORKLOADS) + ["nlm3d_batch64", "nlm3d_grad_8x128"])
     ap.add_argument("--cpu-planes", type=int, default=8, help="z-planes in the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -70,6 +210,12 @@ def main():
     import torch
     import gnlm
 
+    if args.workload == "nlm3d_batch64":
+        print(json.dumps(bench_batch(args)))
+        return
+    if args.workload == "nlm3d_grad_8x128":
+        print(json.dumps(bench_grad(args)))
+        return
     wl = WORKLOADS[args.workload]
     dev = torch.device("cuda:0")
     rp, rs, h = wl["rp"], wl["rs"], wl["h"]
@@ -123,11 +269,7 @@ def main():
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle
         yc = y.cpu().numpy()
-        threads = os.cpu_count() or 1
-        try:
-            threads = len(os.sched_getaffinity(0))
-        except AttributeError:
-            pass
+        threads = _threads()
         if nz > 1:
             nplanes = max(1, min(args.cpu_planes, nz))
             z0 = nz // 2 - nplanes // 2

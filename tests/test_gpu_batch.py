@@ -1,15 +1,17 @@
 """Batched persistent forward and the autotuner.
 
-Bar: bit-identical outputs between the graph-replayed persistent launch, its
-eager launch, and per-volume gnlm3d_forward calls; every tile variant gives
-identical bits."""
+Bars: bit-identical outputs between the graph-replayed persistent launch, its
+eager launch, and per-volume gnlm3d_forward calls (whatever tile variant the
+table selects); against the oracle, relative error floored at 1e-3 of the
+output range <= 1e-5 vs the CPU FP32 path and <= 1e-4 vs FP64 (the FP32
+arithmetic's own distance from FP64)."""
 import numpy as np
 import pytest
 import torch
 
 import gnlm
 import oracle
-from nlm_metrics import max_rel, report
+from nlm_metrics import max_rel, max_rel_fl, report
 
 pytestmark = pytest.mark.gpu
 
@@ -59,9 +61,13 @@ def test_batch_matches_oracle(dev):
     plan.run()
     torch.cuda.synchronize()
     for (u, rp, rs, h), o in zip(vols, outs):
-        ref = oracle.nlm3d_separable_f64(u.cpu().numpy(), rp, rs, h)
+        un = u.cpu().numpy()
+        ref32 = oracle.nlm3d_separable_f32(un, rp, rs, h)
+        ref64 = oracle.nlm3d_separable_f64(un, rp, rs, h)
         got = o.cpu().numpy()
-        assert max_rel(got, ref, floor=1e-3 * (ref.max() - ref.min())) <= 1e-5, report(got, ref)
+        print("batch vol", un.shape, rp, rs, "vs fp32", report(got, ref32), "vs fp64", report(got, ref64))
+        assert max_rel_fl(got, ref32) <= 1e-5, report(got, ref32)
+        assert max_rel_fl(got, ref64) <= 1e-4, report(got, ref64)
     plan.close()
 
 

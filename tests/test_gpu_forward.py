@@ -102,8 +102,9 @@ def test_3d_slab_variant_matches_oracle(dev, shape, rp, rs, h):
         gnlm.reset_tuning()
     ref32 = oracle.nlm3d_brute(u, rp, rs, h)
     ref64 = oracle.nlm3d_brute(u.astype(np.float64), rp, rs, h)
+    print("slab vs fp32", report(y, ref32), "vs fp64", report(y, ref64))
     assert max_rel(y, ref32) <= TOL, report(y, ref32)
-    assert max_rel(y, ref64) <= TOL, report(y, ref64)
+    assert max_rel(y, ref64) <= TOL_FP64, report(y, ref64)
 
 
 def test_3d_large_patch_generic(dev):
