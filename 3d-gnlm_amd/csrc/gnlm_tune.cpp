@@ -27,6 +27,7 @@ std::map<std::pair<int, int>, int> g_table;  // (r_p, r_s) -> variant
 // fits (fastest on the measured configurations), else the direct-sum tiles.
 int first_supported(int r_p, int r_s)
 {
+    if (fwd3d_variant_supported(r_p, r_s, kVSlab2)) return kVSlab2;
     if (fwd3d_variant_supported(r_p, r_s, kVSlab)) return kVSlab;
     for (int v = 0; v < kNumFwd3Variants; ++v)
         if (fwd3d_variant_supported(r_p, r_s, v)) return v;

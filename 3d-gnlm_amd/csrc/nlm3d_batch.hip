@@ -55,7 +55,13 @@ __device__ __forceinline__ void run_tile_rp(const BatchVol &v, const BatchTile &
     a.rs = v.rs;
     a.k = v.k;
     a.sigma = v.sigma;
-    if (v.variant == kVSlab) {
+    if (v.variant == kVSlab2) {
+        using C = SlabCfg<RP>;
+        a.x0 = t.tx * C::TX;
+        a.y0 = t.ty * C::TY;
+        a.z0 = t.tz * C::TZ;
+        fwd3_slab2_tile<RP>(a, lds, v.sigma);
+    } else if (v.variant == kVSlab) {
         using C = SlabCfg<RP>;
         a.x0 = t.tx * C::TX;
         a.y0 = t.ty * C::TY;

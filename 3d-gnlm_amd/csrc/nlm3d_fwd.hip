@@ -16,7 +16,7 @@ struct Fwd3Args {
     int ntx, nty, ntz;
 };
 
-template <int RP>
+template <int RP, bool WIDE>
 __global__ __launch_bounds__(kFwdThreads) void nlm3d_fwd_slab(Fwd3Args a, float sigma)
 {
     extern __shared__ float lds[];
@@ -33,10 +33,13 @@ __global__ __launch_bounds__(kFwdThreads) void nlm3d_fwd_slab(Fwd3Args a, float 
     tl.x0 = (t % a.ntx) * C::TX;
     tl.y0 = ((t / a.ntx) % a.nty) * C::TY;
     tl.z0 = (t / (a.ntx * a.nty)) * C::TZ;
-    fwd3_slab_tile<RP>(tl, lds, sigma);
+    if constexpr (WIDE)
+        fwd3_slab2_tile<RP>(tl, lds, sigma);
+    else
+        fwd3_slab_tile<RP>(tl, lds, sigma);
 }
 
-template <int RP>
+template <int RP, bool WIDE>
 hipError_t launch_slab(const InDesc &in, float *out, float *Zout, int rs, float k, hipStream_t s)
 {
     using C = SlabCfg<RP>;
@@ -49,12 +52,13 @@ hipError_t launch_slab(const InDesc &in, float *out, float *Zout, int rs, float 
     a.ntx = (in.nx + C::TX - 1) / C::TX;
     a.nty = (in.ny + C::TY - 1) / C::TY;
     a.ntz = (in.nz + C::TZ - 1) / C::TZ;
-    const size_t lds = slab_lds_floats(RP, rs) * sizeof(float);
-    hipError_t e = hipFuncSetAttribute((const void *)nlm3d_fwd_slab<RP>,
+    const size_t lds = (WIDE ? slab2_lds_floats(RP, rs) : slab_lds_floats(RP, rs)) * sizeof(float);
+    hipError_t e = hipFuncSetAttribute((const void *)nlm3d_fwd_slab<RP, WIDE>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     const float sigma = std::sqrt(-k);
-    hipLaunchKernelGGL(nlm3d_fwd_slab<RP>, dim3(a.ntx * a.nty * a.ntz), dim3(kFwdThreads), lds, s, a, sigma);
+    hipLaunchKernelGGL((nlm3d_fwd_slab<RP, WIDE>), dim3(a.ntx * a.nty * a.ntz), dim3(kFwdThreads), lds, s, a,
+                       sigma);
     return hipGetLastError();
 }
 
@@ -100,7 +104,8 @@ hipError_t launch_cfg(const InDesc &in, float *out, float *Zout, int rs, float k
 template <int RP>
 hipError_t launch_rp(const InDesc &in, float *out, float *Zout, int rs, float k, int variant, hipStream_t s)
 {
-    if (variant == kVSlab) return launch_slab<RP>(in, out, Zout, rs, k, s);
+    if (variant == kVSlab) return launch_slab<RP, false>(in, out, Zout, rs, k, s);
+    if (variant == kVSlab2) return launch_slab<RP, true>(in, out, Zout, rs, k, s);
     if (variant == kV16x16x8) return launch_cfg<Fwd3Cfg<RP, 16, 16, 8>>(in, out, Zout, rs, k, s);
     return launch_cfg<Fwd3Cfg<RP, 16, 16, 16>>(in, out, Zout, rs, k, s);
 }
@@ -109,6 +114,7 @@ template <int RP>
 size_t lds_rp(int rs, int variant)
 {
     if (variant == kVSlab) return slab_lds_floats(RP, rs) * sizeof(float);
+    if (variant == kVSlab2) return slab2_lds_floats(RP, rs) * sizeof(float);
     if (variant == kV16x16x8) return fwd3_lds_floats<Fwd3Cfg<RP, 16, 16, 8>>(rs) * sizeof(float);
     return fwd3_lds_floats<Fwd3Cfg<RP, 16, 16, 16>>(rs) * sizeof(float);
 }

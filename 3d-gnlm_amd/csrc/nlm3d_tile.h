@@ -384,12 +384,170 @@ __device__ __forceinline__ void fwd3_slab_tile(const Fwd3Tile &a, float *lds, fl
     }
 }
 
+// ===========================================================================
+// Variant "slab2": the slab scheme with the LDS traffic cut by wide accesses.
+// Same tile, same arithmetic order as "slab" (identical bits), but
+//   * the slab is stored z-fastest ([y][x][z], x pitch 36 = 4 mod 32, y pitch
+//     = 4 mod 32), staged with one 128-bit write per 4 planes;
+//   * a phase-1 thread owns a PAIR of rows (y_ext, z_ext = 2p, 2p+1): every
+//     shifted sample pair is one 64-bit read, every s_x pair one 64-bit write;
+//   * the 28 candidate values of a phase-3 column are eight 128-bit reads.
+// Per tile and offset this is ~210 LDS instructions instead of ~430.
+// ===========================================================================
+__host__ __device__ constexpr int slab2_yp(int lx) { return 36 * lx + ((4 - 4 * lx) % 32 + 32) % 32; }
+
+__host__ __device__ constexpr size_t slab2_lds_floats(int rp, int rs)
+{
+    return (size_t)slab2_yp(16 + 2 * (rp + rs)) * (16 + 2 * rs) + (size_t)16 * 16 * 36;
+}
+
+template <int RP>
+__device__ __forceinline__ void fwd3_slab2_tile(const Fwd3Tile &a, float *lds, float sigma)
+{
+    using C = SlabCfg<RP>;
+    constexpr int NT = kFwdThreads;
+    constexpr int XP = 36;                    // x pitch of the slab (floats)
+    const int rs = a.rs;
+    const int LX = C::TX + 2 * (RP + rs);
+    const int LY = C::EY + 2 * rs;
+    const int YP = slab2_yp(LX);              // y pitch of the slab, == 4 mod 32
+    float *SL = lds;                          // slab [ly][lx][lz], lz in [0, 32)
+    float *SX = lds + YP * LY;                // s_x[x][y_ext][z_ext]
+    const int x0 = a.x0, y0 = a.y0, z0 = a.z0;
+    const int tid = threadIdx.x;
+
+    // phase-1 rows: y_ext = tid / 16, z_ext = 2 * (tid % 16) + {0, 1}
+    const int p_y = tid / 16;
+    const int p_z = 2 * (tid % 16);
+    float cen0[C::EX], cen1[C::EX];
+#pragma unroll
+    for (int i = 0; i < C::EX; ++i) {
+        cen0[i] = sigma * load_in(a.in, x0 - RP + i, y0 - RP + p_y, z0 - RP + p_z);
+        cen1[i] = sigma * load_in(a.in, x0 - RP + i, y0 - RP + p_y, z0 - RP + p_z + 1);
+    }
+    // phase-2/3 ownership: column x = tid / 16, y_ext = tid % 16
+    const int cx = tid / C::EY;
+    const int cye = tid % C::EY;
+    float Zacc[C::TZ], Yacc[C::TZ];
+#pragma unroll
+    for (int t = 0; t < C::TZ; ++t) {
+        Zacc[t] = 0.0f;
+        Yacc[t] = 0.0f;
+    }
+
+    for (int dz = -rs; dz <= rs; ++dz) {
+        // ---- stage the slab: one 128-bit LDS write per (x, y, 4 planes) ----
+        __syncthreads();
+        const int xs = x0 - RP - rs, ys = y0 - RP - rs, zs = z0 - RP + dz;
+        for (int i = tid; i < LX * LY * (C::EZ / 4); i += NT) {
+            const int lx = i % LX;
+            const int r = i / LX;
+            const int ly = r % LY;
+            const int q = r / LY;
+            float4 v;
+            v.x = sigma * load_in(a.in, xs + lx, ys + ly, zs + 4 * q + 0);
+            v.y = sigma * load_in(a.in, xs + lx, ys + ly, zs + 4 * q + 1);
+            v.z = sigma * load_in(a.in, xs + lx, ys + ly, zs + 4 * q + 2);
+            v.w = sigma * load_in(a.in, xs + lx, ys + ly, zs + 4 * q + 3);
+            *reinterpret_cast<float4 *>(SL + YP * ly + XP * lx + 4 * q) = v;
+        }
+        __syncthreads();
+
+        for (int dy = -rs; dy <= rs; ++dy)
+            for (int dx = -rs; dx <= rs; ++dx) {
+                // ---- phase 1: two rows, sliding x sums ----
+                {
+                    const float *sh = SL + YP * (p_y + dy + rs) + XP * (rs + dx) + p_z;
+                    float e0[C::EX], e1[C::EX];
+#pragma unroll
+                    for (int i = 0; i < C::EX; ++i) {
+                        const float2 v = *reinterpret_cast<const float2 *>(sh + XP * i);
+                        e0[i] = cen0[i] - v.x;
+                        e1[i] = cen1[i] - v.y;
+                    }
+                    float s0 = e0[0] * e0[0], s1 = e1[0] * e1[0];
+#pragma unroll
+                    for (int i = 1; i <= 2 * RP; ++i) {
+                        s0 = fmaf(e0[i], e0[i], s0);
+                        s1 = fmaf(e1[i], e1[i], s1);
+                    }
+                    float *dst = SX + C::SYP * p_y + p_z;
+                    *reinterpret_cast<float2 *>(dst) = make_float2(s0, s1);
+#pragma unroll
+                    for (int j = 1; j < C::TX; ++j) {
+                        s0 = fmaf(e0[j + 2 * RP], e0[j + 2 * RP], s0);
+                        s0 = fmaf(-e0[j - 1], e0[j - 1], s0);
+                        s1 = fmaf(e1[j + 2 * RP], e1[j + 2 * RP], s1);
+                        s1 = fmaf(-e1[j - 1], e1[j - 1], s1);
+                        *reinterpret_cast<float2 *>(dst + j * C::SXP) = make_float2(s0, s1);
+                    }
+                }
+                __syncthreads();
+
+                // ---- phase 2: column of s_x (8 x b128), DPP y box ----
+                float sxy[C::EZ];
+                const float4 *col = reinterpret_cast<const float4 *>(SX + cx * C::SXP + cye * C::SYP);
+#pragma unroll
+                for (int m = 0; m < C::EZ / 4; ++m) {
+                    const float4 v = col[m];
+                    sxy[4 * m + 0] = dpp_box<RP>(v.x);
+                    sxy[4 * m + 1] = dpp_box<RP>(v.y);
+                    sxy[4 * m + 2] = dpp_box<RP>(v.z);
+                    sxy[4 * m + 3] = dpp_box<RP>(v.w);
+                }
+
+                // ---- phase 3: sliding z sums, weights, accumulation ----
+                const float4 *cz4 = reinterpret_cast<const float4 *>(
+                    SL + YP * (cye + dy + rs) + XP * (cx + dx + RP + rs));
+                float cz[C::EZ];
+#pragma unroll
+                for (int m = 0; m < C::EZ / 4; ++m) {
+                    const float4 v = cz4[m];
+                    cz[4 * m + 0] = v.x;
+                    cz[4 * m + 1] = v.y;
+                    cz[4 * m + 2] = v.z;
+                    cz[4 * m + 3] = v.w;
+                }
+                float S = sxy[0];
+#pragma unroll
+                for (int k = 1; k <= 2 * RP; ++k) S = S + sxy[k];
+#pragma unroll
+                for (int t = 0; t < C::TZ; ++t) {
+                    if (t > 0) S = (S + sxy[t + 2 * RP]) - sxy[t - 1];
+                    const float w = exp2_fast(-S);
+                    Zacc[t] = Zacc[t] + w;
+                    Yacc[t] = fmaf(w, cz[t + RP], Yacc[t]);
+                }
+                __syncthreads();  // s_x is rewritten by the next offset's phase 1
+            }
+    }
+
+    const int gx = x0 + cx, gy = y0 + cye - RP;
+    if (cye >= RP && cye < C::EY - RP && gx < a.in.nx && gy < a.in.ny) {
+        const float inv_sigma = 1.0f / sigma;
+#pragma unroll
+        for (int t = 0; t < C::TZ; ++t) {
+            const int gz = z0 + t;
+            if (gz < a.in.nz) {
+                const long long o = gx + (long long)a.in.nx * (gy + (long long)a.in.ny * gz);
+                a.out[o] = (Yacc[t] / Zacc[t]) * inv_sigma;
+                if (a.Zout) a.Zout[o] = Zacc[t];
+            }
+        }
+    }
+}
+
 // Tile variants.
-enum Fwd3Variant { kV16x16x16 = 0, kV16x16x8 = 1, kVSlab = 2, kNumFwd3Variants = 3 };
+enum Fwd3Variant { kV16x16x16 = 0, kV16x16x8 = 1, kVSlab = 2, kVSlab2 = 3, kNumFwd3Variants = 4 };
+
+__host__ __device__ constexpr bool fwd3_is_slab(int v) { return v == kVSlab || v == kVSlab2; }
 
 // Tile extents of a variant for patch radius rp.
 __host__ __device__ constexpr int fwd3_tx(int v, int rp) { return (void)v, (void)rp, 16; }
-__host__ __device__ constexpr int fwd3_ty(int v, int rp) { return v == kVSlab ? 16 - 2 * rp : 16; }
-__host__ __device__ constexpr int fwd3_tz(int v, int rp) { return v == kVSlab ? 32 - 2 * rp : (v == kV16x16x8 ? 8 : 16); }
+__host__ __device__ constexpr int fwd3_ty(int v, int rp) { return fwd3_is_slab(v) ? 16 - 2 * rp : 16; }
+__host__ __device__ constexpr int fwd3_tz(int v, int rp)
+{
+    return fwd3_is_slab(v) ? 32 - 2 * rp : (v == kV16x16x8 ? 8 : 16);
+}
 
 }  // namespace gnlm

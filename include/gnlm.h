@@ -142,8 +142,9 @@ void gnlm_reset_tuning(void);                        /* forget all choices  */
 /* Variants: 0 = 16x16x16 direct sums, 1 = 16x16x8 direct sums (both bit-
  * identical to the generic path and the CPU oracle's FP32 order),
  * 2 = "slab" (16 x (16-2r_p) x (32-2r_p), DPP y sums, sliding x/z sums,
- * pre-scaled input; agrees with the others to FP32 rounding).  Untuned
- * default: 2 where it fits, else the first direct variant that fits. */
+ * pre-scaled input; agrees with the others to FP32 rounding), 3 = "slab2"
+ * (variant 2 with 64/128-bit LDS accesses; bit-identical to 2).  Untuned
+ * default: 3 where it fits, else 2, else the first direct variant. */
 
 /* uint8 -> float32 element conversion (n elements), for 8-bit inputs such as
  * the reference's (3DNLM/main.cu:7620 copies unsigned char volumes). */

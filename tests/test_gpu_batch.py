@@ -90,8 +90,9 @@ def test_autotune_variants(dev, rp, rs):
         direct = [outs[k] for k in (gnlm.VARIANT_16x16x16, gnlm.VARIANT_16x16x8) if k in outs]
         for o in direct[1:]:
             assert np.array_equal(o, direct[0])
-        if gnlm.VARIANT_SLAB in outs:
-            assert max_rel(outs[gnlm.VARIANT_SLAB], direct[0]) <= 1e-5
+        for sv in (gnlm.VARIANT_SLAB, gnlm.VARIANT_SLAB2):
+            if sv in outs:
+                assert max_rel(outs[sv], direct[0]) <= 1e-5
     finally:
         gnlm.reset_tuning()
 
@@ -101,7 +102,7 @@ def test_batch_slab_and_direct_variants(dev):
     the table selects."""
     vols = _volumes(dev, 6, 3)
     try:
-        for var in (gnlm.VARIANT_16x16x16, gnlm.VARIANT_SLAB):
+        for var in (gnlm.VARIANT_16x16x16, gnlm.VARIANT_SLAB, gnlm.VARIANT_SLAB2):
             for rp in (1, 2):
                 for rs in (3, 5):
                     gnlm.set_variant(rp, rs, var)

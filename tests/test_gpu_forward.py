@@ -90,13 +90,15 @@ SLAB_CASES = [
 ]
 
 
+@pytest.mark.parametrize("variant", ["slab", "slab2"])
 @pytest.mark.parametrize("shape,rp,rs,h", SLAB_CASES)
-def test_3d_slab_variant_matches_oracle(dev, shape, rp, rs, h):
+def test_3d_slab_variant_matches_oracle(dev, shape, rp, rs, h, variant):
     rng = np.random.default_rng(sum(shape) + 31 * rp + rs)
     u = rng.random(shape, dtype=np.float32)
-    print(f"case slab shape={shape} rp={rp} rs={rs}", flush=True)
+    print(f"case {variant} shape={shape} rp={rp} rs={rs}", flush=True)
+    v = gnlm.VARIANT_SLAB if variant == "slab" else gnlm.VARIANT_SLAB2
     try:
-        gnlm.set_variant(rp, rs, gnlm.VARIANT_SLAB)
+        gnlm.set_variant(rp, rs, v)
         y = _run3(dev, u, rp, rs, h, gnlm.PATH_TILED)
     finally:
         gnlm.reset_tuning()
@@ -105,6 +107,21 @@ def test_3d_slab_variant_matches_oracle(dev, shape, rp, rs, h):
     print("slab vs fp32", report(y, ref32), "vs fp64", report(y, ref64))
     assert max_rel(y, ref32) <= TOL, report(y, ref32)
     assert max_rel(y, ref64) <= TOL_FP64, report(y, ref64)
+
+
+@pytest.mark.parametrize("shape,rp,rs,h", SLAB_CASES)
+def test_3d_slab2_equals_slab(dev, shape, rp, rs, h):
+    """The wide-access slab kernel keeps the slab kernel's arithmetic order."""
+    rng = np.random.default_rng(5)
+    u = rng.random(shape, dtype=np.float32)
+    outs = []
+    try:
+        for v in (gnlm.VARIANT_SLAB, gnlm.VARIANT_SLAB2):
+            gnlm.set_variant(rp, rs, v)
+            outs.append(_run3(dev, u, rp, rs, h, gnlm.PATH_TILED))
+    finally:
+        gnlm.reset_tuning()
+    assert np.array_equal(outs[0], outs[1]), report(outs[1], outs[0])
 
 
 def test_3d_large_patch_generic(dev):
