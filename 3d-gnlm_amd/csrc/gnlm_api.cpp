@@ -96,7 +96,10 @@ int gnlm2d_forward_ex(const float *d_in, int in_pad, long long in_sy, float *d_o
     long long in_sz = 0;
     int st = check_padded(in_pad, in_sy, in_sz, nx, ny, R);
     if (st != GNLM_OK) return st;
-    InDesc in{d_in, nx, ny, 1, in_pad, in_sy, in_sz};
+    // An image has no z padding: the single plane sits at z = 0 of the
+    // buffer, so the plane stride must not contribute (load_in adds
+    // sz * (z + pad) with z = 0).
+    InDesc in{d_in, nx, ny, 1, in_pad, in_sy, 0};
     const int P = (2 * r_p + 1) * (2 * r_p + 1);
     const float k = (float)weight_scale(P, h);
     hipStream_t s = (hipStream_t)stream;

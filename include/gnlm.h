@@ -141,7 +141,7 @@ int gnlm_num_variants(void);
 void gnlm_reset_tuning(void);                        /* forget all choices  */
 /* Variants: 0 = 16x16x16 direct sums, 1 = 16x16x8 direct sums (both bit-
  * identical to the generic path and the CPU oracle's FP32 order),
- * 2 = "slab" (16 x (16-2r_p) x (32-2r_p), DPP y sums, sliding x/z sums,
+ * 2 = "slab" (16 x (16-2r_p) x (32-2r_p), DPP y sums, direct x/z sums,
  * pre-scaled input; agrees with the others to FP32 rounding), 3 = "slab2"
  * (variant 2 with 64/128-bit LDS accesses; bit-identical to 2).  Untuned
  * default: 3 where it fits, else 2, else the first direct variant. */
