@@ -18,7 +18,8 @@
 //     8 + 2 r_p s_x values it needs once, sums each output's window directly
 //     in registers, and accumulates Z and Y in registers.
 // Sums follow the canonical order (x sums fused with fma, then y sums left to
-// right), identical to the CPU oracle and the generic path.
+// right; weights summed per dy, then over dy), identical to the CPU oracle and
+// the generic path.
 #include "gnlm_internal.h"
 
 namespace gnlm {
@@ -110,7 +111,13 @@ __global__ __launch_bounds__(NT) void nlm2d_fwd_tiled(Fwd2Args a)
     const int candbase = (R + cy0) * LX + (R + cx);
 
     int it = 0;
-    for (int dy = -a.rs; dy <= a.rs; ++dy)
+    for (int dy = -a.rs; dy <= a.rs; ++dy) {
+        float Zd[G::YR], Yd[G::YR];  // per-dy partial sums (two-level accumulation)
+#pragma unroll
+        for (int m = 0; m < G::YR; ++m) {
+            Zd[m] = 0.0f;
+            Yd[m] = 0.0f;
+        }
         for (int dx = -a.rs; dx <= a.rs; ++dx, ++it) {
             float *SX = SXB + (it & 1) * G::SXN;
             const int doff = dy * LX + dx;
@@ -146,10 +153,16 @@ __global__ __launch_bounds__(NT) void nlm2d_fwd_tiled(Fwd2Args a)
 #pragma unroll
                 for (int j = 0; j <= 2 * RP; ++j) S = S + col[m + j];
                 const float w = exp2_fast(S * a.k);
-                Zacc[m] = Zacc[m] + w;
-                Yacc[m] = fmaf(w, cand[m * LX], Yacc[m]);
+                Zd[m] = Zd[m] + w;
+                Yd[m] = fmaf(w, cand[m * LX], Yd[m]);
             }
         }
+#pragma unroll
+        for (int m = 0; m < G::YR; ++m) {
+            Zacc[m] = Zacc[m] + Zd[m];
+            Yacc[m] = Yacc[m] + Yd[m];
+        }
+    }
 
     const int gx = x0 + cx;
     if (gx < a.in.nx) {

@@ -45,6 +45,17 @@ struct BatchTile {
 
 __global__ void reset_counter(int *c) { *c = 0; }
 
+// Each (variant, r_p) tile body is a separate, non-inlined function: inlined
+// together into the one persistent kernel, their register demands add up and
+// push values into AGPRs inside the hot loops; called, each body keeps the
+// allocation it has in its per-volume kernel.
+template <int RP>
+__device__ __noinline__ void tile_slab2(const Fwd3Tile &a, float *lds) { fwd3_slab2_tile<RP>(a, lds, a.sigma); }
+template <int RP>
+__device__ __noinline__ void tile_slab(const Fwd3Tile &a, float *lds) { fwd3_slab_tile<RP>(a, lds, a.sigma); }
+template <class C>
+__device__ __noinline__ void tile_direct(const Fwd3Tile &a, float *lds) { fwd3_tile<C>(a, lds); }
+
 template <int RP>
 __device__ __forceinline__ void run_tile_rp(const BatchVol &v, const BatchTile &t, float *lds)
 {
@@ -60,23 +71,23 @@ __device__ __forceinline__ void run_tile_rp(const BatchVol &v, const BatchTile &
         a.x0 = t.tx * C::TX;
         a.y0 = t.ty * C::TY;
         a.z0 = t.tz * C::TZ;
-        fwd3_slab2_tile<RP>(a, lds, v.sigma);
+        tile_slab2<RP>(a, lds);
     } else if (v.variant == kVSlab) {
         using C = SlabCfg<RP>;
         a.x0 = t.tx * C::TX;
         a.y0 = t.ty * C::TY;
         a.z0 = t.tz * C::TZ;
-        fwd3_slab_tile<RP>(a, lds, v.sigma);
+        tile_slab<RP>(a, lds);
     } else if (v.variant == kV16x16x8) {
         a.x0 = t.tx * 16;
         a.y0 = t.ty * 16;
         a.z0 = t.tz * 8;
-        fwd3_tile<Fwd3Cfg<RP, 16, 16, 8>>(a, lds);
+        tile_direct<Fwd3Cfg<RP, 16, 16, 8>>(a, lds);
     } else {
         a.x0 = t.tx * 16;
         a.y0 = t.ty * 16;
         a.z0 = t.tz * 16;
-        fwd3_tile<Fwd3Cfg<RP, 16, 16, 16>>(a, lds);
+        tile_direct<Fwd3Cfg<RP, 16, 16, 16>>(a, lds);
     }
 }
 

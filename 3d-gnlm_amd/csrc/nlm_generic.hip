@@ -7,7 +7,8 @@
 // scratch volumes, then the weight is accumulated:
 //   pass X : s_x(x,y,z)  = sum_i fma(e,e)  on  x in [0,nx), y,z extended by r_p
 //   pass Y : s_xy(x,y,z) = sum_j s_x         on  y in [0,ny), z extended
-//   pass ZW: S = sum_k s_xy;  w = exp2(S k);  Z += w;  Y = fma(w, u~(x+d), Y)
+//   pass ZW: S = sum_k s_xy;  w = exp2(S k);  Zd += w;  Yd = fma(w, u~(x+d), Yd)
+//   after each dz (3D) / dy (2D): Z += Zd, Y += Yd (two-level accumulation)
 // The arithmetic order is the canonical one shared with the tiled kernels and
 // the CPU oracle, so all paths agree bit for bit on the patch distances.
 // The passes are HBM/L2-bound (about 30 B per voxel per offset) and each
@@ -24,6 +25,7 @@ struct GenArgs {
     float *sx;        // nx * (ny + 2rp) * (nz + 2rpz)
     float *sxy;       // nx * ny * (nz + 2rpz)
     float *Z, *Y;     // nx * ny * nz accumulators
+    float *Zd, *Yd;   // per-block partials (block = one dz in 3D, one dy in 2D)
     float k;
 };
 
@@ -78,8 +80,20 @@ __global__ void gen_pass_zw(GenArgs a)
         float S = 0.0f;
         for (int k = 0; k <= 2 * a.rpz; ++k) S = S + col[(long long)k * plane];
         const float w = exp2_fast(S * a.k);
-        a.Z[i] = a.Z[i] + w;
-        a.Y[i] = fmaf(w, load_in(a.in, x + a.dx, y + a.dy, z + a.dz), a.Y[i]);
+        a.Zd[i] = a.Zd[i] + w;
+        a.Yd[i] = fmaf(w, load_in(a.in, x + a.dx, y + a.dy, z + a.dz), a.Yd[i]);
+    }
+}
+
+// Close a block of the two-level accumulation: totals += partials, partials = 0.
+__global__ void gen_fold_block(GenArgs a, long long n)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        a.Z[i] = a.Z[i] + a.Zd[i];
+        a.Y[i] = a.Y[i] + a.Yd[i];
+        a.Zd[i] = 0.0f;
+        a.Yd[i] = 0.0f;
     }
 }
 
@@ -117,7 +131,7 @@ hipError_t fwd_generic_launch(const InDesc &in, float *out, float *Zout, int rp,
     const long long nsxy = nx * ny * (nz + 2 * rpz);
     const long long nv = nx * ny * nz;
     float *ws = nullptr;
-    hipError_t e = hipMalloc((void **)&ws, sizeof(float) * (size_t)(nsx + nsxy + 2 * nv));
+    hipError_t e = hipMalloc((void **)&ws, sizeof(float) * (size_t)(nsx + nsxy + 4 * nv));
     if (e != hipSuccess) return e;
     GenArgs a;
     a.in = in;
@@ -127,10 +141,14 @@ hipError_t fwd_generic_launch(const InDesc &in, float *out, float *Zout, int rp,
     a.sxy = ws + nsx;
     a.Z = a.sxy + nsxy;
     a.Y = a.Z + nv;
+    a.Zd = a.Y + nv;
+    a.Yd = a.Zd + nv;
     a.k = k;
-    hipLaunchKernelGGL(gen_fill0, dim3(grid_for(2 * nv)), dim3(256), 0, s, a.Z, 2 * nv);
-    for (int dz = -rsz; dz <= rsz; ++dz)
-        for (int dy = -rs; dy <= rs; ++dy)
+    hipLaunchKernelGGL(gen_fill0, dim3(grid_for(4 * nv)), dim3(256), 0, s, a.Z, 4 * nv);
+    // two-level accumulation: blocks are dz planes of offsets in 3D, dy rows in 2D
+    const bool is3d = rsz > 0;
+    for (int dz = -rsz; dz <= rsz; ++dz) {
+        for (int dy = -rs; dy <= rs; ++dy) {
             for (int dx = -rs; dx <= rs; ++dx) {
                 a.dx = dx;
                 a.dy = dy;
@@ -139,6 +157,10 @@ hipError_t fwd_generic_launch(const InDesc &in, float *out, float *Zout, int rp,
                 hipLaunchKernelGGL(gen_pass_y, dim3(grid_for(nsxy)), dim3(256), 0, s, a);
                 hipLaunchKernelGGL(gen_pass_zw, dim3(grid_for(nv)), dim3(256), 0, s, a);
             }
+            if (!is3d) hipLaunchKernelGGL(gen_fold_block, dim3(grid_for(nv)), dim3(256), 0, s, a, nv);
+        }
+        if (is3d) hipLaunchKernelGGL(gen_fold_block, dim3(grid_for(nv)), dim3(256), 0, s, a, nv);
+    }
     hipLaunchKernelGGL(gen_finish, dim3(grid_for(nv)), dim3(256), 0, s, a.Z, a.Y, out, Zout, nv);
     e = hipGetLastError();
     // The workspace is released once the stream has drained the passes.

@@ -31,8 +31,13 @@
  *   sxy = ((sx_{y-r} + sx_{y-r+1}) + ...) + sx_{y+r}
  *   S   = ((sxy_{z-r} + ...) + sxy_{z+r})
  *   w   = expf(-(S / P) / (h*h))            (P = (2r_p+1)^3, as main.cu:108)
- *   Z   = Z + w,  Y = fma(w, u(x+d), Y)     offsets in (dz, dy, dx) order
+ *   two-level accumulation: for each dz (3D) / dy (2D) a partial
+ *     Zd = Zd + w,  Yd = fma(w, u(x+d), Yd)  over the inner offsets in
+ *     (dy, dx) / (dx) order, then Z = Z + Zd, Y = Y + Yd
  *   out = Y / Z
+ * (Summing 1331 weights into one running FP32 total lets its rounding grow
+ * with the total; blocking by dz cuts that error about tenfold and makes the
+ * FP32 result agree with FP64 to ~1e-6.)
  * The file is compiled with -ffp-contract=off so that only the explicit
  * fmaf() calls fuse.
  */
@@ -108,7 +113,8 @@ void oracle_nlm3d_brute_f32(const float *u, int nx, int ny, int nz,
         for (int yy = 0; yy < ny; ++yy)
             for (int x = 0; x < nx; ++x) {
                 float Z = 0.0f, Y = 0.0f;
-                for (int dz = -rs; dz <= rs; ++dz)
+                for (int dz = -rs; dz <= rs; ++dz) {
+                    float Zd = 0.0f, Yd = 0.0f;
                     for (int dy = -rs; dy <= rs; ++dy)
                         for (int dx = -rs; dx <= rs; ++dx) {
                             float S = 0.0f;
@@ -126,9 +132,12 @@ void oracle_nlm3d_brute_f32(const float *u, int nx, int ny, int nz,
                                 S = S + sxy;
                             }
                             float w = expf(-(S / P) / hh);
-                            Z = Z + w;
-                            Y = fmaf(w, U32(x + dx, yy + dy, z + dz), Y);
+                            Zd = Zd + w;
+                            Yd = fmaf(w, U32(x + dx, yy + dy, z + dz), Yd);
                         }
+                    Z = Z + Zd;
+                    Y = Y + Yd;
+                }
                 long o = x + (long)nx * (yy + (long)ny * z);
                 y[o] = Y / Z;
                 if (Zout) Zout[o] = Z;
@@ -179,7 +188,8 @@ void oracle_nlm2d_brute_f32(const float *u, int nx, int ny, int rp, int rs,
     for (int yy = 0; yy < ny; ++yy)
         for (int x = 0; x < nx; ++x) {
             float Z = 0.0f, Y = 0.0f;
-            for (int dy = -rs; dy <= rs; ++dy)
+            for (int dy = -rs; dy <= rs; ++dy) {
+                float Zd = 0.0f, Yd = 0.0f;
                 for (int dx = -rs; dx <= rs; ++dx) {
                     float S = 0.0f;
                     for (int py = -rp; py <= rp; ++py) {
@@ -191,9 +201,12 @@ void oracle_nlm2d_brute_f32(const float *u, int nx, int ny, int rp, int rs,
                         S = S + sx;
                     }
                     float w = expf(-(S / P) / hh);
-                    Z = Z + w;
-                    Y = fmaf(w, U2(x + dx, yy + dy), Y);
+                    Zd = Zd + w;
+                    Yd = fmaf(w, U2(x + dx, yy + dy), Yd);
                 }
+                Z = Z + Zd;
+                Y = Y + Yd;
+            }
             long o = x + (long)nx * yy;
             y[o] = Y / Z;
             if (Zout) Zout[o] = Z;
@@ -239,7 +252,9 @@ int oracle_nlm3d_separable_f32(const float *u, int nx, int ny, int nz,
         float *sxy = (float *)malloc(sizeof(float) * wx * ny * wz);
         float *Zs = (float *)malloc(sizeof(float) * (long)nx * ny * SLAB);
         float *Ys = (float *)malloc(sizeof(float) * (long)nx * ny * SLAB);
-        if (!sx || !sxy || !Zs || !Ys) {
+        float *Zd = (float *)malloc(sizeof(float) * (long)nx * ny * SLAB);
+        float *Yd = (float *)malloc(sizeof(float) * (long)nx * ny * SLAB);
+        if (!sx || !sxy || !Zs || !Ys || !Zd || !Yd) {
 #pragma omp atomic write
             fail = 1;
         } else {
@@ -252,7 +267,9 @@ int oracle_nlm3d_separable_f32(const float *u, int nx, int ny, int nz,
                 const int tz = zb - za;
                 memset(Zs, 0, sizeof(float) * (long)nx * ny * SLAB);
                 memset(Ys, 0, sizeof(float) * (long)nx * ny * SLAB);
-                for (int dz = -rs; dz <= rs; ++dz)
+                for (int dz = -rs; dz <= rs; ++dz) {
+                    memset(Zd, 0, sizeof(float) * (long)nx * ny * SLAB);
+                    memset(Yd, 0, sizeof(float) * (long)nx * ny * SLAB);
                     for (int dy = -rs; dy <= rs; ++dy)
                         for (int dx = -rs; dx <= rs; ++dx) {
                             /* x-stage on rows y in [-rp, ny+rp), z in [za-rp, zb+rp) */
@@ -284,12 +301,12 @@ int oracle_nlm3d_separable_f32(const float *u, int nx, int ny, int nz,
                                         for (int x = 0; x < nx; ++x) o[x] = o[x] + b[x];
                                     }
                                 }
-                            /* z-stage, weights, accumulation */
+                            /* z-stage, weights, accumulation into the dz partials */
                             for (int kz = 0; kz < tz; ++kz)
                                 for (int jy = 0; jy < ny; ++jy) {
                                     const float *cand = pu + ex * (jy + R + dy + ey * (za + kz + R + dz)) + R + dx;
-                                    float *Zr = Zs + (long)nx * (jy + (long)ny * kz);
-                                    float *Yr = Ys + (long)nx * (jy + (long)ny * kz);
+                                    float *Zr = Zd + (long)nx * (jy + (long)ny * kz);
+                                    float *Yr = Yd + (long)nx * (jy + (long)ny * kz);
                                     for (int x = 0; x < nx; ++x) {
                                         float S = 0.0f;
                                         for (int pz = 0; pz <= 2 * rp; ++pz)
@@ -300,6 +317,11 @@ int oracle_nlm3d_separable_f32(const float *u, int nx, int ny, int nz,
                                     }
                                 }
                         }
+                    for (long i = 0; i < (long)nx * ny * tz; ++i) {
+                        Zs[i] = Zs[i] + Zd[i];
+                        Ys[i] = Ys[i] + Yd[i];
+                    }
+                }
                 for (int kz = 0; kz < tz; ++kz)
                     for (int jy = 0; jy < ny; ++jy)
                         for (int x = 0; x < nx; ++x) {
@@ -308,7 +330,7 @@ int oracle_nlm3d_separable_f32(const float *u, int nx, int ny, int nz,
                         }
             }
         }
-        free(sx); free(sxy); free(Zs); free(Ys);
+        free(sx); free(sxy); free(Zs); free(Ys); free(Zd); free(Yd);
     }
     free(pu);
     return fail ? -1 : 0;
@@ -441,7 +463,9 @@ int oracle_nlm2d_separable_f32(const float *u, int nx, int ny, int rp, int rs,
         float *sx = (float *)malloc(sizeof(float) * nx * (BAND + 2 * rp));
         float *Zs = (float *)malloc(sizeof(float) * nx * BAND);
         float *Ys = (float *)malloc(sizeof(float) * nx * BAND);
-        if (!sx || !Zs || !Ys) {
+        float *Zd = (float *)malloc(sizeof(float) * nx * BAND);
+        float *Yd = (float *)malloc(sizeof(float) * nx * BAND);
+        if (!sx || !Zs || !Ys || !Zd || !Yd) {
 #pragma omp atomic write
             fail = 1;
         } else {
@@ -454,7 +478,9 @@ int oracle_nlm2d_separable_f32(const float *u, int nx, int ny, int rp, int rs,
                 const int ty = yb - ya;
                 memset(Zs, 0, sizeof(float) * nx * BAND);
                 memset(Ys, 0, sizeof(float) * nx * BAND);
-                for (int dy = -rs; dy <= rs; ++dy)
+                for (int dy = -rs; dy <= rs; ++dy) {
+                    memset(Zd, 0, sizeof(float) * nx * BAND);
+                    memset(Yd, 0, sizeof(float) * nx * BAND);
                     for (int dx = -rs; dx <= rs; ++dx) {
                         for (int jy = 0; jy < ty + 2 * rp; ++jy) {
                             const long yy = ya - rp + jy + R;
@@ -472,8 +498,8 @@ int oracle_nlm2d_separable_f32(const float *u, int nx, int ny, int rp, int rs,
                         }
                         for (int jy = 0; jy < ty; ++jy) {
                             const float *cand = pu + ex * (ya + jy + R + dy) + R + dx;
-                            float *Zr = Zs + (long)nx * jy;
-                            float *Yr = Ys + (long)nx * jy;
+                            float *Zr = Zd + (long)nx * jy;
+                            float *Yr = Yd + (long)nx * jy;
                             for (int x = 0; x < nx; ++x) {
                                 float S = 0.0f;
                                 for (int py = 0; py <= 2 * rp; ++py)
@@ -484,12 +510,17 @@ int oracle_nlm2d_separable_f32(const float *u, int nx, int ny, int rp, int rs,
                             }
                         }
                     }
+                    for (long i = 0; i < (long)nx * ty; ++i) {
+                        Zs[i] = Zs[i] + Zd[i];
+                        Ys[i] = Ys[i] + Yd[i];
+                    }
+                }
                 for (int jy = 0; jy < ty; ++jy)
                     for (int x = 0; x < nx; ++x)
                         y[x + (long)nx * (ya + jy)] = Ys[x + (long)nx * jy] / Zs[x + (long)nx * jy];
             }
         }
-        free(sx); free(Zs); free(Ys);
+        free(sx); free(Zs); free(Ys); free(Zd); free(Yd);
     }
     free(pu);
     return fail ? -1 : 0;
