@@ -13,48 +13,63 @@
 // the terms that land on x+d and using S_{-d}(x+d) = S_d(x).  Then
 //   dL/dh = (2c/h) sum_{x in volume} sum_d w_d(x) S_d(x) (G(x) u~(x+d) - GY(x))
 // and dL/du(i) = sum of grad(z) over the positions z of the extended domain
-// that reflect-101 maps onto voxel i.
+// [-R, n+R) (R = r_p + r_s) that reflect-101 maps onto voxel i.
 //
 // Nothing of size voxels x offsets is stored: patch distances are recomputed
-// per offset from the input, and only y and Z from the forward are read.
+// per offset, and only y and Z from the forward are read (as G and GY,
+// formed once by a prologue kernel).
 //
-// Kernel structure: one workgroup (256 threads) per 16x8x8 block of gradient
-// positions of the extended domain [-R, n+R)^3 (R = r_p + r_s).  The block's
-// u~ (halo 2 r_p + r_s) and (G, GY) (halo r_p + r_s) are staged in LDS once;
-// for each offset six short phases separated by barriers form S on the block
-// grown by r_p, the weights and E, the patch box of E, and accumulate the
-// gradient in registers (4 positions per thread).  All sums are direct, in
-// the canonical order (x, then y, then z), so S matches the forward exactly.
-// Each thread owns fixed positions for the whole offset loop and the per-
-// block dL/dh partials are reduced in a fixed order: results are
-// deterministic.
+// Kernel: one workgroup (256 threads) per gradient tile T of
+// (16 - 2 r_p) x (16 - 2 r_p) x 8 positions of the extended domain.  The tile
+// grown by r_p, Q = 16 x 16 x (8 + 2 r_p), is where S, w and E live, so one
+// thread owns one (x, y) column of Q.  The tile's u~ block (halo 2 r_p + r_s)
+// is staged in LDS once; G, GY and u~ of a thread's own Q column sit in
+// registers; G and GY at shifted positions are read through the L1/L2 cache.
+// Per offset:
+//   S1  x sums of squared differences on rows of Q grown by r_p in y and z
+//       (one row per thread, centre samples in registers)        -> LDS
+//   S2  per Q column: y sums, z sums -> S; w = exp2(S k); E; dL/dh terms
+//       on the tile's own volume voxels                           -> LDS (E, w)
+//   E1  x sums of E on rows of Q, outputs on T's x range          -> LDS
+//   E2  per T column: y and z sums of E, gradient accumulation in registers
+// with three barriers.  All sums are direct, in the CPU oracle's order, so S
+// matches the forward's direct-sum variants exactly; every thread owns fixed
+// positions and the dL/dh partials are reduced in a fixed order, so the
+// results are deterministic.
 #include "gnlm_internal.h"
 
 namespace gnlm {
 namespace {
 
 constexpr int NT = 256;
-constexpr int BX = 16, BY = 8, BZ = 8;   // gradient positions per block
-constexpr int PER = BX * BY * BZ / NT;    // positions per thread (4)
+
+template <int RP_>
+struct B2 {
+    static constexpr int RP = RP_;
+    static constexpr int QX = 16, QY = 16;            // grown tile, x and y
+    static constexpr int TX = QX - 2 * RP, TY = QY - 2 * RP;
+    static constexpr int TZ = 8;
+    static constexpr int QZ = TZ + 2 * RP;
+    static constexpr int DX = QX + 2 * RP;            // samples per S1 row
+    static constexpr int DY = QY + 2 * RP, DZ = QZ + 2 * RP;
+    static constexpr int NR1 = DY * DZ;               // S1 rows
+    static constexpr int ROWS1 = (NR1 + NT - 1) / NT;
+    static constexpr int NR1P = NR1 | 1;              // odd pitch of the s_x buffer
+    static constexpr int NRE = QY * QZ;               // E1 rows
+    static constexpr int NREP = NRE | 1;
+    static constexpr int QN = QX * QY * QZ;           // floats in E / W buffers
+};
 
 struct BwdArgs {
     const float *u;     // unpadded input
-    const float *y;     // forward output
-    const float *Z;     // forward normaliser
-    const float *g;     // dL/dy
+    const float *GG;    // (G, GY) interleaved per voxel
     float *gext;        // gradient on the extended domain, (n+2R)^3
     float *dh_part;     // one partial per block
     int nx, ny, nz;
-    int rp, rs;
+    int rs;
     float k;            // exp2 scale: w = exp2(S * k)
     float negc;         // -c
     int nbx, nby, nbz;  // blocks per axis over the extended domain
-};
-
-struct BwdLds {
-    // region sizes (elements per axis)
-    int ux, uy, uz;     // u~ block: B + 2(2rp+rs)
-    int gx, gy, gz;     // (G, GY) block: B + 2(rp+rs)
 };
 
 __device__ __forceinline__ bool in_vol(int x, int y, int z, int nx, int ny, int nz)
@@ -62,164 +77,213 @@ __device__ __forceinline__ bool in_vol(int x, int y, int z, int nx, int ny, int 
     return (unsigned)x < (unsigned)nx && (unsigned)y < (unsigned)ny && (unsigned)z < (unsigned)nz;
 }
 
+__device__ __forceinline__ float2 load_gg(const BwdArgs &a, int x, int y, int z)
+{
+    if (!in_vol(x, y, z, a.nx, a.ny, a.nz)) return make_float2(0.0f, 0.0f);
+    const long long o = x + (long long)a.nx * (y + (long long)a.ny * z);
+    return *reinterpret_cast<const float2 *>(a.GG + 2 * o);
+}
+
+template <int RP>
+size_t bwd_lds_floats(int rs)
+{
+    using C = B2<RP>;
+    const int HU = 2 * RP + rs;  // u~ halo around T
+    const size_t lu = (size_t)((C::TX + 2 * HU) | 1) * (C::TY + 2 * HU) * (C::TZ + 2 * HU);
+    size_t n = lu + (size_t)C::QX * C::NR1P + 2 * (size_t)C::QN + (size_t)C::TX * C::NREP;
+    if (n < NT) n = NT;
+    return n;
+}
+
+template <int RP>
 __global__ __launch_bounds__(NT) void nlm3d_bwd_tiled(BwdArgs a)
 {
+    using C = B2<RP>;
     extern __shared__ float lds[];
-    const int rp = a.rp, rs = a.rs;
-    const int HU = 2 * rp + rs, HG = rp + rs;
-    const int ux = BX + 2 * HU, uy = BY + 2 * HU, uz = BZ + 2 * HU;
-    const int gx = BX + 2 * HG, gy = BY + 2 * HG, gz = BZ + 2 * HG;
-    // work regions (grown block): Q = B + 2rp per axis
-    const int qx = BX + 2 * rp, qy = BY + 2 * rp, qz = BZ + 2 * rp;
-    const int dy2 = BY + 4 * rp, dz2 = BZ + 4 * rp;  // rows needed by the x-stage
+    const int rs = a.rs;
+    const int HU = 2 * RP + rs;
+    const int LXr = C::TX + 2 * HU;
+    const int LX = LXr | 1, LY = C::TY + 2 * HU, LZ = C::TZ + 2 * HU;
+    const int LXY = LX * LY;
+    float *U = lds;                               // u~ block around T, halo HU
+    float *SX = U + LXY * LZ;                     // S1 output: [x_q][row] pitch NR1P
+    float *EB = SX + C::QX * C::NR1P;             // E on Q: [z][y][x]
+    float *WB = EB + C::QN;                       // w on Q: [z][y][x]
+    float *EX = WB + C::QN;                       // E1 output: [x_t][row] pitch NREP
 
-    float *U = lds;                               // ux*uy*uz
-    float *GG = U + ux * uy * uz;                 // 2*gx*gy*gz  (G, GY interleaved)
-    float *SX = GG + 2 * gx * gy * gz;            // qx * dy2 * dz2
-    float *SXY = SX + qx * dy2 * dz2;             // qx * qy * dz2
-    float *W = SXY + qx * qy * dz2;               // qx*qy*qz
-    float *EE = W + qx * qy * qz;                 // qx*qy*qz
-    float *EX = EE + qx * qy * qz;                // BX * qy * qz
-    float *EXY = EX + BX * qy * qz;               // BX * BY * qz
-
-    const int R = rp + rs;
+    const int R = RP + rs;
     const int b = blockIdx.x;
-    const int X0 = (b % a.nbx) * BX - R;
-    const int Y0 = ((b / a.nbx) % a.nby) * BY - R;
-    const int Z0 = (b / (a.nbx * a.nby)) * BZ - R;
+    // T origin in volume coordinates; the extended domain starts at -R
+    const int X0 = (b % a.nbx) * C::TX - R;
+    const int Y0 = ((b / a.nbx) % a.nby) * C::TY - R;
+    const int Z0 = (b / (a.nbx * a.nby)) * C::TZ - R;
     const int tid = threadIdx.x;
 
-    // ---- stage u~ ----
-    for (int i = tid; i < ux * uy * uz; i += NT) {
-        const int lx = i % ux, r = i / ux, ly = r % uy, lz = r / uy;
-        const int x = reflect101(X0 - HU + lx, a.nx), y = reflect101(Y0 - HU + ly, a.ny),
-                  z = reflect101(Z0 - HU + lz, a.nz);
-        U[i] = a.u[x + (long long)a.nx * (y + (long long)a.ny * z)];
-    }
-    // ---- stage G = g/Z, GY = g*y/Z (zero outside the volume) ----
-    for (int i = tid; i < gx * gy * gz; i += NT) {
-        const int lx = i % gx, r = i / gx, ly = r % gy, lz = r / gy;
-        const int x = X0 - HG + lx, y = Y0 - HG + ly, z = Z0 - HG + lz;
-        float G = 0.0f, GY = 0.0f;
-        if (in_vol(x, y, z, a.nx, a.ny, a.nz)) {
-            const long long o = x + (long long)a.nx * (y + (long long)a.ny * z);
-            G = a.g[o] / a.Z[o];
-            GY = G * a.y[o];
+    // ---- stage u~ around T ----
+    for (int i = tid; i < LXY * LZ; i += NT) {
+        const int lx = i % LX, r = i / LX, ly = r % LY, lz = r / LY;
+        float v = 0.0f;
+        if (lx < LXr) {
+            const int x = reflect101(X0 - HU + lx, a.nx), y = reflect101(Y0 - HU + ly, a.ny),
+                      z = reflect101(Z0 - HU + lz, a.nz);
+            v = a.u[x + (long long)a.nx * (y + (long long)a.ny * z)];
         }
-        GG[2 * i] = G;
-        GG[2 * i + 1] = GY;
+        U[i] = v;
     }
     __syncthreads();
+    // LDS index of T-local (x, y, z) (T-local 0 = X0)
+    auto uidx = [&](int x, int y, int z) { return (x + HU) + LX * ((y + HU) + LY * (z + HU)); };
 
-    // fixed ownership of gradient positions: p = tid + j*NT
-    float gacc[PER];
+    // ---- S1 rows: D-row (y_q in [-RP, QY+RP), z_q in [-RP, QZ+RP)) of Q;
+    //      Q-local coordinate q = T-local + RP ----
+    float cen[C::ROWS1][C::DX];
+    int rowbase[C::ROWS1];
 #pragma unroll
-    for (int j = 0; j < PER; ++j) gacc[j] = 0.0f;
-    float dh = 0.0f;
+    for (int j = 0; j < C::ROWS1; ++j) {
+        const int r = tid + j * NT;
+        const int ry = r % C::DY, rz = r / C::DY;
+        // samples x_q in [-RP, QX+RP) -> T-local x = x_q - RP in [-2RP, ...)
+        rowbase[j] = uidx(-2 * RP, ry - 2 * RP, rz - 2 * RP);
+        if (r < C::NR1) {
+#pragma unroll
+            for (int i = 0; i < C::DX; ++i) cen[j][i] = U[rowbase[j] + i];
+        }
+    }
 
-    // index helpers: U index of block-local (x,y,z) (block-local 0 = X0)
-    auto uidx = [&](int x, int y, int z) { return (x + HU) + ux * ((y + HU) + uy * (z + HU)); };
-    auto gidx = [&](int x, int y, int z) { return 2 * ((x + HG) + gx * ((y + HG) + gy * (z + HG))); };
+    // ---- S2 ownership: Q column (qx, qy); own G, GY, u~ for z_q in [0, QZ) ----
+    const int qx = tid % C::QX, qy = tid / C::QX;
+    const int cxT = qx - RP, cyT = qy - RP;  // T-local coordinates of the column
+    float Gc[C::QZ], GYc[C::QZ], uc[C::QZ];
+#pragma unroll
+    for (int z = 0; z < C::QZ; ++z) {
+        const int zT = z - RP;
+        const float2 g = load_gg(a, X0 + cxT, Y0 + cyT, Z0 + zT);
+        Gc[z] = g.x;
+        GYc[z] = g.y;
+        uc[z] = U[uidx(cxT, cyT, zT)];
+    }
+    // is the column's position x of T (and of the volume) for dL/dh?
+    const bool col_in_T = (cxT >= 0 && cxT < C::TX && cyT >= 0 && cyT < C::TY);
+
+    // ---- E2 ownership: T column (tx, ty) if tid < TX*TY ----
+    const bool e2_active = tid < C::TX * C::TY;
+    const int tx = tid % C::TX, ty = tid / C::TX;
+    float gacc[C::TZ];
+#pragma unroll
+    for (int z = 0; z < C::TZ; ++z) gacc[z] = 0.0f;
+    float dh = 0.0f;
 
     for (int dz = -rs; dz <= rs; ++dz)
         for (int dy = -rs; dy <= rs; ++dy)
             for (int dx = -rs; dx <= rs; ++dx) {
-                const int du = dx + ux * (dy + uy * dz);
-                const int dg = 2 * (dx + gx * (dy + gy * dz));
-                // A1: x-stage of squared differences over x in Q, y,z in B+4rp
-                for (int i = tid; i < qx * dy2 * dz2; i += NT) {
-                    const int lx = i % qx, r = i / qx, ly = r % dy2, lz = r / dy2;
-                    const int c = uidx(lx - rp - rp, ly - 2 * rp, lz - 2 * rp);
-                    float acc = 0.0f;
-                    for (int p = 0; p <= 2 * rp; ++p) {
-                        const float e = U[c + p] - U[c + p + du];
-                        acc = fmaf(e, e, acc);
-                    }
-                    SX[i] = acc;
-                }
-                __syncthreads();
-                // A2: y-stage over x,y in Q, z in B+4rp
-                for (int i = tid; i < qx * qy * dz2; i += NT) {
-                    const int lx = i % qx, r = i / qx, ly = r % qy, lz = r / qy;
-                    const float *col = SX + lx + qx * (ly + dy2 * lz);
-                    float s = 0.0f;
-                    for (int p = 0; p <= 2 * rp; ++p) s = s + col[qx * p];
-                    SXY[i] = s;
-                }
-                __syncthreads();
-                // A3: S, w, E on Q; dL/dh terms on the block's own volume voxels
-                for (int i = tid; i < qx * qy * qz; i += NT) {
-                    const int lx = i % qx, r = i / qx, ly = r % qy, lz = r / qy;
-                    const float *col = SXY + lx + qx * (ly + qy * lz);
-                    float S = 0.0f;
-                    for (int p = 0; p <= 2 * rp; ++p) S = S + col[qx * qy * p];
-                    const float w = exp2_fast(S * a.k);
-                    const int x = lx - rp, y = ly - rp, z = lz - rp;  // block-local
-                    const int ci = uidx(x, y, z);
-                    const int gi = gidx(x, y, z);
-                    const float uc = U[ci], ud = U[ci + du];
-                    const float t1 = GG[gi] * ud - GG[gi + 1];
-                    const float t2 = GG[gi + dg] * uc - GG[gi + dg + 1];
-                    W[i] = w;
-                    EE[i] = a.negc * w * (t1 + t2);
-                }
-                __syncthreads();
-                // A4: x-stage of E over x in B, y,z in Q
-                for (int i = tid; i < BX * qy * qz; i += NT) {
-                    const int lx = i % BX, r = i / BX, ly = r % qy, lz = r / qy;
-                    const float *row = EE + lx + qx * (ly + qy * lz);
-                    float s = 0.0f;
-                    for (int p = 0; p <= 2 * rp; ++p) s = s + row[p];
-                    EX[i] = s;
-                }
-                __syncthreads();
-                // A5: y-stage of E over x,y in B, z in Q
-                for (int i = tid; i < BX * BY * qz; i += NT) {
-                    const int lx = i % BX, r = i / BX, ly = r % BY, lz = r / BY;
-                    const float *col = EX + lx + BX * (ly + qy * lz);
-                    float s = 0.0f;
-                    for (int p = 0; p <= 2 * rp; ++p) s = s + col[BX * p];
-                    EXY[i] = s;
-                }
-                __syncthreads();
-                // A6: z-stage of E and gradient accumulation at owned positions
+                const int du = dx + LX * (dy + LY * dz);
+                // ---- S1: x sums of squared differences ----
 #pragma unroll
-                for (int j = 0; j < PER; ++j) {
-                    const int i = tid + j * NT;
-                    const int lx = i % BX, r = i / BX, ly = r % BY, lz = r / BY;
-                    const float *col = EXY + lx + BX * (ly + BY * lz);
-                    float bE = 0.0f;
-                    for (int p = 0; p <= 2 * rp; ++p) bE = bE + col[BX * BY * p];
-                    const int wi = (lx + rp) + qx * ((ly + rp) + qy * (lz + rp));
-                    const int ci = uidx(lx, ly, lz);
-                    const int gi = gidx(lx, ly, lz);
-                    const float w = W[wi];
-                    gacc[j] = fmaf(GG[gi + dg], w, gacc[j]);
-                    gacc[j] = fmaf(2.0f * bE, U[ci] - U[ci + du], gacc[j]);
-                    // dL/dh: x = this position, if it is a volume voxel
-                    const int x = X0 + lx, y = Y0 + ly, z = Z0 + lz;
-                    if (in_vol(x, y, z, a.nx, a.ny, a.nz)) {
-                        // recompute S at x exactly as A3 did for the grown-block
-                        // position (lx+rp, ly+rp, lz+rp): SXY planes lz+rp .. lz+3rp
-                        const float *c2 = SXY + (lx + rp) + qx * ((ly + rp) + qy * (lz + rp));
-                        float S = 0.0f;
-                        for (int p = 0; p <= 2 * rp; ++p) S = S + c2[qx * qy * p];
-                        const float t1 = GG[gi] * U[ci + du] - GG[gi + 1];
-                        dh = fmaf(w * S, t1, dh);
+                for (int j = 0; j < C::ROWS1; ++j) {
+                    const int r = tid + j * NT;
+                    if (r < C::NR1) {
+                        const float *sh = U + rowbase[j] + du;
+                        float e[C::DX];
+#pragma unroll
+                        for (int i = 0; i < C::DX; ++i) e[i] = cen[j][i] - sh[i];
+#pragma unroll
+                        for (int xq = 0; xq < C::QX; ++xq) {
+                            float acc = 0.0f;
+#pragma unroll
+                            for (int i = 0; i <= 2 * RP; ++i) acc = fmaf(e[xq + i], e[xq + i], acc);
+                            SX[xq * C::NR1P + r] = acc;
+                        }
                     }
                 }
-                // A1 of the next offset writes SX only; the buffers read here
-                // (W, EXY, SXY) are rewritten after at least one more barrier.
+                __syncthreads();
+
+                // ---- S2: y and z sums on the Q column, w, E, dL/dh ----
+                {
+                    float sxy[C::DZ];
+#pragma unroll
+                    for (int pz = 0; pz < C::DZ; ++pz) {
+                        const float *col = SX + qx * C::NR1P + pz * C::DY + qy;
+                        float s = 0.0f;
+#pragma unroll
+                        for (int j = 0; j <= 2 * RP; ++j) s = s + col[j];
+                        sxy[pz] = s;
+                    }
+                    const float *ush = U + uidx(cxT + dx, cyT + dy, -RP + dz);
+#pragma unroll
+                    for (int z = 0; z < C::QZ; ++z) {
+                        float S = 0.0f;
+#pragma unroll
+                        for (int kk = 0; kk <= 2 * RP; ++kk) S = S + sxy[z + kk];
+                        const float w = exp2_fast(S * a.k);
+                        const float ud = ush[z * LXY];
+                        const float2 gd = load_gg(a, X0 + cxT + dx, Y0 + cyT + dy, Z0 + z - RP + dz);
+                        const float t1 = Gc[z] * ud - GYc[z];
+                        const float t2 = gd.x * uc[z] - gd.y;
+                        const int qi = qx + C::QX * (qy + C::QY * z);
+                        WB[qi] = w;
+                        EB[qi] = a.negc * w * (t1 + t2);
+                        const int zT = z - RP;
+                        if (col_in_T && zT >= 0 && zT < C::TZ &&
+                            in_vol(X0 + cxT, Y0 + cyT, Z0 + zT, a.nx, a.ny, a.nz))
+                            dh = fmaf(w * S, t1, dh);
+                    }
+                }
+                __syncthreads();
+
+                // ---- E1: x sums of E over rows of Q, outputs on T's x range ----
+                for (int r = tid; r < C::NRE; r += NT) {
+                    const float *row = EB + C::QX * r;  // row r = (y_q, z_q), x_q fastest
+                    float ev[C::QX];
+#pragma unroll
+                    for (int i = 0; i < C::QX; ++i) ev[i] = row[i];
+#pragma unroll
+                    for (int xt = 0; xt < C::TX; ++xt) {
+                        float s = 0.0f;
+#pragma unroll
+                        for (int i = 0; i <= 2 * RP; ++i) s = s + ev[xt + i];
+                        EX[xt * C::NREP + r] = s;
+                    }
+                }
+                __syncthreads();
+
+                // ---- E2: y and z sums of E on the T column, accumulate ----
+                if (e2_active) {
+                    float exy[C::QZ];
+#pragma unroll
+                    for (int zq = 0; zq < C::QZ; ++zq) {
+                        const float *col = EX + tx * C::NREP + zq * C::QY + ty;
+                        float s = 0.0f;
+#pragma unroll
+                        for (int j = 0; j <= 2 * RP; ++j) s = s + col[j];
+                        exy[zq] = s;
+                    }
+                    const float *u0 = U + uidx(tx, ty, 0);
+#pragma unroll
+                    for (int z = 0; z < C::TZ; ++z) {
+                        float bE = 0.0f;
+#pragma unroll
+                        for (int kk = 0; kk <= 2 * RP; ++kk) bE = bE + exy[z + kk];
+                        const float w = WB[(tx + RP) + C::QX * ((ty + RP) + C::QY * (z + RP))];
+                        const float2 gd = load_gg(a, X0 + tx + dx, Y0 + ty + dy, Z0 + z + dz);
+                        const float uz = u0[z * LXY], uzd = u0[z * LXY + du];
+                        gacc[z] = fmaf(gd.x, w, gacc[z]);
+                        gacc[z] = fmaf(2.0f * bE, uz - uzd, gacc[z]);
+                    }
+                }
+                // The next offset's S1 writes SX only (read in S2 before the
+                // second barrier); EB/WB are rewritten in S2 after the next
+                // S1 barrier, EX in E1 after the next S2 barrier.
             }
 
-    // ---- write the gradient of the block's extended-domain positions ----
-    const long long ex = a.nx + 2LL * R, ey = a.ny + 2LL * R, ez = a.nz + 2LL * R;
+    // ---- write the gradient of the tile's extended-domain positions ----
+    if (e2_active) {
+        const long long ex = a.nx + 2LL * R, ey = a.ny + 2LL * R, ez = a.nz + 2LL * R;
+        const long long x = X0 + tx + R, y = Y0 + ty + R;
 #pragma unroll
-    for (int j = 0; j < PER; ++j) {
-        const int i = tid + j * NT;
-        const int lx = i % BX, r = i / BX, ly = r % BY, lz = r / BY;
-        const long long x = X0 + lx + R, y = Y0 + ly + R, z = Z0 + lz + R;
-        if (x < ex && y < ey && z < ez) a.gext[x + ex * (y + ey * z)] = gacc[j];
+        for (int z = 0; z < C::TZ; ++z) {
+            const long long zz = Z0 + z + R;
+            if (x < ex && y < ey && zz < ez) a.gext[x + ex * (y + ey * zz)] = gacc[z];
+        }
     }
     // ---- deterministic block reduction of dL/dh ----
     __syncthreads();
@@ -231,6 +295,17 @@ __global__ __launch_bounds__(NT) void nlm3d_bwd_tiled(BwdArgs a)
         __syncthreads();
     }
     if (tid == 0) a.dh_part[b] = red[0];
+}
+
+// G = g / Z and GY = g * y / Z, interleaved.
+__global__ void make_gg(const float *g, const float *y, const float *Z, float *GG, long long n)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const float G = g[i] / Z[i];
+        GG[2 * i] = G;
+        GG[2 * i + 1] = G * y[i];
+    }
 }
 
 // dL/du(i) = sum of gext over the preimages of i under reflect-101 within the
@@ -293,61 +368,84 @@ __global__ void reduce_dh(const float *part, int n, float scale, float *out)
     if (threadIdx.x == 0) *out = red[0] * scale;
 }
 
-size_t bwd_lds_bytes(int rp, int rs)
+size_t bwd_lds_bytes_rp(int rp, int rs)
 {
-    const int HU = 2 * rp + rs, HG = rp + rs;
-    const size_t ux = BX + 2 * HU, uy = BY + 2 * HU, uz = BZ + 2 * HU;
-    const size_t gx = BX + 2 * HG, gy = BY + 2 * HG, gz = BZ + 2 * HG;
-    const size_t qx = BX + 2 * rp, qy = BY + 2 * rp, qz = BZ + 2 * rp;
-    const size_t dy2 = BY + 4 * rp, dz2 = BZ + 4 * rp;
-    size_t n = ux * uy * uz + 2 * gx * gy * gz + qx * dy2 * dz2 + qx * qy * dz2 + 2 * qx * qy * qz +
-               BX * qy * qz + BX * BY * qz;
-    if (n < NT) n = NT;
-    return n * sizeof(float);
+    switch (rp) {
+    case 0: return bwd_lds_floats<0>(rs) * sizeof(float);
+    case 1: return bwd_lds_floats<1>(rs) * sizeof(float);
+    case 2: return bwd_lds_floats<2>(rs) * sizeof(float);
+    default: return (size_t)-1;
+    }
+}
+
+template <int RP>
+hipError_t launch_bwd(BwdArgs a, size_t lds, hipStream_t s)
+{
+    using C = B2<RP>;
+    a.nbx = (int)((a.nx + 2LL * (RP + a.rs) + C::TX - 1) / C::TX);
+    a.nby = (int)((a.ny + 2LL * (RP + a.rs) + C::TY - 1) / C::TY);
+    a.nbz = (int)((a.nz + 2LL * (RP + a.rs) + C::TZ - 1) / C::TZ);
+    hipError_t e = hipFuncSetAttribute((const void *)nlm3d_bwd_tiled<RP>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(nlm3d_bwd_tiled<RP>, dim3(a.nbx * a.nby * a.nbz), dim3(NT), lds, s, a);
+    return hipGetLastError();
+}
+
+template <int RP>
+int bwd_blocks(int nx, int ny, int nz, int rs)
+{
+    using C = B2<RP>;
+    const long long R = RP + rs;
+    return (int)(((nx + 2 * R + C::TX - 1) / C::TX) * ((ny + 2 * R + C::TY - 1) / C::TY) *
+                 ((nz + 2 * R + C::TZ - 1) / C::TZ));
 }
 
 }  // namespace
 
 bool bwd3d_supported(int r_p, int r_s)
 {
-    return r_p >= 0 && r_s >= 0 && bwd_lds_bytes(r_p, r_s) <= 160 * 1024;
+    return r_p >= 0 && r_p <= 2 && r_s >= 0 && bwd_lds_bytes_rp(r_p, r_s) <= kLdsLimitBytes;
 }
 
 hipError_t bwd3d_launch(const float *u, const float *y, const float *Z, const float *g, int nx, int ny, int nz,
                         int rp, int rs, float h, float *gu, float *gh, hipStream_t s)
 {
     const int R = rp + rs;
+    const long long nv = (long long)nx * ny * nz;
     const long long ext = (nx + 2LL * R) * (ny + 2LL * R) * (nz + 2LL * R);
+    const int nblocks = rp == 0 ? bwd_blocks<0>(nx, ny, nz, rs)
+                                : (rp == 1 ? bwd_blocks<1>(nx, ny, nz, rs) : bwd_blocks<2>(nx, ny, nz, rs));
     BwdArgs a;
     a.u = u;
-    a.y = y;
-    a.Z = Z;
-    a.g = g;
     a.nx = nx;
     a.ny = ny;
     a.nz = nz;
-    a.rp = rp;
     a.rs = rs;
     const int P = (2 * rp + 1) * (2 * rp + 1) * (2 * rp + 1);
     const double c = 1.0 / ((double)P * (double)h * (double)h);
     a.k = (float)weight_scale(P, h);
     a.negc = (float)(-c);
-    a.nbx = (int)((nx + 2LL * R + BX - 1) / BX);
-    a.nby = (int)((ny + 2LL * R + BY - 1) / BY);
-    a.nbz = (int)((nz + 2LL * R + BZ - 1) / BZ);
-    const int nblocks = a.nbx * a.nby * a.nbz;
     float *ws = nullptr;
-    hipError_t e = hipMallocAsync((void **)&ws, sizeof(float) * (size_t)(ext + nblocks), s);
+    hipError_t e = hipMallocAsync((void **)&ws, sizeof(float) * (size_t)(ext + nblocks + 2 * nv + 2), s);
     if (e != hipSuccess) return e;
     a.gext = ws;
     a.dh_part = ws + ext;
-    const size_t lds = bwd_lds_bytes(rp, rs);
-    e = hipFuncSetAttribute((const void *)nlm3d_bwd_tiled, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    float *GG = a.dh_part + nblocks;
+    // keep the interleaved (G, GY) 8-byte aligned
+    if (reinterpret_cast<uintptr_t>(GG) & 7) GG += 1;
+    a.GG = GG;
+    long long gblk = (nv + 255) / 256;
+    if (gblk > 65536) gblk = 65536;
+    hipLaunchKernelGGL(make_gg, dim3((unsigned)gblk), dim3(256), 0, s, g, y, Z, GG, nv);
+    const size_t lds = bwd_lds_bytes_rp(rp, rs);
+    if (rp == 0)
+        e = launch_bwd<0>(a, lds, s);
+    else if (rp == 1)
+        e = launch_bwd<1>(a, lds, s);
+    else
+        e = launch_bwd<2>(a, lds, s);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(nlm3d_bwd_tiled, dim3(nblocks), dim3(NT), lds, s, a);
-        long long nv = (long long)nx * ny * nz;
-        long long gblk = (nv + 255) / 256;
-        if (gblk > 65536) gblk = 65536;
         hipLaunchKernelGGL(fold_grad, dim3((unsigned)gblk), dim3(256), 0, s, (const float *)a.gext, gu, nx, ny, nz, R);
         if (gh) hipLaunchKernelGGL(reduce_dh, dim3(1), dim3(256), 0, s, (const float *)a.dh_part, nblocks,
                                    (float)(2.0 * c / (double)h), gh);

@@ -90,8 +90,8 @@ int gnlm2d_forward_ex(const float *d_in, int in_pad, long long in_sy, float *d_o
  * writes d_gin = dL/d(input) (reflect-101 copies folded back onto their
  * source voxels) and, if d_gh is not NULL, one float dL/dh.  Patch distances
  * are recomputed; nothing of size voxels x offsets is stored.  Results are
- * deterministic.  Returns GNLM_EUNSUPPORTED when the tile's LDS image for
- * (r_p, r_s) exceeds 160 KiB (r_p = 1 supports r_s <= 4). */
+ * deterministic.  Returns GNLM_EUNSUPPORTED for r_p > 2 or when the tile's
+ * LDS image exceeds 160 KiB (r_p = 1: r_s <= 7; r_p = 2: r_s <= 5). */
 int gnlm3d_backward(const float *d_in, const float *d_out, const float *d_Z, const float *d_gout, int nx,
                     int ny, int nz, int r_p, int r_s, float h, float *d_gin, float *d_gh, void *stream);
 
