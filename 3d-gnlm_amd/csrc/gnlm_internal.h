@@ -26,6 +26,13 @@ hipError_t fwd3d_tiled_launch_variant(const InDesc &in, float *out, float *Zout,
 hipError_t fwd3d_tiled_launch(const InDesc &in, float *out, float *Zout, int r_p, int r_s, float k,
                               hipStream_t s);
 
+// ---- "rows" variant (nlm3d_rows.hip) ---------------------------------------
+// r_p in {1, 2} with r_s up to 7 / 6; pads the input into a stream-ordered
+// workspace first (caller-padded input needs pad >= r_p + r_s).
+bool fwd3d_rows_supported(int r_p, int r_s);
+size_t fwd3d_rows_lds_bytes(int r_p, int r_s);
+hipError_t fwd3d_rows_launch(const InDesc &in, float *out, float *Zout, int r_p, int r_s, float k, hipStream_t s);
+
 // ---- autotuner (gnlm_tune.cpp) ---------------------------------------------
 // Variant for (r_p, r_s): the tuned one if gnlm_autotune ran, otherwise the
 // first supported variant.  Thread-safe.

@@ -23,10 +23,12 @@ namespace {
 std::mutex g_mu;
 std::map<std::pair<int, int>, int> g_table;  // (r_p, r_s) -> variant
 
-// Default when (r_p, r_s) has not been tuned: the slab variant where it
-// fits (fastest on the measured configurations), else the direct-sum tiles.
+// Default when (r_p, r_s) has not been tuned: the rows variant where it is
+// instantiated (fastest on the measured configurations), then the slab
+// variants, else the direct-sum tiles.
 int first_supported(int r_p, int r_s)
 {
+    if (fwd3d_variant_supported(r_p, r_s, kVRows)) return kVRows;
     if (fwd3d_variant_supported(r_p, r_s, kVSlab2)) return kVSlab2;
     if (fwd3d_variant_supported(r_p, r_s, kVSlab)) return kVSlab;
     for (int v = 0; v < kNumFwd3Variants; ++v)

@@ -22,6 +22,7 @@
 #include <new>
 #include <vector>
 #include "gnlm_internal.h"
+#include "nlm3d_rows.h"
 #include "nlm3d_tile.h"
 
 namespace gnlm {
@@ -36,6 +37,11 @@ struct BatchVol {
     float sigma;
     int variant;
     int ntx, nty;
+    // kVRows volumes: padded copy (filled by batch_pad inside the graph) and
+    // the Z accumulator (Zout, or plan workspace when the caller wants no Z)
+    PadVol pv;
+    float *Zacc;
+    int px, py, pz, lp, halo;
 };
 
 struct BatchTile {
@@ -55,6 +61,67 @@ template <int RP>
 __device__ __noinline__ void tile_slab(const Fwd3Tile &a, float *lds) { fwd3_slab_tile<RP>(a, lds, a.sigma); }
 template <class C>
 __device__ __noinline__ void tile_direct(const Fwd3Tile &a, float *lds) { fwd3_tile<C>(a, lds); }
+template <int RP, int RS>
+__device__ __noinline__ void tile_rows(const RowsTile &a, float *lds) { rows_tile<RowsCfg<RP, RS>>(a, lds); }
+
+// kVRows tile: r_s is a template parameter of the tile body (its dx loop is
+// unrolled), so dispatch on it; the cases match nlm3d_rows.hip.
+template <int RP>
+__device__ __forceinline__ void run_rows_rp(const BatchVol &v, const BatchTile &t, float *lds)
+{
+    RowsTile a;
+    a.v = v.pv;
+    a.out = v.out;
+    a.Zacc = v.Zacc;
+    a.nx = v.in.nx;
+    a.ny = v.in.ny;
+    a.nz = v.in.nz;
+    a.k = v.k;
+    a.x0 = t.tx * 16;
+    a.y0 = t.ty * fwd3_ty(kVRows, RP);
+    a.z0 = t.tz * fwd3_tz(kVRows, RP);
+    if constexpr (RP == 1) {
+        switch (v.rs) {
+        case 1: tile_rows<1, 1>(a, lds); break;
+        case 2: tile_rows<1, 2>(a, lds); break;
+        case 3: tile_rows<1, 3>(a, lds); break;
+        case 4: tile_rows<1, 4>(a, lds); break;
+        case 5: tile_rows<1, 5>(a, lds); break;
+        case 6: tile_rows<1, 6>(a, lds); break;
+        default: tile_rows<1, 7>(a, lds); break;
+        }
+    } else if constexpr (RP == 2) {
+        switch (v.rs) {
+        case 1: tile_rows<2, 1>(a, lds); break;
+        case 2: tile_rows<2, 2>(a, lds); break;
+        case 3: tile_rows<2, 3>(a, lds); break;
+        case 4: tile_rows<2, 4>(a, lds); break;
+        case 5: tile_rows<2, 5>(a, lds); break;
+        default: tile_rows<2, 6>(a, lds); break;
+        }
+    }
+}
+
+// Padded copies of the kVRows volumes: blockIdx.y picks the volume.
+__global__ void batch_pad(const BatchVol *__restrict__ vols, const int *__restrict__ ids)
+{
+    const BatchVol &v = vols[ids[blockIdx.y]];
+    float *dst = const_cast<float *>(v.pv.p);
+    const long long n = (long long)v.px * v.py * v.pz;
+    const int r = v.halo;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const int ix = (int)(i % v.px);
+        const long long q = i / v.px;
+        const int x = ix - v.lp;
+        const int y = (int)(q % v.py) - r;
+        const int z = (int)(q / v.py) - r;
+        float val = 0.0f;
+        if (x >= -r && x < v.in.nx + r && y >= -r && y < v.in.ny + r && z >= -r && z < v.in.nz + r)
+            val = load_in(v.in, x, y, z);
+        dst[i] = val;
+    }
+}
 
 template <int RP>
 __device__ __forceinline__ void run_tile_rp(const BatchVol &v, const BatchTile &t, float *lds)
@@ -66,7 +133,9 @@ __device__ __forceinline__ void run_tile_rp(const BatchVol &v, const BatchTile &
     a.rs = v.rs;
     a.k = v.k;
     a.sigma = v.sigma;
-    if (v.variant == kVSlab2) {
+    if (v.variant == kVRows) {
+        run_rows_rp<RP>(v, t, lds);
+    } else if (v.variant == kVSlab2) {
         using C = SlabCfg<RP>;
         a.x0 = t.tx * C::TX;
         a.y0 = t.ty * C::TY;
@@ -91,7 +160,7 @@ __device__ __forceinline__ void run_tile_rp(const BatchVol &v, const BatchTile &
     }
 }
 
-__global__ __launch_bounds__(kFwdThreads) void nlm3d_fwd_persistent(const BatchVol *__restrict__ vols,
+__global__ __launch_bounds__(kFwdThreads) __attribute__((amdgpu_waves_per_eu(2))) void nlm3d_fwd_persistent(const BatchVol *__restrict__ vols,
                                                                      const BatchTile *__restrict__ tiles,
                                                                      int ntiles, int *counter)
 {
@@ -125,6 +194,10 @@ struct gnlm_batch_plan {
     BatchVol *d_vols = nullptr;
     BatchTile *d_tiles = nullptr;
     int *d_counter = nullptr;
+    int npad = 0;              // kVRows volumes
+    int *d_padids = nullptr;   // their indices
+    long long padmax = 0;      // largest padded copy (floats)
+    float *d_ws = nullptr;     // padded copies + Z accumulators
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -139,6 +212,8 @@ void plan_free(gnlm_batch_plan *p)
     if (p->d_vols) (void)hipFree(p->d_vols);
     if (p->d_tiles) (void)hipFree(p->d_tiles);
     if (p->d_counter) (void)hipFree(p->d_counter);
+    if (p->d_padids) (void)hipFree(p->d_padids);
+    if (p->d_ws) (void)hipFree(p->d_ws);
     delete p;
 }
 
@@ -147,6 +222,21 @@ int st_of(hipError_t e)
     if (e == hipSuccess) return GNLM_OK;
     if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return GNLM_ENOMEM;
     return GNLM_EHIP;
+}
+
+// The launches of one batch run: [padded copies] -> counter reset -> persistent kernel.
+hipError_t enqueue_batch(const gnlm_batch_plan *p, hipStream_t s)
+{
+    if (p->npad > 0) {
+        long long g = (p->padmax + 255) / 256;
+        if (g > 2048) g = 2048;
+        hipLaunchKernelGGL(batch_pad, dim3((unsigned)g, (unsigned)p->npad), dim3(256), 0, s,
+                           (const BatchVol *)p->d_vols, (const int *)p->d_padids);
+    }
+    hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, s, p->d_counter);
+    hipLaunchKernelGGL(nlm3d_fwd_persistent, dim3(p->nblocks), dim3(kFwdThreads), p->lds, s,
+                       (const BatchVol *)p->d_vols, (const BatchTile *)p->d_tiles, p->ntiles, p->d_counter);
+    return hipGetLastError();
 }
 
 }  // namespace
@@ -161,6 +251,9 @@ int gnlm_batch_plan_create(const gnlm_batch_volume *vols, int n, gnlm_batch_plan
     std::vector<BatchVol> hv((size_t)n);
     std::vector<BatchTile> ht;
     std::vector<double> cost;
+    std::vector<int> padids;
+    std::vector<long long> wsoff((size_t)n, -1), zoff((size_t)n, -1);
+    long long wsn = 0, padmax = 0;
     size_t lds = 0;
     for (int i = 0; i < n; ++i) {
         const gnlm_batch_volume &v = vols[i];
@@ -183,6 +276,30 @@ int gnlm_batch_plan_create(const gnlm_batch_volume *vols, int n, gnlm_batch_plan
         b.nty = (v.ny + ty - 1) / ty;
         const int ntz = (v.nz + tz - 1) / tz;
         lds = std::max(lds, fwd3d_lds_bytes(v.r_p, v.r_s, b.variant));
+        b.Zacc = nullptr;
+        b.pv = PadVol{nullptr, 0, 0};
+        b.px = b.py = b.pz = b.lp = b.halo = 0;
+        if (b.variant == kVRows) {
+            // geometry of the padded copy, as in nlm3d_rows.hip
+            b.halo = v.r_p + v.r_s;
+            b.lp = 4 * ((b.halo + 3) / 4);
+            b.px = b.ntx * tx + 2 * b.lp;
+            b.py = b.nty * ty + 2 * b.halo;
+            b.pz = ntz * tz + 2 * b.halo;
+            const long long np = (long long)b.px * b.py * b.pz;
+            const long long nv = (long long)v.nx * v.ny * v.nz;
+            if (np >= (1LL << 31) || nv >= (1LL << 31)) return GNLM_EUNSUPPORTED;
+            b.pv.px = b.px;
+            b.pv.pxy = b.px * b.py;
+            wsoff[(size_t)i] = wsn;
+            wsn += (np + 63) / 64 * 64;
+            if (!v.d_Z) {
+                zoff[(size_t)i] = wsn;
+                wsn += (nv + 63) / 64 * 64;
+            }
+            padmax = std::max(padmax, np);
+            padids.push_back(i);
+        }
         const double c = (double)tx * ty * tz * std::pow(2.0 * v.r_s + 1.0, 3);
         for (int z = 0; z < ntz; ++z)
             for (int y = 0; y < b.nty; ++y)
@@ -212,6 +329,18 @@ int gnlm_batch_plan_create(const gnlm_batch_volume *vols, int n, gnlm_batch_plan
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void *)nlm3d_fwd_persistent, kFwdThreads, lds);
     if (e == hipSuccess && occ < 1) e = hipErrorInvalidConfiguration;
     if (e == hipSuccess) p->nblocks = std::min(ncu * occ, p->ntiles);
+    p->npad = (int)padids.size();
+    p->padmax = padmax;
+    if (e == hipSuccess && wsn > 0) e = hipMalloc((void **)&p->d_ws, sizeof(float) * (size_t)wsn);
+    if (e == hipSuccess && p->npad > 0) e = hipMalloc((void **)&p->d_padids, sizeof(int) * padids.size());
+    if (e == hipSuccess && p->npad > 0)
+        e = hipMemcpy(p->d_padids, padids.data(), sizeof(int) * padids.size(), hipMemcpyHostToDevice);
+    for (int i = 0; i < n && e == hipSuccess; ++i) {
+        BatchVol &b = hv[(size_t)i];
+        if (b.variant != kVRows) continue;
+        b.pv.p = p->d_ws + wsoff[(size_t)i];
+        b.Zacc = b.Zout ? b.Zout : p->d_ws + zoff[(size_t)i];
+    }
     if (e == hipSuccess) e = hipMalloc((void **)&p->d_vols, sizeof(BatchVol) * hv.size());
     if (e == hipSuccess) e = hipMalloc((void **)&p->d_tiles, sizeof(BatchTile) * sorted.size());
     if (e == hipSuccess) e = hipMalloc((void **)&p->d_counter, sizeof(int));
@@ -223,11 +352,8 @@ int gnlm_batch_plan_create(const gnlm_batch_volume *vols, int n, gnlm_batch_plan
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, cs, p->d_counter);
-        hipLaunchKernelGGL(nlm3d_fwd_persistent, dim3(p->nblocks), dim3(kFwdThreads), lds, cs,
-                           (const BatchVol *)p->d_vols, (const BatchTile *)p->d_tiles, p->ntiles, p->d_counter);
+        e = enqueue_batch(p, cs);
         hipError_t e2 = hipStreamEndCapture(cs, &p->graph);
-        e = hipGetLastError();
         if (e == hipSuccess) e = e2;
     }
     if (e == hipSuccess) e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
@@ -249,12 +375,7 @@ int gnlm_batch_run(gnlm_batch_plan *plan, void *stream)
 int gnlm_batch_run_eager(gnlm_batch_plan *plan, void *stream)
 {
     if (!plan || !plan->d_counter) return GNLM_ESTATE;
-    hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, s, plan->d_counter);
-    hipLaunchKernelGGL(nlm3d_fwd_persistent, dim3(plan->nblocks), dim3(kFwdThreads), plan->lds, s,
-                       (const BatchVol *)plan->d_vols, (const BatchTile *)plan->d_tiles, plan->ntiles,
-                       plan->d_counter);
-    return st_of(hipGetLastError());
+    return st_of(enqueue_batch(plan, (hipStream_t)stream));
 }
 
 int gnlm_batch_plan_info(const gnlm_batch_plan *plan, int *ntiles, int *nblocks)

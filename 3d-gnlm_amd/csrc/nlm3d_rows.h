@@ -1,0 +1,327 @@
+// nlm3d_rows.h -- "rows" forward variant of the 3D NLM (device code), shared
+// by the per-volume launch (nlm3d_fwd.hip) and the persistent batched launch
+// (nlm3d_batch.hip).
+//
+// Same mathematics as the other variants (3DNLM/main.cu:52-68 patch SSD over
+// the full offset cube, :107-111 weights, :125 normalisation), organised so
+// that the input never sits in LDS:
+//
+//   * A pre-pass copies the volume with its reflect-101 border (the
+//     reference pads on the host, 3DNLM/nlm.cpp:36-41) into an aligned HBM
+//     buffer: rows start on 16-byte boundaries and every sample a tile can
+//     touch exists, so the kernel reads rows with 128-bit global loads and
+//     no border logic.  The tile's working set (~100 KiB) stays in L2.
+//   * One workgroup (256 threads, 4 wave64) per 16 x TY x TZ tile; LDS holds
+//     only two small partial-sum buffers (~70 KiB for r_p = 2), so two
+//     workgroups share a CU and hide each other's barriers and latencies.
+//   * Offsets are visited dz, dy, dx.  Phase-1 threads own whole rows
+//     (y_ext, z_ext) of the tile grown by r_p: the centre samples of a row
+//     stay in registers for the whole kernel, and the shifted row for one
+//     (dy, dz) is loaded once (16 + 2 LP samples) and serves all 2 r_s + 1
+//     dx offsets from registers -- the dx shift is a compile-time register
+//     index inside a uniform switch.
+//   * phase 1: squared differences, x box sums (doubling tree) -> s_x rows
+//              (four 128-bit LDS stores per row);
+//     phase 2: (x vector, z_ext, run of y) items: y box sums -> s_xy;
+//     phase 3: (x, y, run of z) items: z box sums, w = exp2(S k), per-dz
+//              partial sums in registers; the candidate samples u~(x+d)
+//              come straight from the padded copy (L1/L2 hits).
+//     Two barriers per offset; every LDS access pattern is conflict-free
+//     for 64 banks (row pitches of 20 floats, plane paddings of 16 floats).
+//   * Weights accumulate in two levels (partial over the (dy, dx) offsets of
+//     one dz, then the total over dz) as in the other variants; the totals
+//     live in the output buffers between dz steps (one read-modify-write of
+//     two floats per voxel per dz), which keeps the register file for the
+//     rows and partials.
+// Box sums add non-negative terms only (no running add/subtract windows), in
+// a pairwise tree instead of the CPU oracle's left-to-right chain, so the
+// results agree with the oracle to rounding (~1e-7 relative in S) rather
+// than bit for bit.
+#pragma once
+#include <type_traits>
+#include "gnlm_device.h"
+
+namespace gnlm {
+
+constexpr int kRowsThreads = 256;
+
+// Padded, aligned copy of a volume.  Sample (x, y, z) of the reflect-101
+// extension is p[(z + R) * pxy + (y + R) * px + (x + LP)] for x in
+// [-LP, nxr + LP), y and z in [-R, n?r + R), where nxr, nyr, nzr round the
+// volume up to whole tiles; samples outside [-R, n + R) hold 0 and only feed
+// outputs that are discarded.  The copy holds fewer than 2^31 floats.
+struct PadVol {
+    const float *p;
+    int px;   // row pitch in floats (multiple of 4)
+    int pxy;  // plane pitch in floats
+};
+
+template <int RP_, int RS_>
+struct RowsCfg {
+    static constexpr int RP = RP_, RS = RS_;
+    static constexpr int W = 2 * RP + 1;                 // patch width
+    static constexpr int R = RP + RS;                    // halo of the copy in y, z
+    static constexpr int LP = 4 * ((R + 3) / 4);         // halo of the copy in x (aligned)
+    static constexpr int TX = 16;
+    static constexpr int NS = TX + 2 * LP;               // samples per loaded row
+    static constexpr int TY = RP == 1 ? 14 : 12;
+    static constexpr int TZ = RP == 1 ? 14 : 28;
+    static constexpr int EX = TX + 2 * RP, EY = TY + 2 * RP, EZ = TZ + 2 * RP;
+    static constexpr int NROWS = EY * EZ;
+    static constexpr int ROWS = NROWS / kRowsThreads;    // phase-1 rows per thread
+    static_assert(EY == 16 && NROWS % kRowsThreads == 0, "16-lane groups hold one z_ext plane of rows");
+    // phase 2 items: (x pair, z_ext, run of RY y), P2R per thread
+    static constexpr int VX = 2;
+    static constexpr int RY = RP == 1 ? 7 : 6;
+    static constexpr int NXV = TX / VX;
+    static constexpr int P2N = NXV * EZ * (TY / RY);
+    static constexpr int P2R = P2N / kRowsThreads;
+    static_assert(TY % RY == 0 && P2N % kRowsThreads == 0, "whole phase-2 rounds");
+    // phase 3 items: (x, y, run of RZ z)
+    static constexpr int RZ = 7;
+    static constexpr int P3N = TX * TY * (TZ / RZ);
+    static constexpr int P3R = (P3N + kRowsThreads - 1) / kRowsThreads;
+    static_assert(TZ % RZ == 0, "whole z runs");
+    // LDS: s_x [z_ext][y_ext][x] (row pitch 20, plane pitch 20*16+16),
+    //      s_xy [z_ext][y][x] (plane pitch 16*TY+16)
+    static constexpr int SXR = 20;
+    static constexpr int SXP = SXR * EY + 16;
+    static constexpr int SXN = SXP * EZ;
+    static constexpr int SYP = 16 * TY + 16;
+    static constexpr int SYN = SYP * EZ;
+    static constexpr int LDS_FLOATS = SXN + SYN;
+};
+
+// s[i] = v[i] + v[i+1] + ... + v[i+W-1] for i < N, as a pairwise tree whose
+// pair sums are shared between neighbouring outputs (W = 3, 5, 7).
+template <int W, int N>
+__device__ __forceinline__ void box_tree(const float *v, float *s)
+{
+    static_assert(W == 3 || W == 5 || W == 7, "patch widths 3, 5, 7");
+    constexpr int NP = N + W - 3;
+    float p[NP];
+#pragma unroll
+    for (int i = 0; i < NP; ++i) p[i] = v[i] + v[i + 1];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        if constexpr (W == 3) s[i] = p[i] + v[i + 2];
+        else if constexpr (W == 5) s[i] = (p[i] + p[i + 2]) + v[i + 4];
+        else s[i] = ((p[i] + p[i + 2]) + p[i + 4]) + v[i + 6];
+    }
+}
+
+template <int NS>
+__device__ __forceinline__ void load_row(const float *src, float *dst)
+{
+#pragma unroll
+    for (int j = 0; j < NS / 4; ++j) {
+        const float4 t = reinterpret_cast<const float4 *>(src)[j];
+        dst[4 * j + 0] = t.x;
+        dst[4 * j + 1] = t.y;
+        dst[4 * j + 2] = t.z;
+        dst[4 * j + 3] = t.w;
+    }
+}
+
+// Workgroup barrier that waits for LDS traffic only: global loads in flight
+// (the next rows, candidate samples) stay in flight across it.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+struct RowsTile {
+    PadVol v;
+    float *out;    // y, and the running Y total between dz steps
+    float *Zacc;   // running Z total between dz steps; Z at the end
+    int nx, ny, nz;
+    float k;       // w = exp2(S * k), k = -log2(e) / (P h^2)
+    int x0, y0, z0;
+};
+
+// phase 1 for the offset whose first shifted sample is sh[SH]
+template <class C, int SH>
+__device__ __forceinline__ void rows_phase1(const float (&cen)[C::ROWS][C::EX], const float (&sh)[C::ROWS][C::NS],
+                                            float *SX, const int (&sxw)[C::ROWS])
+{
+#pragma unroll
+    for (int q = 0; q < C::ROWS; ++q) {
+        float qv[C::EX];
+#pragma unroll
+        for (int i = 0; i < C::EX; ++i) {
+            const float e = cen[q][i] - sh[q][SH + i];
+            qv[i] = e * e;
+        }
+        float s[C::TX];
+        box_tree<C::W, C::TX>(qv, s);
+        float4 *dst = reinterpret_cast<float4 *>(SX + sxw[q]);
+#pragma unroll
+        for (int j = 0; j < C::TX / 4; ++j) dst[j] = make_float4(s[4 * j], s[4 * j + 1], s[4 * j + 2], s[4 * j + 3]);
+    }
+}
+
+// compile-time loop: f(std::integral_constant<int, I>) for I = I0 .. I1
+template <int I, int I1, class F>
+__device__ __forceinline__ void static_for(F &&f)
+{
+    if constexpr (I <= I1) {
+        f(std::integral_constant<int, I>{});
+        static_for<I + 1, I1>(f);
+    }
+}
+
+template <class C>
+__device__ __forceinline__ void rows_phase2(const float *src, float *dst)
+{
+    constexpr int NIN = C::RY + 2 * C::RP;
+    float v[C::VX][NIN];
+#pragma unroll
+    for (int j = 0; j < NIN; ++j) {
+        if constexpr (C::VX == 4) {
+            const float4 t = *reinterpret_cast<const float4 *>(src + j * C::SXR);
+            v[0][j] = t.x;
+            v[1][j] = t.y;
+            v[2][j] = t.z;
+            v[3][j] = t.w;
+        } else {
+            const float2 t = *reinterpret_cast<const float2 *>(src + j * C::SXR);
+            v[0][j] = t.x;
+            v[1][j] = t.y;
+        }
+    }
+    float s[C::VX][C::RY];
+#pragma unroll
+    for (int c = 0; c < C::VX; ++c) box_tree<C::W, C::RY>(v[c], s[c]);
+#pragma unroll
+    for (int o = 0; o < C::RY; ++o) {
+        if constexpr (C::VX == 4)
+            *reinterpret_cast<float4 *>(dst + o * 16) = make_float4(s[0][o], s[1][o], s[2][o], s[3][o]);
+        else
+            *reinterpret_cast<float2 *>(dst + o * 16) = make_float2(s[0][o], s[1][o]);
+    }
+}
+
+template <class C>
+__device__ __forceinline__ void rows_tile(const RowsTile &a, float *lds)
+{
+    constexpr int RP = C::RP, RS = C::RS, RZ = C::RZ, NT = kRowsThreads;
+    float *SX = lds;
+    float *SXY = lds + C::SXN;
+    const int tid = threadIdx.x;
+    const float *P = a.v.p;
+    const int px = a.v.px, pxy = a.v.pxy;
+
+    // ---- phase-1 rows: y_ext = r % 16, z_ext = r / 16 ----
+    float cen[C::ROWS][C::EX];
+    unsigned rbase[C::ROWS];  // unsigned: loads take a uniform base + 32-bit lane offset
+    int sxw[C::ROWS];
+#pragma unroll
+    for (int q = 0; q < C::ROWS; ++q) {
+        const int r = tid + q * NT;
+        const int yp = r % C::EY, zp = r / C::EY;
+        rbase[q] = (a.z0 - RP + zp + C::R) * pxy + (a.y0 - RP + yp + C::R) * px + a.x0;
+        sxw[q] = zp * C::SXP + yp * C::SXR;
+        float row[C::NS];
+        load_row<C::NS>(P + rbase[q], row);
+#pragma unroll
+        for (int i = 0; i < C::EX; ++i) cen[q][i] = row[C::LP - RP + i];
+    }
+
+    // ---- phase-2 items: lanes run x pair fastest, then z_ext ----
+    int p2src[C::P2R], p2dst[C::P2R];
+#pragma unroll
+    for (int m = 0; m < C::P2R; ++m) {
+        const int it = tid + m * NT;
+        const int xv = it % C::NXV;
+        const int zp2 = (it / C::NXV) % C::EZ;
+        const int yr2 = it / (C::NXV * C::EZ);
+        p2src[m] = zp2 * C::SXP + yr2 * C::RY * C::SXR + xv * C::VX;
+        p2dst[m] = zp2 * C::SYP + yr2 * C::RY * 16 + xv * C::VX;
+    }
+
+    // ---- phase-3 items ----
+    int p3r[C::P3R];
+    unsigned cb[C::P3R];
+    bool p3ok[C::P3R];
+#pragma unroll
+    for (int k = 0; k < C::P3R; ++k) {
+        const int it = tid + k * NT;
+        p3ok[k] = (C::P3N % NT == 0) || it < C::P3N;
+        const int itc = p3ok[k] ? it : C::P3N - 1;
+        const int x = itc % 16, y = (itc / 16) % C::TY, zr = itc / (16 * C::TY);
+        p3r[k] = zr * RZ * C::SYP + y * 16 + x;
+        cb[k] = (a.z0 + zr * RZ + C::R) * pxy + (a.y0 + y + C::R) * px + a.x0 + x + C::LP;
+    }
+
+    for (int dz = -RS; dz <= RS; ++dz) {
+        float Zd[C::P3R][RZ], Yd[C::P3R][RZ];
+#pragma unroll
+        for (int k = 0; k < C::P3R; ++k)
+#pragma unroll
+            for (int o = 0; o < RZ; ++o) {
+                Zd[k][o] = 0.0f;
+                Yd[k][o] = 0.0f;
+            }
+        for (int dy = -RS; dy <= RS; ++dy) {
+            const int soff = dz * pxy + dy * px;
+            float sh[C::ROWS][C::NS];
+            const float *Ps = P + soff;  // uniform base, per-lane offset rbase[q]
+#pragma unroll
+            for (int q = 0; q < C::ROWS; ++q) load_row<C::NS>(Ps + rbase[q], sh[q]);
+
+            // dx unrolled: the shift into sh is a compile-time register index
+            static_for<-RS, RS>([&](auto dxc) {
+                constexpr int DX = decltype(dxc)::value;
+                rows_phase1<C, C::LP - RP + DX>(cen, sh, SX, sxw);
+                // candidate samples u~(x + d), in flight across the barriers
+                const float *Pc = P + (soff + DX);
+                float cand[C::P3R][RZ];
+#pragma unroll
+                for (int o = 0; o < RZ; ++o) {
+                    const float *Po = Pc + o * pxy;  // uniform base, per-lane offset cb[k]
+#pragma unroll
+                    for (int k = 0; k < C::P3R; ++k) cand[k][o] = Po[cb[k]];
+                }
+                lds_barrier();
+#pragma unroll
+                for (int m = 0; m < C::P2R; ++m) rows_phase2<C>(SX + p2src[m], SXY + p2dst[m]);
+                lds_barrier();
+#pragma unroll
+                for (int k = 0; k < C::P3R; ++k) {
+                    if (p3ok[k]) {
+                        float v[RZ + 2 * RP];
+#pragma unroll
+                        for (int j = 0; j < RZ + 2 * RP; ++j) v[j] = SXY[p3r[k] + j * C::SYP];
+                        float S[RZ];
+                        box_tree<C::W, RZ>(v, S);
+#pragma unroll
+                        for (int o = 0; o < RZ; ++o) {
+                            const float w = exp2_fast(S[o] * a.k);
+                            Zd[k][o] = Zd[k][o] + w;
+                            Yd[k][o] = fmaf(w, cand[k][o], Yd[k][o]);
+                        }
+                    }
+                }
+            });
+        }
+        // close the dz block: totals += partials (kept in the output buffers)
+#pragma unroll
+        for (int k = 0; k < C::P3R; ++k) {
+            const int it = tid + k * NT;
+            const int x = it % 16, y = (it / 16) % C::TY, zr = it / (16 * C::TY);
+            const int gx = a.x0 + x, gy = a.y0 + y, gz = a.z0 + zr * RZ;
+            if (!p3ok[k] || gx >= a.nx || gy >= a.ny) continue;
+#pragma unroll
+            for (int o = 0; o < RZ; ++o) {
+                if (gz + o >= a.nz) continue;
+                const int oi = gx + a.nx * (gy + a.ny * (gz + o));
+                float Z = Zd[k][o], Y = Yd[k][o];
+                if (dz != -RS) {
+                    Z = a.Zacc[oi] + Z;
+                    Y = a.out[oi] + Y;
+                }
+                a.Zacc[oi] = Z;
+                a.out[oi] = (dz == RS) ? Y / Z : Y;
+            }
+        }
+    }
+}
+
+}  // namespace gnlm

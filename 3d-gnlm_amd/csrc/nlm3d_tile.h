@@ -570,16 +570,20 @@ __device__ __forceinline__ void fwd3_slab2_tile(const Fwd3Tile &a, float *lds, f
 }
 
 // Tile variants.
-enum Fwd3Variant { kV16x16x16 = 0, kV16x16x8 = 1, kVSlab = 2, kVSlab2 = 3, kNumFwd3Variants = 4 };
+// kVRows is the padded-copy variant of nlm3d_rows.h.
+enum Fwd3Variant { kV16x16x16 = 0, kV16x16x8 = 1, kVSlab = 2, kVSlab2 = 3, kVRows = 4, kNumFwd3Variants = 5 };
 
 __host__ __device__ constexpr bool fwd3_is_slab(int v) { return v == kVSlab || v == kVSlab2; }
 
-// Tile extents of a variant for patch radius rp.
+// Tile extents of a variant for patch radius rp (kVRows: RowsCfg).
 __host__ __device__ constexpr int fwd3_tx(int v, int rp) { return (void)v, (void)rp, 16; }
-__host__ __device__ constexpr int fwd3_ty(int v, int rp) { return fwd3_is_slab(v) ? 16 - 2 * rp : 16; }
+__host__ __device__ constexpr int fwd3_ty(int v, int rp)
+{
+    return v == kVRows ? (rp == 1 ? 14 : 12) : (fwd3_is_slab(v) ? 16 - 2 * rp : 16);
+}
 __host__ __device__ constexpr int fwd3_tz(int v, int rp)
 {
-    return fwd3_is_slab(v) ? 32 - 2 * rp : (v == kV16x16x8 ? 8 : 16);
+    return v == kVRows ? (rp == 1 ? 14 : 28) : (fwd3_is_slab(v) ? 32 - 2 * rp : (v == kV16x16x8 ? 8 : 16));
 }
 
 }  // namespace gnlm

@@ -148,7 +148,7 @@ def reset_tuning():
     lib().gnlm_reset_tuning()
 
 
-VARIANT_16x16x16, VARIANT_16x16x8, VARIANT_SLAB, VARIANT_SLAB2 = 0, 1, 2, 3
+VARIANT_16x16x16, VARIANT_16x16x8, VARIANT_SLAB, VARIANT_SLAB2, VARIANT_ROWS = 0, 1, 2, 3, 4
 
 
 def check(status, where=""):

@@ -124,6 +124,59 @@ def test_3d_slab2_equals_slab(dev, shape, rp, rs, h):
     assert np.array_equal(outs[0], outs[1]), report(outs[1], outs[0])
 
 
+ROWS_CASES = [
+    ((28, 24, 32), 2, 5, 0.1),    # whole 16 x 12 x 28 tiles
+    ((37, 29, 33), 2, 5, 0.2),    # partial tiles on every axis
+    ((17, 19, 23), 2, 3, 0.25),
+    ((12, 12, 12), 2, 6, 0.3),    # largest instantiated r_s for r_p = 2
+    ((30, 14, 16), 1, 3, 0.3),
+    ((20, 20, 20), 1, 7, 0.3),    # largest instantiated r_s for r_p = 1
+    ((16, 16, 16), 1, 1, 0.3),
+    ((9, 8, 7), 1, 4, 0.3),       # smaller than one tile, reflection folds
+    ((5, 6, 7), 1, 3, 0.3),       # dims smaller than the halo
+    ((1, 20, 20), 1, 2, 0.3),     # single plane
+]
+
+
+@pytest.mark.parametrize("shape,rp,rs,h", ROWS_CASES)
+def test_3d_rows_variant_matches_oracle(dev, shape, rp, rs, h):
+    """Padded-copy "rows" variant: pairwise-tree box sums, so it agrees with
+    the oracle's left-to-right order to rounding (<= 1e-5 relative, uniform
+    random input, against the FP32 path and FP64)."""
+    rng = np.random.default_rng(sum(shape) + 17 * rp + rs)
+    u = rng.random(shape, dtype=np.float32)
+    print(f"case rows shape={shape} rp={rp} rs={rs}", flush=True)
+    try:
+        gnlm.set_variant(rp, rs, gnlm.VARIANT_ROWS)
+        y = _run3(dev, u, rp, rs, h, gnlm.PATH_TILED)
+    finally:
+        gnlm.reset_tuning()
+    ref32 = oracle.nlm3d_brute(u, rp, rs, h)
+    ref64 = oracle.nlm3d_brute(u.astype(np.float64), rp, rs, h)
+    print("rows vs fp32", report(y, ref32), "vs fp64", report(y, ref64))
+    assert max_rel(y, ref32) <= TOL, report(y, ref32)
+    assert max_rel(y, ref64) <= TOL, report(y, ref64)
+
+
+def test_3d_rows_padded_equals_reflect(dev):
+    """A caller-padded input (the drop-in layout) gives the same bits as the
+    reflect-on-read input through the rows variant."""
+    rng = np.random.default_rng(11)
+    u = rng.random((19, 26, 35), dtype=np.float32)
+    rp, rs, h = 2, 4, 0.2
+    R = rp + rs
+    up = np.pad(u, R, mode="reflect")
+    try:
+        gnlm.set_variant(rp, rs, gnlm.VARIANT_ROWS)
+        a = _run3(dev, u, rp, rs, h, gnlm.PATH_TILED)
+        t = torch.from_numpy(np.ascontiguousarray(up)).to(dev)
+        nz, ny, nx = u.shape
+        b = gnlm.nlm3d_padded(t, R, nx, ny, nz, rp, rs, h, gnlm.PATH_TILED).cpu().numpy()
+    finally:
+        gnlm.reset_tuning()
+    assert np.array_equal(a, b), report(b, a)
+
+
 def test_3d_large_patch_generic(dev):
     # the reference's default geometry class (NS = 17 -> r_p = 8), small volume
     u, _ = gnlm.phantom3d(20, 18, 12, n_ellipsoids=3, axis_lo=3, axis_hi=7, sigma=0.05, seed=3)
