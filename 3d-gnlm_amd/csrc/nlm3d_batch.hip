@@ -4,18 +4,24 @@
 // No counterpart in the reference (3DNLM/main.cu:7607-7680 processes one
 // compile-time volume per call).  Design:
 //   * at plan creation every volume is cut into tiles of the variant the
-//     autotuner picked for its (r_p, r_s); the tiles of all volumes form one
-//     work list ordered longest-first by cost (tile voxels x offsets), and the
-//     list and the per-volume descriptors are uploaded once;
+//     autotuner picked for its (r_p, r_s).  Each volume's tile list (x
+//     fastest) is split into 8 contiguous ranges, one per XCD, and XCD j's
+//     work queue holds range j of every volume, volumes ordered longest-first
+//     by tile cost (tile voxels x offsets).  Tiles that share halo samples
+//     thus run on one XCD and share its L2 (a single global queue spreads
+//     neighbouring tiles over all 8 L2s and their working sets overflow
+//     them); every XCD gets an eighth of every volume, so the queues carry
+//     equal work;
 //   * the persistent kernel runs as many workgroups as fit on the device at
-//     once; each repeatedly takes the next tile index from a device counter
-//     (one atomicAdd by one thread, broadcast through LDS) and processes the
-//     tile with the same device function the per-volume launch uses, so the
-//     outputs are bit-identical to per-volume launches.  Every workgroup exits
-//     as soon as the counter passes the end of the list;
-//   * the graph holds [counter reset kernel -> persistent kernel]; replaying
-//     it re-runs the whole batch with one launch (no hipMemset in the graph:
-//     the counter is cleared by a kernel).
+//     once; workgroup b serves the queue of XCD b % 8 (the dispatcher's
+//     round-robin placement), taking the next tile index with one atomicAdd
+//     broadcast through LDS, and steals from the other queues once its own
+//     is empty.  It processes a tile with the same device function the
+//     per-volume launch uses, so the outputs are bit-identical to per-volume
+//     launches.  Every workgroup exits once all queues are exhausted;
+//   * the graph holds [padding pass -> counter reset kernel -> persistent
+//     kernel]; replaying it re-runs the whole batch with one launch (no
+//     hipMemset in the graph: the counters are cleared by a kernel).
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cmath>
@@ -49,7 +55,12 @@ struct BatchTile {
     int tx, ty, tz;
 };
 
-__global__ void reset_counter(int *c) { *c = 0; }
+constexpr int kNumXcd = 8;
+
+__global__ void reset_counters(int *c)
+{
+    if (threadIdx.x < kNumXcd) c[threadIdx.x] = 0;
+}
 
 // Each (variant, r_p) tile body is a separate, non-inlined function: inlined
 // together into the one persistent kernel, their register demands add up and
@@ -162,16 +173,30 @@ __device__ __forceinline__ void run_tile_rp(const BatchVol &v, const BatchTile &
 
 __global__ __launch_bounds__(kFwdThreads) __attribute__((amdgpu_waves_per_eu(2))) void nlm3d_fwd_persistent(const BatchVol *__restrict__ vols,
                                                                      const BatchTile *__restrict__ tiles,
-                                                                     int ntiles, int *counter)
+                                                                     const int *__restrict__ qstart, int *counters)
 {
-    extern __shared__ float lds[];
+    extern __shared__ __align__(16) float lds[];  // 128-bit LDS accesses in the tile bodies
     __shared__ int next;
+    const int xcd = blockIdx.x % kNumXcd;
+    int tried = 0;  // queues found empty so far (thread 0 only)
     for (;;) {
-        if (threadIdx.x == 0) next = atomicAdd(counter, 1);
+        if (threadIdx.x == 0) {
+            int t = -1;
+            while (tried < kNumXcd) {
+                const int q = (xcd + tried) % kNumXcd;
+                const int i = qstart[q] + atomicAdd(&counters[q], 1);
+                if (i < qstart[q + 1]) {
+                    t = i;
+                    break;
+                }
+                ++tried;
+            }
+            next = t;
+        }
         __syncthreads();
         const int t = next;
         __syncthreads();  // all threads have read `next` and finished the previous tile's LDS use
-        if (t >= ntiles) break;
+        if (t < 0) break;
         const BatchTile tl = tiles[t];
         const BatchVol v = vols[tl.vol];
         switch (v.rp) {
@@ -193,7 +218,7 @@ struct gnlm_batch_plan {
     size_t lds = 0;
     BatchVol *d_vols = nullptr;
     BatchTile *d_tiles = nullptr;
-    int *d_counter = nullptr;
+    int *d_counter = nullptr;  // kNumXcd queue counters, then kNumXcd + 1 queue starts
     int npad = 0;              // kVRows volumes
     int *d_padids = nullptr;   // their indices
     long long padmax = 0;      // largest padded copy (floats)
@@ -233,9 +258,10 @@ hipError_t enqueue_batch(const gnlm_batch_plan *p, hipStream_t s)
         hipLaunchKernelGGL(batch_pad, dim3((unsigned)g, (unsigned)p->npad), dim3(256), 0, s,
                            (const BatchVol *)p->d_vols, (const int *)p->d_padids);
     }
-    hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, s, p->d_counter);
+    hipLaunchKernelGGL(reset_counters, dim3(1), dim3(64), 0, s, p->d_counter);
     hipLaunchKernelGGL(nlm3d_fwd_persistent, dim3(p->nblocks), dim3(kFwdThreads), p->lds, s,
-                       (const BatchVol *)p->d_vols, (const BatchTile *)p->d_tiles, p->ntiles, p->d_counter);
+                       (const BatchVol *)p->d_vols, (const BatchTile *)p->d_tiles,
+                       (const int *)(p->d_counter + kNumXcd), p->d_counter);
     return hipGetLastError();
 }
 
@@ -249,8 +275,8 @@ int gnlm_batch_plan_create(const gnlm_batch_volume *vols, int n, gnlm_batch_plan
     *out = nullptr;
     if (!vols || n < 1) return GNLM_EINVAL;
     std::vector<BatchVol> hv((size_t)n);
-    std::vector<BatchTile> ht;
-    std::vector<double> cost;
+    std::vector<std::vector<BatchTile>> vt;  // tiles of each volume, x fastest
+    std::vector<double> cost;                 // per-tile cost of each volume
     std::vector<int> padids;
     std::vector<long long> wsoff((size_t)n, -1), zoff((size_t)n, -1);
     long long wsn = 0, padmax = 0;
@@ -277,7 +303,7 @@ int gnlm_batch_plan_create(const gnlm_batch_volume *vols, int n, gnlm_batch_plan
         const int ntz = (v.nz + tz - 1) / tz;
         lds = std::max(lds, fwd3d_lds_bytes(v.r_p, v.r_s, b.variant));
         b.Zacc = nullptr;
-        b.pv = PadVol{nullptr, 0, 0};
+        b.pv = PadVol{nullptr, 0, 0, 0u};
         b.px = b.py = b.pz = b.lp = b.halo = 0;
         if (b.variant == kVRows) {
             // geometry of the padded copy, as in nlm3d_rows.hip
@@ -288,9 +314,10 @@ int gnlm_batch_plan_create(const gnlm_batch_volume *vols, int n, gnlm_batch_plan
             b.pz = ntz * tz + 2 * b.halo;
             const long long np = (long long)b.px * b.py * b.pz;
             const long long nv = (long long)v.nx * v.ny * v.nz;
-            if (np >= (1LL << 31) || nv >= (1LL << 31)) return GNLM_EUNSUPPORTED;
+            if (np >= (1LL << 30) || nv >= (1LL << 31)) return GNLM_EUNSUPPORTED;
             b.pv.px = b.px;
             b.pv.pxy = b.px * b.py;
+            b.pv.bytes = (unsigned)(sizeof(float) * np);
             wsoff[(size_t)i] = wsn;
             wsn += (np + 63) / 64 * 64;
             if (!v.d_Z) {
@@ -300,20 +327,27 @@ int gnlm_batch_plan_create(const gnlm_batch_volume *vols, int n, gnlm_batch_plan
             padmax = std::max(padmax, np);
             padids.push_back(i);
         }
-        const double c = (double)tx * ty * tz * std::pow(2.0 * v.r_s + 1.0, 3);
+        cost.push_back((double)tx * ty * tz * std::pow(2.0 * v.r_s + 1.0, 3));
+        vt.emplace_back();
         for (int z = 0; z < ntz; ++z)
             for (int y = 0; y < b.nty; ++y)
-                for (int x = 0; x < b.ntx; ++x) {
-                    ht.push_back(BatchTile{i, x, y, z});
-                    cost.push_back(c);
-                }
+                for (int x = 0; x < b.ntx; ++x) vt.back().push_back(BatchTile{i, x, y, z});
     }
-    // longest first; ties keep volume/tile order (stable)
-    std::vector<size_t> order(ht.size());
-    for (size_t i = 0; i < order.size(); ++i) order[i] = i;
-    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
-    std::vector<BatchTile> sorted(ht.size());
-    for (size_t i = 0; i < order.size(); ++i) sorted[i] = ht[order[i]];
+    // volumes longest-first by tile cost (stable); queue j = range j of each volume
+    std::vector<int> vorder((size_t)n);
+    for (int i = 0; i < n; ++i) vorder[(size_t)i] = i;
+    std::stable_sort(vorder.begin(), vorder.end(), [&](int a, int b) { return cost[(size_t)a] > cost[(size_t)b]; });
+    std::vector<BatchTile> sorted;
+    std::vector<int> qstart(kNumXcd + 1, 0);
+    for (int q = 0; q < kNumXcd; ++q) {
+        qstart[(size_t)q] = (int)sorted.size();
+        for (int i : vorder) {
+            const std::vector<BatchTile> &t = vt[(size_t)i];
+            const size_t lo = t.size() * (size_t)q / kNumXcd, hi = t.size() * (size_t)(q + 1) / kNumXcd;
+            sorted.insert(sorted.end(), t.begin() + (long)lo, t.begin() + (long)hi);
+        }
+    }
+    qstart[kNumXcd] = (int)sorted.size();
 
     gnlm_batch_plan *p = new (std::nothrow) gnlm_batch_plan();
     if (!p) return GNLM_ENOMEM;
@@ -343,7 +377,9 @@ int gnlm_batch_plan_create(const gnlm_batch_volume *vols, int n, gnlm_batch_plan
     }
     if (e == hipSuccess) e = hipMalloc((void **)&p->d_vols, sizeof(BatchVol) * hv.size());
     if (e == hipSuccess) e = hipMalloc((void **)&p->d_tiles, sizeof(BatchTile) * sorted.size());
-    if (e == hipSuccess) e = hipMalloc((void **)&p->d_counter, sizeof(int));
+    if (e == hipSuccess) e = hipMalloc((void **)&p->d_counter, sizeof(int) * (2 * kNumXcd + 1));
+    if (e == hipSuccess)
+        e = hipMemcpy(p->d_counter + kNumXcd, qstart.data(), sizeof(int) * (kNumXcd + 1), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(p->d_vols, hv.data(), sizeof(BatchVol) * hv.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(p->d_tiles, sorted.data(), sizeof(BatchTile) * sorted.size(), hipMemcpyHostToDevice);

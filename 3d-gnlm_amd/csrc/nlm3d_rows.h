@@ -49,11 +49,13 @@ constexpr int kRowsThreads = 256;
 // extension is p[(z + R) * pxy + (y + R) * px + (x + LP)] for x in
 // [-LP, nxr + LP), y and z in [-R, n?r + R), where nxr, nyr, nzr round the
 // volume up to whole tiles; samples outside [-R, n + R) hold 0 and only feed
-// outputs that are discarded.  The copy holds fewer than 2^31 floats.
+// outputs that are discarded.  The copy is smaller than 4 GiB: the kernel
+// addresses it with 32-bit byte offsets.
 struct PadVol {
     const float *p;
-    int px;   // row pitch in floats (multiple of 4)
-    int pxy;  // plane pitch in floats
+    int px;          // row pitch in floats (multiple of 4)
+    int pxy;         // plane pitch in floats
+    unsigned bytes;  // size of the copy (< 4 GiB: offsets are 32-bit)
 };
 
 template <int RP_, int RS_>
@@ -92,6 +94,13 @@ struct RowsCfg {
     static constexpr int LDS_FLOATS = SXN + SYN;
 };
 
+// Global-address-space view of the output buffers.  When the tile body is
+// a called function (the persistent batched kernel) its pointers arrive
+// generic and memory operations through them become flat operations, which
+// count against the LDS wait counter as well (every lds_barrier would wait
+// for them); address space 1 keeps them global.
+typedef __attribute__((address_space(1))) float gfloat;
+
 // s[i] = v[i] + v[i+1] + ... + v[i+W-1] for i < N, as a pairwise tree whose
 // pair sums are shared between neighbouring outputs (W = 3, 5, 7).
 template <int W, int N>
@@ -110,16 +119,42 @@ __device__ __forceinline__ void box_tree(const float *v, float *s)
     }
 }
 
+// The padded copy is read with buffer loads: a scalar resource (base, size),
+// a 32-bit per-lane byte offset that is fixed for the whole tile, and a
+// uniform (scalar) byte offset for the current offset d -- no vector address
+// arithmetic in the loops.  Accesses are in bounds by construction; loads
+// past the resource's size would return 0.
+struct PadRsrc {
+    __amdgpu_buffer_rsrc_t r;
+};
+
+__device__ __forceinline__ PadRsrc pad_rsrc(const PadVol &v)
+{
+    // the base may arrive in vector registers (called tile bodies): it is
+    // uniform, so take it from the first lane to keep the resource scalar
+    const unsigned long long b = (unsigned long long)v.p;
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)b);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(b >> 32));
+    const unsigned nb = __builtin_amdgcn_readfirstlane(v.bytes);
+    void *p = (void *)(((unsigned long long)hi << 32) | lo);
+    return PadRsrc{__builtin_amdgcn_make_buffer_rsrc(p, (short)0, (int)nb, 0x00020000)};
+}
+
+__device__ __forceinline__ float pad_load(const PadRsrc &r, unsigned voff, int soff)
+{
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r.r, (int)voff, soff, 0));
+}
+
 template <int NS>
-__device__ __forceinline__ void load_row(const float *src, float *dst)
+__device__ __forceinline__ void pad_load_row(const PadRsrc &r, unsigned voff, int soff, float *dst)
 {
 #pragma unroll
     for (int j = 0; j < NS / 4; ++j) {
-        const float4 t = reinterpret_cast<const float4 *>(src)[j];
-        dst[4 * j + 0] = t.x;
-        dst[4 * j + 1] = t.y;
-        dst[4 * j + 2] = t.z;
-        dst[4 * j + 3] = t.w;
+        const auto t = __builtin_amdgcn_raw_buffer_load_b128(r.r, (int)(voff + 16u * j), soff, 0);
+        dst[4 * j + 0] = __uint_as_float(t[0]);
+        dst[4 * j + 1] = __uint_as_float(t[1]);
+        dst[4 * j + 2] = __uint_as_float(t[2]);
+        dst[4 * j + 3] = __uint_as_float(t[3]);
     }
 }
 
@@ -139,10 +174,11 @@ struct RowsTile {
 // phase 1 for the offset whose first shifted sample is sh[SH]
 template <class C, int SH>
 __device__ __forceinline__ void rows_phase1(const float (&cen)[C::ROWS][C::EX], const float (&sh)[C::ROWS][C::NS],
-                                            float *SX, const int (&sxw)[C::ROWS])
+                                            float *SX, const int (&sxw)[C::ROWS], const bool (&ok)[C::ROWS])
 {
 #pragma unroll
     for (int q = 0; q < C::ROWS; ++q) {
+        if (!ok[q]) continue;
         float qv[C::EX];
 #pragma unroll
         for (int i = 0; i < C::EX; ++i) {
@@ -199,39 +235,52 @@ __device__ __forceinline__ void rows_phase2(const float *src, float *dst)
 }
 
 template <class C>
-__device__ __forceinline__ void rows_tile(const RowsTile &a, float *lds)
+__device__ __forceinline__ void rows_tile(const RowsTile &ta, float *lds)
 {
+    const RowsTile a = ta;  // registers: no reloads through a generic pointer in the loops
     constexpr int RP = C::RP, RS = C::RS, RZ = C::RZ, NT = kRowsThreads;
     float *SX = lds;
     float *SXY = lds + C::SXN;
     const int tid = threadIdx.x;
-    const float *P = a.v.p;
-    const int px = a.v.px, pxy = a.v.pxy;
+    const PadRsrc rs = pad_rsrc(a.v);
+    gfloat *out = (gfloat *)a.out;
+    gfloat *Zacc = (gfloat *)a.Zacc;
+    const int px = __builtin_amdgcn_readfirstlane(a.v.px), pxy = __builtin_amdgcn_readfirstlane(a.v.pxy);
+    const float kw = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(a.k)));
+
+    // z planes of this tile inside the volume: a partial tile at the far z
+    // end skips the rows, items and loads that only feed discarded planes
+    // (wave-uniform: each wave covers 4 consecutive z_ext values in phase 1)
+    const int tzv = min(C::TZ, a.nz - a.z0);
 
     // ---- phase-1 rows: y_ext = r % 16, z_ext = r / 16 ----
     float cen[C::ROWS][C::EX];
-    unsigned rbase[C::ROWS];  // unsigned: loads take a uniform base + 32-bit lane offset
+    unsigned rbase[C::ROWS];  // byte offsets: loads take a uniform base + 32-bit lane offset
     int sxw[C::ROWS];
+    bool p1ok[C::ROWS];
 #pragma unroll
     for (int q = 0; q < C::ROWS; ++q) {
         const int r = tid + q * NT;
         const int yp = r % C::EY, zp = r / C::EY;
-        rbase[q] = (a.z0 - RP + zp + C::R) * pxy + (a.y0 - RP + yp + C::R) * px + a.x0;
+        p1ok[q] = zp < tzv + 2 * RP;
+        rbase[q] = 4u * (unsigned)((a.z0 - RP + zp + C::R) * pxy + (a.y0 - RP + yp + C::R) * px + a.x0);
         sxw[q] = zp * C::SXP + yp * C::SXR;
         float row[C::NS];
-        load_row<C::NS>(P + rbase[q], row);
+        pad_load_row<C::NS>(rs, rbase[q], 0, row);
 #pragma unroll
-        for (int i = 0; i < C::EX; ++i) cen[q][i] = row[C::LP - RP + i];
+        for (int i = 0; i < C::EX; ++i) cen[q][i] = p1ok[q] ? row[C::LP - RP + i] : 0.0f;
     }
 
     // ---- phase-2 items: lanes run x pair fastest, then z_ext ----
     int p2src[C::P2R], p2dst[C::P2R];
+    bool p2ok[C::P2R];
 #pragma unroll
     for (int m = 0; m < C::P2R; ++m) {
         const int it = tid + m * NT;
         const int xv = it % C::NXV;
         const int zp2 = (it / C::NXV) % C::EZ;
         const int yr2 = it / (C::NXV * C::EZ);
+        p2ok[m] = zp2 < tzv + 2 * RP;
         p2src[m] = zp2 * C::SXP + yr2 * C::RY * C::SXR + xv * C::VX;
         p2dst[m] = zp2 * C::SYP + yr2 * C::RY * 16 + xv * C::VX;
     }
@@ -243,11 +292,11 @@ __device__ __forceinline__ void rows_tile(const RowsTile &a, float *lds)
 #pragma unroll
     for (int k = 0; k < C::P3R; ++k) {
         const int it = tid + k * NT;
-        p3ok[k] = (C::P3N % NT == 0) || it < C::P3N;
-        const int itc = p3ok[k] ? it : C::P3N - 1;
+        const int itc = ((C::P3N % NT == 0) || it < C::P3N) ? it : C::P3N - 1;
         const int x = itc % 16, y = (itc / 16) % C::TY, zr = itc / (16 * C::TY);
+        p3ok[k] = ((C::P3N % NT == 0) || it < C::P3N) && zr * RZ < tzv;
         p3r[k] = zr * RZ * C::SYP + y * 16 + x;
-        cb[k] = (a.z0 + zr * RZ + C::R) * pxy + (a.y0 + y + C::R) * px + a.x0 + x + C::LP;
+        cb[k] = 4u * (unsigned)((a.z0 + zr * RZ + C::R) * pxy + (a.y0 + y + C::R) * px + a.x0 + x + C::LP);
     }
 
     for (int dz = -RS; dz <= RS; ++dz) {
@@ -262,26 +311,23 @@ __device__ __forceinline__ void rows_tile(const RowsTile &a, float *lds)
         for (int dy = -RS; dy <= RS; ++dy) {
             const int soff = dz * pxy + dy * px;
             float sh[C::ROWS][C::NS];
-            const float *Ps = P + soff;  // uniform base, per-lane offset rbase[q]
 #pragma unroll
-            for (int q = 0; q < C::ROWS; ++q) load_row<C::NS>(Ps + rbase[q], sh[q]);
+            for (int q = 0; q < C::ROWS; ++q) pad_load_row<C::NS>(rs, rbase[q], 4 * soff, sh[q]);
 
             // dx unrolled: the shift into sh is a compile-time register index
             static_for<-RS, RS>([&](auto dxc) {
                 constexpr int DX = decltype(dxc)::value;
-                rows_phase1<C, C::LP - RP + DX>(cen, sh, SX, sxw);
+                rows_phase1<C, C::LP - RP + DX>(cen, sh, SX, sxw, p1ok);
                 // candidate samples u~(x + d), in flight across the barriers
-                const float *Pc = P + (soff + DX);
                 float cand[C::P3R][RZ];
 #pragma unroll
-                for (int o = 0; o < RZ; ++o) {
-                    const float *Po = Pc + o * pxy;  // uniform base, per-lane offset cb[k]
+                for (int o = 0; o < RZ; ++o)
 #pragma unroll
-                    for (int k = 0; k < C::P3R; ++k) cand[k][o] = Po[cb[k]];
-                }
+                    for (int k = 0; k < C::P3R; ++k) cand[k][o] = pad_load(rs, cb[k], 4 * (soff + DX + o * pxy));
                 lds_barrier();
 #pragma unroll
-                for (int m = 0; m < C::P2R; ++m) rows_phase2<C>(SX + p2src[m], SXY + p2dst[m]);
+                for (int m = 0; m < C::P2R; ++m)
+                    if (p2ok[m]) rows_phase2<C>(SX + p2src[m], SXY + p2dst[m]);
                 lds_barrier();
 #pragma unroll
                 for (int k = 0; k < C::P3R; ++k) {
@@ -293,7 +339,7 @@ __device__ __forceinline__ void rows_tile(const RowsTile &a, float *lds)
                         box_tree<C::W, RZ>(v, S);
 #pragma unroll
                         for (int o = 0; o < RZ; ++o) {
-                            const float w = exp2_fast(S[o] * a.k);
+                            const float w = exp2_fast(S[o] * kw);
                             Zd[k][o] = Zd[k][o] + w;
                             Yd[k][o] = fmaf(w, cand[k][o], Yd[k][o]);
                         }
@@ -307,18 +353,18 @@ __device__ __forceinline__ void rows_tile(const RowsTile &a, float *lds)
             const int it = tid + k * NT;
             const int x = it % 16, y = (it / 16) % C::TY, zr = it / (16 * C::TY);
             const int gx = a.x0 + x, gy = a.y0 + y, gz = a.z0 + zr * RZ;
-            if (!p3ok[k] || gx >= a.nx || gy >= a.ny) continue;
+            if (!p3ok[k] || it >= C::P3N || gx >= a.nx || gy >= a.ny) continue;
 #pragma unroll
             for (int o = 0; o < RZ; ++o) {
                 if (gz + o >= a.nz) continue;
                 const int oi = gx + a.nx * (gy + a.ny * (gz + o));
                 float Z = Zd[k][o], Y = Yd[k][o];
                 if (dz != -RS) {
-                    Z = a.Zacc[oi] + Z;
-                    Y = a.out[oi] + Y;
+                    Z = Zacc[oi] + Z;
+                    Y = out[oi] + Y;
                 }
-                a.Zacc[oi] = Z;
-                a.out[oi] = (dz == RS) ? Y / Z : Y;
+                Zacc[oi] = Z;
+                out[oi] = (dz == RS) ? Y / Z : Y;
             }
         }
     }

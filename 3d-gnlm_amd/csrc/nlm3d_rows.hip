@@ -79,7 +79,7 @@ hipError_t rows_launch_t(const InDesc &in, float *out, float *Zout, float k, hip
     const int pz = a.ntz * C::TZ + 2 * C::R;
     const long long npad = (long long)px * py * pz;
     const long long nv = (long long)in.nx * in.ny * in.nz;
-    if (npad >= (1LL << 31) || nv >= (1LL << 31)) return hipErrorInvalidValue;
+    if (npad >= (1LL << 30) || nv >= (1LL << 31)) return hipErrorInvalidValue;
     const size_t bytes = sizeof(float) * (size_t)(npad + (Zout ? 0 : nv));
     float *ws = nullptr;
     hipError_t e = hipMallocAsync((void **)&ws, bytes, s);
@@ -87,6 +87,7 @@ hipError_t rows_launch_t(const InDesc &in, float *out, float *Zout, float k, hip
     a.v.p = ws;
     a.v.px = px;
     a.v.pxy = px * py;
+    a.v.bytes = (unsigned)(sizeof(float) * npad);
     a.Zacc = Zout ? Zout : ws + npad;
     long long g = (npad + 255) / 256;
     if (g > 65536) g = 65536;

@@ -2,9 +2,14 @@
 
 Bars: bit-identical outputs between the graph-replayed persistent launch, its
 eager launch, and per-volume gnlm3d_forward calls (whatever tile variant the
-table selects); against the oracle, relative error floored at 1e-3 of the
-output range <= 1e-5 vs the CPU FP32 path and <= 1e-4 vs FP64 (the FP32
-arithmetic's own distance from FP64)."""
+table selects).  Against the oracle (relative error floored at 1e-3 of the
+output range): <= 1e-5 vs FP64 evaluated at the float32 h the FP32 paths
+receive, and vs the CPU FP32 path <= 1e-5 plus that path's own distance from
+the same FP64 result.  These volumes use h = sigma down to 0.02, where the
+weights exp(-S/(P h^2)) make the output sensitive to the last bit of the
+exponent scale: the CPU FP32 path rounds h^2 in float while the kernels fold
+log2(e)/(P h^2) in double, and the two FP32 results can sit on opposite sides
+of the FP64 value, ~1e-5 apart, while each is within ~6e-6 of it."""
 import numpy as np
 import pytest
 import torch
@@ -62,12 +67,15 @@ def test_batch_matches_oracle(dev):
     torch.cuda.synchronize()
     for (u, rp, rs, h), o in zip(vols, outs):
         un = u.cpu().numpy()
-        ref32 = oracle.nlm3d_separable_f32(un, rp, rs, h)
-        ref64 = oracle.nlm3d_separable_f64(un, rp, rs, h)
+        h32 = float(np.float32(h))  # the value both FP32 paths receive
+        ref32 = oracle.nlm3d_separable_f32(un, rp, rs, h32)
+        ref64 = oracle.nlm3d_separable_f64(un, rp, rs, h32)
         got = o.cpu().numpy()
-        print("batch vol", un.shape, rp, rs, "vs fp32", report(got, ref32), "vs fp64", report(got, ref64))
-        assert max_rel_fl(got, ref32) <= 1e-5, report(got, ref32)
-        assert max_rel_fl(got, ref64) <= 1e-4, report(got, ref64)
+        cpu_err = max_rel_fl(ref32, ref64)
+        print("batch vol", un.shape, rp, rs, "vs fp32", report(got, ref32), "vs fp64", report(got, ref64),
+              "cpu fp32 vs fp64", cpu_err)
+        assert max_rel_fl(got, ref64) <= 1e-5, report(got, ref64)
+        assert max_rel_fl(got, ref32) <= 1e-5 + cpu_err, report(got, ref32)
     plan.close()
 
 
