@@ -48,10 +48,18 @@ hipError_t fwd2d_tiled_launch(const InDesc &in, float *out, float *Zout, int r_p
 hipError_t fwd_generic_launch(const InDesc &in, float *out, float *Zout, int rp, int rs, int rpz, int rsz,
                               float k, hipStream_t s);
 
-// ---- adjoint (nlm3d_bwd.hip) -------------------------------------------------
+// ---- adjoint (nlm3d_bwd.hip, nlm3d_bwd_rows.hip) ----------------------------
 bool bwd3d_supported(int r_p, int r_s);
 hipError_t bwd3d_launch(const float *u, const float *y, const float *Z, const float *g, int nx, int ny, int nz,
                         int rp, int rs, float h, float *gu, float *gh, hipStream_t s);
+// rows-organised adjoint for r_p = 1, r_s in 1..7 (bwd3d_launch dispatches to it)
+bool bwd3d_rows_supported(int r_p, int r_s);
+hipError_t bwd3d_rows_launch(const float *u, const float *y, const float *Z, const float *g, int nx, int ny, int nz,
+                             int r_p, int r_s, float h, float *gu, float *gh, hipStream_t s);
+// shared tail kernels: fold the extended-domain gradient onto the volume
+// through the reflect-101 preimages; fixed-order sum of dL/dh partials x scale
+hipError_t bwd3d_fold_launch(const float *gext, float *gu, int nx, int ny, int nz, int R, hipStream_t s);
+hipError_t bwd3d_reduce_dh_launch(const float *part, int n, float scale, float *out, hipStream_t s);
 
 // ---- misc (gnlm_misc.hip) --------------------------------------------------
 hipError_t u8_to_f32_launch(const uint8_t *in, float *out, long long n, hipStream_t s);

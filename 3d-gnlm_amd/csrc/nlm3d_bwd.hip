@@ -405,12 +405,29 @@ int bwd_blocks(int nx, int ny, int nz, int rs)
 
 bool bwd3d_supported(int r_p, int r_s)
 {
-    return r_p >= 0 && r_p <= 2 && r_s >= 0 && bwd_lds_bytes_rp(r_p, r_s) <= kLdsLimitBytes;
+    return bwd3d_rows_supported(r_p, r_s) ||
+           (r_p >= 0 && r_p <= 2 && r_s >= 0 && bwd_lds_bytes_rp(r_p, r_s) <= kLdsLimitBytes);
+}
+
+hipError_t bwd3d_fold_launch(const float *gext, float *gu, int nx, int ny, int nz, int R, hipStream_t s)
+{
+    const long long nv = (long long)nx * ny * nz;
+    long long gblk = (nv + 255) / 256;
+    if (gblk > 65536) gblk = 65536;
+    hipLaunchKernelGGL(fold_grad, dim3((unsigned)gblk), dim3(256), 0, s, gext, gu, nx, ny, nz, R);
+    return hipGetLastError();
+}
+
+hipError_t bwd3d_reduce_dh_launch(const float *part, int n, float scale, float *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(reduce_dh, dim3(1), dim3(256), 0, s, part, n, scale, out);
+    return hipGetLastError();
 }
 
 hipError_t bwd3d_launch(const float *u, const float *y, const float *Z, const float *g, int nx, int ny, int nz,
                         int rp, int rs, float h, float *gu, float *gh, hipStream_t s)
 {
+    if (bwd3d_rows_supported(rp, rs)) return bwd3d_rows_launch(u, y, Z, g, nx, ny, nz, rp, rs, h, gu, gh, s);
     const int R = rp + rs;
     const long long nv = (long long)nx * ny * nz;
     const long long ext = (nx + 2LL * R) * (ny + 2LL * R) * (nz + 2LL * R);

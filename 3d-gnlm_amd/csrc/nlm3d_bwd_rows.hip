@@ -1,0 +1,354 @@
+// nlm3d_bwd_rows.hip -- adjoint of the 3D NLM for patch radius 1, organised
+// like the rows forward (nlm3d_rows.h).
+//
+// Same gather form as nlm3d_bwd.hip (3DNLM/main.cu:52-68,107-111,125 is the
+// forward being differentiated):
+//   grad(z) = sum_d [ G(z+d) w_d(z) + 2 boxE_d(z) (u~(z) - u~(z+d)) ]
+//   E_d(x)  = -c w_d(x) [ G(x) u~(x+d) - GY(x) + G(x+d) u~(x) - GY(x+d) ]
+//   dL/dh   = (2c/h) sum_x sum_d w_d(x) S_d(x) (G(x) u~(x+d) - GY(x))
+// with G = g/Z and GY = g*y/Z zero outside the volume, on the extended
+// domain [-R, n+R) (R = r_p + r_s), folded back onto the volume through the
+// reflect-101 preimages afterwards.
+//
+// Organisation (one workgroup of 256 threads per gradient tile T of
+// 14 x 14 x 10 positions; Q = T grown by r_p = 16 x 16 x 12 is where S, w
+// and E live; D = Q grown by r_p holds the squared differences):
+//   * a pre-pass writes a padded copy of u~ and a padded, interleaved (G, GY)
+//     array (zero outside the volume); the tile reads both with buffer loads
+//     whose per-lane offsets are fixed for the tile;
+//   * per offset d (dz, dy, dx):
+//       phase 1  x box sums of squared differences on the 18 x 14 rows of D
+//                -> LDS; the centre samples of a row stay in registers, the
+//                shifted ones are read per offset (L1 hits across dx);
+//       phase 2  y box sums -> LDS (the forward's phase 2);
+//       phase 3  one thread per Q column (x = lane in a 16-lane row, y = row
+//                of lanes): z box sums -> S, w, E, the G(z+d) w term and the
+//                dL/dh terms; the x box of E with DPP row shifts, its z box in
+//                registers -> LDS;
+//       phase 4  the same thread: y box of that from LDS and the
+//                2 boxE (u~(z) - u~(z+d)) term.
+//     Three barriers per offset.  G, GY and u~ of a thread's own column stay
+//     in registers; the values at x + d come with buffer loads issued before
+//     phase 1, so their latency hides behind phases 1 and 2.
+// Nothing of size voxels x offsets is stored.  Every position of the
+// extended domain belongs to exactly one tile; dL/dh partials are summed per
+// lane, then per workgroup in a fixed order, so results are deterministic.
+#include "gnlm_internal.h"
+#include "nlm3d_rows.h"
+#include "nlm3d_tile.h"
+
+namespace gnlm {
+namespace {
+
+constexpr int NT = 256;
+
+template <int RS_>
+struct BwdRowsCfg {
+    static constexpr int RP = 1, RS = RS_, W = 3, R = RP + RS;
+    static constexpr int H = 2 * R + RP;                  // halo of the padded copies (all axes)
+    // gradient tile T, grown tile Q, phase-1 rows D
+    static constexpr int QY = 16, QZ = 12;
+    static constexpr int GX = 16 - 2 * RP, GY = QY - 2 * RP, GZ = QZ - 2 * RP;
+    static constexpr int DY = QY + 2 * RP, DZ = QZ + 2 * RP;
+    // names the shared phase-1/2 code uses
+    static constexpr int TX = 16;                         // x outputs per phase-1 row (Q's width)
+    static constexpr int EX = TX + 2 * RP;                // centre samples per row
+    static constexpr int NROWS = DY * DZ;
+    static_assert(NROWS <= NT, "one phase-1 row per thread");
+    static constexpr int VX = 2, RY = 8, NXV = TX / VX;
+    static constexpr int P2N = NXV * DZ * (QY / RY);
+    static_assert(P2N <= NT, "one phase-2 item per thread");
+    // LDS: s_x [z_d][y_d][x] (row pitch 20, plane pitch 400 == 16 mod 64),
+    //      s_xy [z_d][y_q][x] (plane pitch 272), boxed E [z_t][y_q][x]
+    static constexpr int SXR = 20, SXP = 400, SXN = SXP * DZ;
+    static constexpr int SYP = 16 * QY + 16, SYN = SYP * DZ;
+    static constexpr int EZP = 16 * QY, EZN = EZP * GZ;
+    static constexpr int LDS_FLOATS = SXN + 16 + SYN + EZN;  // + a spare s_x row
+};
+
+struct BwdRowsArgs {
+    PadVol pu;       // padded u~: ext position (x, y, z) at (z+H)*pxy + (y+H)*px + (x+H)
+    PadVol pg;       // padded (G, GY), same geometry, float2 elements
+    float *gext;     // gradient on the extended domain, (n + 2R)^3
+    float *dh_part;  // one partial per workgroup
+    int nx, ny, nz;
+    float k;         // w = exp2(S * k)
+    float negc2;     // -2c: E carries the factor 2 of the B term
+    int ntx, nty, ntz;
+};
+
+__device__ __forceinline__ float2 pad_load2(const PadRsrc &r, unsigned voff, int soff)
+{
+    const auto t = __builtin_amdgcn_raw_buffer_load_b64(r.r, (int)voff, soff, 0);
+    return make_float2(__uint_as_float(t[0]), __uint_as_float(t[1]));
+}
+
+template <int RS>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void nlm3d_bwd_rows(BwdRowsArgs a)
+{
+    using C = BwdRowsCfg<RS>;
+    constexpr int RP = C::RP, QZ = C::QZ, GZ = C::GZ;
+    extern __shared__ __align__(16) float lds[];
+    float *SX = lds;
+    float *SXY = SX + C::SXN + 16;
+    float *EZB = SXY + C::SYN;
+    const int tid = threadIdx.x;
+    const PadRsrc ru = pad_rsrc(a.pu);
+    const PadRsrc rg = pad_rsrc(a.pg);
+    const int px = __builtin_amdgcn_readfirstlane(a.pu.px), pxy = __builtin_amdgcn_readfirstlane(a.pu.pxy);
+    const float kw = a.k, negc2 = a.negc2;
+
+    const int t = xcd_tile(blockIdx.x, a.ntx * a.nty * a.ntz);
+    // gradient tile origin on the extended domain
+    const int Xt = -C::R + (t % a.ntx) * C::GX;
+    const int Yt = -C::R + ((t / a.ntx) % a.nty) * C::GY;
+    const int Zt = -C::R + (t / (a.ntx * a.nty)) * C::GZ;
+    auto pidx = [&](int x, int y, int z) { return (z + C::H) * pxy + (y + C::H) * px + (x + C::H); };
+
+    // ---- phase-1 row (y_d, z_d) of D; element m of a loaded row is x = Xt - RP - R + m ----
+    const bool p1ok = tid < C::NROWS;
+    const int r1 = p1ok ? tid : 0;
+    const int yd = r1 % C::DY, zd = r1 / C::DY;
+    // surplus threads (tid >= NROWS) compute row 0 again into a spare LDS row
+    // per-lane offsets sit at d = (-RS, -RS, -RS): the scalar offsets added
+    // below (unsigned in the buffer address) are then all >= 0
+    const unsigned rbase = 4u * (unsigned)pidx(Xt - RP - C::R, Yt - 2 * RP + yd - RS, Zt - 2 * RP + zd - RS);
+    const int sxw = p1ok ? zd * C::SXP + yd * C::SXR : C::SXN;
+    float cen[C::EX];
+#pragma unroll
+    for (int i = 0; i < C::EX; ++i) cen[i] = pad_load(ru, rbase, 4 * (RS * pxy + RS * px + RS + i));
+
+    // ---- phase-2 item ----
+    const bool p2ok = tid < C::P2N;
+    const int it2 = p2ok ? tid : 0;
+    const int xv = it2 % C::NXV, zp2 = (it2 / C::NXV) % C::DZ, yr2 = it2 / (C::NXV * C::DZ);
+    const int p2src = zp2 * C::SXP + yr2 * C::RY * C::SXR + xv * C::VX;
+    const int p2dst = zp2 * C::SYP + yr2 * C::RY * 16 + xv * C::VX;
+
+    // ---- phase-3/4 column: x_q = lane within a 16-lane row, y_q = row ----
+    const int xq = tid % 16, yq = tid / 16;
+    const bool inT = xq >= RP && xq < 16 - RP && yq >= RP && yq < C::QY - RP;
+    const int cx = Xt - RP + xq, cy = Yt - RP + yq, cz0 = Zt - RP;
+    const unsigned colU = 4u * (unsigned)pidx(cx - RS, cy - RS, cz0 - RS);
+    const unsigned colG = 8u * (unsigned)pidx(cx - RS, cy - RS, cz0 - RS);
+    const int s0 = RS * pxy + RS * px + RS;  // d = 0
+    float uc[QZ], Gc[QZ], GYc[QZ];
+#pragma unroll
+    for (int z = 0; z < QZ; ++z) {
+        uc[z] = pad_load(ru, colU, 4 * (s0 + z * pxy));
+        const float2 g = pad_load2(rg, colG, 8 * (s0 + z * pxy));
+        Gc[z] = g.x;
+        GYc[z] = g.y;
+    }
+    const int sxyr = yq * 16 + xq;  // s_xy column base
+    const int ezw = yq * 16 + xq;   // boxed-E column base
+    const int ezlo = (yq > 0 ? yq - 1 : 0) * 16 + xq;        // y neighbours (clamped in
+    const int ezhi = (yq < C::QY - 1 ? yq + 1 : yq) * 16 + xq;  // the halo rows, unused)
+    float gacc[GZ];
+#pragma unroll
+    for (int z = 0; z < GZ; ++z) gacc[z] = 0.0f;
+    float dh = 0.0f;
+
+    for (int dz = -RS; dz <= RS; ++dz)
+        for (int dy = -RS; dy <= RS; ++dy)
+            for (int dx = -RS; dx <= RS; ++dx) {
+                const int sd = (dz + RS) * pxy + (dy + RS) * px + (dx + RS);  // >= 0
+                // values at x + d for this thread's column, in flight across phases 1-2
+                float ud[QZ], Gd[QZ], GYd[QZ];
+#pragma unroll
+                for (int z = 0; z < QZ; ++z) {
+                    ud[z] = pad_load(ru, colU, 4 * (sd + z * pxy));
+                    const float2 g = pad_load2(rg, colG, 8 * (sd + z * pxy));
+                    Gd[z] = g.x;
+                    GYd[z] = g.y;
+                }
+                // phase 1: x sums of squared differences on the rows of D (the
+                // shifted samples come straight from the copy: L1 hits across dx)
+                {
+                    float qv[C::EX];
+#pragma unroll
+                    for (int i = 0; i < C::EX; ++i) {
+                        const float e = cen[i] - pad_load(ru, rbase, 4 * (sd + i));
+                        qv[i] = e * e;
+                    }
+                    float sx[C::TX];
+                    box_tree<3, C::TX>(qv, sx);
+                    float4 *dst = reinterpret_cast<float4 *>(SX + sxw);
+#pragma unroll
+                    for (int j = 0; j < C::TX / 4; ++j)
+                        dst[j] = make_float4(sx[4 * j], sx[4 * j + 1], sx[4 * j + 2], sx[4 * j + 3]);
+                }
+                lds_barrier();
+                // phase 2: y sums
+                if (p2ok) rows_phase2<C>(SX + p2src, SXY + p2dst);
+                lds_barrier();
+                // phase 3: z sums, weights, E, the G(z+d) w term, dL/dh terms
+                float v[QZ + 2 * RP];
+#pragma unroll
+                for (int j = 0; j < QZ + 2 * RP; ++j) v[j] = SXY[j * C::SYP + sxyr];
+                float S[QZ];
+                box_tree<3, QZ>(v, S);
+                float EX[QZ];
+#pragma unroll
+                for (int z = 0; z < QZ; ++z) {
+                    const float w = exp2_fast(S[z] * kw);
+                    const float t1 = fmaf(Gc[z], ud[z], -GYc[z]);
+                    const float t2 = fmaf(Gd[z], uc[z], -GYd[z]);
+                    const float E = (negc2 * w) * (t1 + t2);
+                    if (z >= RP && z < QZ - RP) {
+                        gacc[z - RP] = fmaf(Gd[z], w, gacc[z - RP]);
+                        dh = fmaf(w * S[z], t1, dh);
+                    }
+                    EX[z] = dpp_box<1>(E);  // x box across the 16-lane row
+                }
+                float EZ[GZ];
+                box_tree<3, GZ>(EX, EZ);
+#pragma unroll
+                for (int z = 0; z < GZ; ++z) EZB[z * C::EZP + ezw] = EZ[z];
+                lds_barrier();
+                // phase 4: y box of the boxed E, the 2 boxE (u~(z) - u~(z+d)) term
+#pragma unroll
+                for (int z = 0; z < GZ; ++z) {
+                    const float bE = (EZB[z * C::EZP + ezlo] + EZB[z * C::EZP + ezw]) + EZB[z * C::EZP + ezhi];
+                    gacc[z] = fmaf(bE, uc[z + RP] - ud[z + RP], gacc[z]);
+                }
+            }
+
+    // ---- gradient of the tile's extended-domain positions ----
+    if (inT) {
+        const long long ex = a.nx + 2LL * C::R, ey = a.ny + 2LL * C::R, ez = a.nz + 2LL * C::R;
+        const long long x = cx + C::R, y = cy + C::R;
+        if (x < ex && y < ey) {
+#pragma unroll
+            for (int z = 0; z < GZ; ++z) {
+                const long long zz = cz0 + RP + z + C::R;
+                if (zz < ez) a.gext[x + ex * (y + ey * zz)] = gacc[z];
+            }
+        }
+    }
+    // ---- dL/dh: positions of T only (G, GY vanish outside the volume) ----
+    __syncthreads();
+    float *red = lds;
+    red[tid] = inT ? dh : 0.0f;
+    __syncthreads();
+    for (int s = NT / 2; s > 0; s >>= 1) {
+        if (tid < s) red[tid] = red[tid] + red[tid + s];
+        __syncthreads();
+    }
+    if (tid == 0) a.dh_part[blockIdx.x] = red[0];
+}
+
+// Padded u~ (reflect-101 on [-R, n+R), 0 beyond) and padded (G, GY) = (g/Z,
+// g*y/Z) inside the volume, (0, 0) outside; both with origin offset H.
+__global__ void bwd_pad(const float *u, const float *g, const float *y, const float *Z, int nx, int ny, int nz,
+                        float *pu, float2 *pg, int px, int py, int pz, int H, int R)
+{
+    const long long n = (long long)px * py * pz;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const int x = (int)(i % px) - H;
+        const long long q = i / px;
+        const int yy = (int)(q % py) - H;
+        const int z = (int)(q / py) - H;
+        float v = 0.0f;
+        float2 gg = make_float2(0.0f, 0.0f);
+        if (x >= -R && x < nx + R && yy >= -R && yy < ny + R && z >= -R && z < nz + R) {
+            const long long o = reflect101(x, nx) + (long long)nx * (reflect101(yy, ny) + (long long)ny * reflect101(z, nz));
+            v = u[o];
+            if (x >= 0 && x < nx && yy >= 0 && yy < ny && z >= 0 && z < nz) {
+                const float G = g[o] / Z[o];
+                gg = make_float2(G, G * y[o]);
+            }
+        }
+        pu[i] = v;
+        pg[i] = gg;
+    }
+}
+
+template <int RS>
+constexpr size_t bwd_rows_lds()
+{
+    return BwdRowsCfg<RS>::LDS_FLOATS * sizeof(float);
+}
+
+template <int RS>
+hipError_t bwd_rows_t(const float *u, const float *y, const float *Z, const float *g, int nx, int ny, int nz,
+                      float h, float *gu, float *gh, hipStream_t s)
+{
+    using C = BwdRowsCfg<RS>;
+    BwdRowsArgs a;
+    a.nx = nx;
+    a.ny = ny;
+    a.nz = nz;
+    a.ntx = (nx + 2 * C::R + C::GX - 1) / C::GX;
+    a.nty = (ny + 2 * C::R + C::GY - 1) / C::GY;
+    a.ntz = (nz + 2 * C::R + C::GZ - 1) / C::GZ;
+    // positions read: [-H, nt * G + RP) on each axis
+    const int px = a.ntx * C::GX + C::RP + C::H, py = a.nty * C::GY + C::RP + C::H,
+              pz = a.ntz * C::GZ + C::RP + C::H;
+    const long long npad = (long long)px * py * pz;
+    const long long ext = (nx + 2LL * C::R) * (ny + 2LL * C::R) * (nz + 2LL * C::R);
+    const int nblocks = a.ntx * a.nty * a.ntz;
+    // 32-bit byte offsets into the (G, GY) copy (8 bytes per position)
+    if (8 * npad >= (1LL << 32)) return hipErrorInvalidValue;
+    const int P = 27;
+    const double c = 1.0 / ((double)P * (double)h * (double)h);
+    a.k = (float)weight_scale(P, h);
+    a.negc2 = (float)(-2.0 * c);
+    float *ws = nullptr;
+    // layout: (G, GY) copy | u~ copy | gradient on the extended domain | dL/dh partials
+    const size_t nfl = (size_t)(2 * npad + npad + ext + nblocks);
+    hipError_t e = hipMallocAsync((void **)&ws, sizeof(float) * nfl, s);
+    if (e != hipSuccess) return e;
+    float2 *pg = reinterpret_cast<float2 *>(ws);
+    float *pu = ws + 2 * npad;
+    a.gext = pu + npad;
+    a.dh_part = a.gext + ext;
+    a.pu = PadVol{pu, px, px * py, (unsigned)(4 * npad)};
+    a.pg = PadVol{reinterpret_cast<const float *>(pg), px, px * py, (unsigned)(8 * npad)};
+    long long gb = (npad + 255) / 256;
+    if (gb > 65536) gb = 65536;
+    hipLaunchKernelGGL(bwd_pad, dim3((unsigned)gb), dim3(256), 0, s, u, g, y, Z, nx, ny, nz, pu, pg, px, py, pz, C::H,
+                       C::R);
+    e = hipGetLastError();
+    if (e == hipSuccess) {
+        const size_t lds = bwd_rows_lds<RS>();
+        e = hipFuncSetAttribute((const void *)nlm3d_bwd_rows<RS>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(nlm3d_bwd_rows<RS>, dim3(nblocks), dim3(NT), lds, s, a);
+            e = hipGetLastError();
+        }
+    }
+    if (e == hipSuccess) e = bwd3d_fold_launch(a.gext, gu, nx, ny, nz, C::R, s);
+    if (e == hipSuccess && gh) e = bwd3d_reduce_dh_launch(a.dh_part, nblocks, (float)(2.0 * c / (double)h), gh, s);
+    const hipError_t e2 = hipFreeAsync(ws, s);
+    return e != hipSuccess ? e : e2;
+}
+
+#define GNLM_BWD_ROWS_CASES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7)
+
+}  // namespace
+
+bool bwd3d_rows_supported(int r_p, int r_s)
+{
+    if (r_p != 1) return false;
+#define GNLM_BR_SUP(RS) \
+    if (r_s == RS) return bwd_rows_lds<RS>() <= kLdsLimitBytes;
+    GNLM_BWD_ROWS_CASES(GNLM_BR_SUP)
+#undef GNLM_BR_SUP
+    return false;
+}
+
+hipError_t bwd3d_rows_launch(const float *u, const float *y, const float *Z, const float *g, int nx, int ny, int nz,
+                             int r_p, int r_s, float h, float *gu, float *gh, hipStream_t s)
+{
+    if (r_p != 1) return hipErrorInvalidValue;
+#define GNLM_BR_RUN(RS) \
+    if (r_s == RS) return bwd_rows_t<RS>(u, y, Z, g, nx, ny, nz, h, gu, gh, s);
+    GNLM_BWD_ROWS_CASES(GNLM_BR_RUN)
+#undef GNLM_BR_RUN
+    return hipErrorInvalidValue;
+}
+
+}  // namespace gnlm

@@ -263,10 +263,11 @@ __device__ __forceinline__ void rows_tile(const RowsTile &ta, float *lds)
         const int r = tid + q * NT;
         const int yp = r % C::EY, zp = r / C::EY;
         p1ok[q] = zp < tzv + 2 * RP;
-        rbase[q] = 4u * (unsigned)((a.z0 - RP + zp + C::R) * pxy + (a.y0 - RP + yp + C::R) * px + a.x0);
+        // at offset (dy, dz) = (-RS, -RS): every scalar offset added below is >= 0
+        rbase[q] = 4u * (unsigned)((a.z0 - RP + zp + C::R - RS) * pxy + (a.y0 - RP + yp + C::R - RS) * px + a.x0);
         sxw[q] = zp * C::SXP + yp * C::SXR;
         float row[C::NS];
-        pad_load_row<C::NS>(rs, rbase[q], 0, row);
+        pad_load_row<C::NS>(rs, rbase[q], 4 * (RS * pxy + RS * px), row);
 #pragma unroll
         for (int i = 0; i < C::EX; ++i) cen[q][i] = p1ok[q] ? row[C::LP - RP + i] : 0.0f;
     }
@@ -296,7 +297,9 @@ __device__ __forceinline__ void rows_tile(const RowsTile &ta, float *lds)
         const int x = itc % 16, y = (itc / 16) % C::TY, zr = itc / (16 * C::TY);
         p3ok[k] = ((C::P3N % NT == 0) || it < C::P3N) && zr * RZ < tzv;
         p3r[k] = zr * RZ * C::SYP + y * 16 + x;
-        cb[k] = 4u * (unsigned)((a.z0 + zr * RZ + C::R) * pxy + (a.y0 + y + C::R) * px + a.x0 + x + C::LP);
+        // candidate u~(x + d) at d = (-RS, -RS, -RS)
+        cb[k] = 4u * (unsigned)((a.z0 + zr * RZ + C::R - RS) * pxy + (a.y0 + y + C::R - RS) * px + a.x0 + x + C::LP -
+                                RS);
     }
 
     for (int dz = -RS; dz <= RS; ++dz) {
@@ -309,10 +312,11 @@ __device__ __forceinline__ void rows_tile(const RowsTile &ta, float *lds)
                 Yd[k][o] = 0.0f;
             }
         for (int dy = -RS; dy <= RS; ++dy) {
-            const int soff = dz * pxy + dy * px;
+            // scalar offsets are unsigned in the buffer address: count from d = -RS
+            const int soffp = (dz + RS) * pxy + (dy + RS) * px;
             float sh[C::ROWS][C::NS];
 #pragma unroll
-            for (int q = 0; q < C::ROWS; ++q) pad_load_row<C::NS>(rs, rbase[q], 4 * soff, sh[q]);
+            for (int q = 0; q < C::ROWS; ++q) pad_load_row<C::NS>(rs, rbase[q], 4 * soffp, sh[q]);
 
             // dx unrolled: the shift into sh is a compile-time register index
             static_for<-RS, RS>([&](auto dxc) {
@@ -323,7 +327,7 @@ __device__ __forceinline__ void rows_tile(const RowsTile &ta, float *lds)
 #pragma unroll
                 for (int o = 0; o < RZ; ++o)
 #pragma unroll
-                    for (int k = 0; k < C::P3R; ++k) cand[k][o] = pad_load(rs, cb[k], 4 * (soff + DX + o * pxy));
+                    for (int k = 0; k < C::P3R; ++k) cand[k][o] = pad_load(rs, cb[k], 4 * (soffp + DX + RS + o * pxy));
                 lds_barrier();
 #pragma unroll
                 for (int m = 0; m < C::P2R; ++m)
