@@ -84,7 +84,7 @@ __device__ __forceinline__ float2 pad_load2(const PadRsrc &r, unsigned voff, int
 }
 
 template <int RS>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void nlm3d_bwd_rows(BwdRowsArgs a)
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void nlm3d_bwd_rows(BwdRowsArgs a)
 {
     using C = BwdRowsCfg<RS>;
     constexpr int RP = C::RP, QZ = C::QZ, GZ = C::GZ;
@@ -149,70 +149,100 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void nl
     for (int z = 0; z < GZ; ++z) gacc[z] = 0.0f;
     float dh = 0.0f;
 
-    for (int dz = -RS; dz <= RS; ++dz)
-        for (int dy = -RS; dy <= RS; ++dy)
-            for (int dx = -RS; dx <= RS; ++dx) {
-                const int sd = (dz + RS) * pxy + (dy + RS) * px + (dx + RS);  // >= 0
-                // values at x + d for this thread's column, in flight across phases 1-2
-                float ud[QZ], Gd[QZ], GYd[QZ];
+    // offsets in (dz, dy, dx) order as one flat loop; sd(o) is the scalar
+    // offset of d relative to d = (-RS, -RS, -RS) (always >= 0)
+    constexpr int NW = 2 * RS + 1, NOFF = NW * NW * NW;
+    // scalar offset of d relative to d = (-RS, -RS, -RS) (always >= 0), stepped
+    // through the offsets in (dz, dy, dx) order
+    auto sd_next = [&](int sd, int o) {  // offset index o -> o + 1
+        const int ix = o % NW;
+        if (ix < NW - 1) return sd + 1;
+        const int iy = (o / NW) % NW;
+        if (iy < NW - 1) return sd + px - (NW - 1);
+        return sd + pxy - (NW - 1) * px - (NW - 1);
+    };
+    auto load_col = [&](int sd, float *ud, float *Gd, float *GYd) {
 #pragma unroll
-                for (int z = 0; z < QZ; ++z) {
-                    ud[z] = pad_load(ru, colU, 4 * (sd + z * pxy));
-                    const float2 g = pad_load2(rg, colG, 8 * (sd + z * pxy));
-                    Gd[z] = g.x;
-                    GYd[z] = g.y;
-                }
-                // phase 1: x sums of squared differences on the rows of D (the
-                // shifted samples come straight from the copy: L1 hits across dx)
-                {
-                    float qv[C::EX];
+        for (int z = 0; z < QZ; ++z) {
+            ud[z] = pad_load(ru, colU, 4 * (sd + z * pxy));
+            const float2 g = pad_load2(rg, colG, 8 * (sd + z * pxy));
+            Gd[z] = g.x;
+            GYd[z] = g.y;
+        }
+    };
+    // The shifted samples of this thread's phase-1 row for all dx of one
+    // (dy, dz): loaded once (a lane reads its own row, so these loads touch a
+    // cache line per lane) and slid by one sample per dx in registers.
+    constexpr int NWIN = C::EX + 2 * RS;
+    float win[NWIN];
+    // the column values at x + d, one offset ahead
+    float ud[QZ], Gd[QZ], GYd[QZ];
+    load_col(0, ud, Gd, GYd);
+
+    int sd = 0;
+    for (int o = 0; o < NOFF; ++o) {
+        const int sn = (o + 1 < NOFF) ? sd_next(sd, o) : sd;  // next offset (reloads the last one at the end)
+        if (o % NW == 0) {  // dx = -RS: a new (dy, dz)
 #pragma unroll
-                    for (int i = 0; i < C::EX; ++i) {
-                        const float e = cen[i] - pad_load(ru, rbase, 4 * (sd + i));
-                        qv[i] = e * e;
-                    }
-                    float sx[C::TX];
-                    box_tree<3, C::TX>(qv, sx);
-                    float4 *dst = reinterpret_cast<float4 *>(SX + sxw);
+            for (int m = 0; m < NWIN; ++m) win[m] = pad_load(ru, rbase, 4 * (sd + m));
+        }
+        // phase 1: x sums of squared differences on the rows of D
+        {
+            float qv[C::EX];
 #pragma unroll
-                    for (int j = 0; j < C::TX / 4; ++j)
-                        dst[j] = make_float4(sx[4 * j], sx[4 * j + 1], sx[4 * j + 2], sx[4 * j + 3]);
-                }
-                lds_barrier();
-                // phase 2: y sums
-                if (p2ok) rows_phase2<C>(SX + p2src, SXY + p2dst);
-                lds_barrier();
-                // phase 3: z sums, weights, E, the G(z+d) w term, dL/dh terms
-                float v[QZ + 2 * RP];
-#pragma unroll
-                for (int j = 0; j < QZ + 2 * RP; ++j) v[j] = SXY[j * C::SYP + sxyr];
-                float S[QZ];
-                box_tree<3, QZ>(v, S);
-                float EX[QZ];
-#pragma unroll
-                for (int z = 0; z < QZ; ++z) {
-                    const float w = exp2_fast(S[z] * kw);
-                    const float t1 = fmaf(Gc[z], ud[z], -GYc[z]);
-                    const float t2 = fmaf(Gd[z], uc[z], -GYd[z]);
-                    const float E = (negc2 * w) * (t1 + t2);
-                    if (z >= RP && z < QZ - RP) {
-                        gacc[z - RP] = fmaf(Gd[z], w, gacc[z - RP]);
-                        dh = fmaf(w * S[z], t1, dh);
-                    }
-                    EX[z] = dpp_box<1>(E);  // x box across the 16-lane row
-                }
-                float EZ[GZ];
-                box_tree<3, GZ>(EX, EZ);
-#pragma unroll
-                for (int z = 0; z < GZ; ++z) EZB[z * C::EZP + ezw] = EZ[z];
-                lds_barrier();
-                // phase 4: y box of the boxed E, the 2 boxE (u~(z) - u~(z+d)) term
-#pragma unroll
-                for (int z = 0; z < GZ; ++z) {
-                    const float bE = (EZB[z * C::EZP + ezlo] + EZB[z * C::EZP + ezw]) + EZB[z * C::EZP + ezhi];
-                    gacc[z] = fmaf(bE, uc[z + RP] - ud[z + RP], gacc[z]);
-                }
+            for (int i = 0; i < C::EX; ++i) {
+                const float e = cen[i] - win[i];
+                qv[i] = e * e;
             }
+            float sx[C::TX];
+            box_tree<3, C::TX>(qv, sx);
+            float4 *dst = reinterpret_cast<float4 *>(SX + sxw);
+#pragma unroll
+            for (int j = 0; j < C::TX / 4; ++j)
+                dst[j] = make_float4(sx[4 * j], sx[4 * j + 1], sx[4 * j + 2], sx[4 * j + 3]);
+        }
+        // slide the window to the next dx
+#pragma unroll
+        for (int m = 0; m < NWIN - 1; ++m) win[m] = win[m + 1];
+        lds_barrier();
+        // phase 2: y sums
+        if (p2ok) rows_phase2<C>(SX + p2src, SXY + p2dst);
+        lds_barrier();
+        // phase 3: z sums, weights, E, the G(z+d) w term, dL/dh terms
+        float v[QZ + 2 * RP];
+#pragma unroll
+        for (int j = 0; j < QZ + 2 * RP; ++j) v[j] = SXY[j * C::SYP + sxyr];
+        float S[QZ];
+        box_tree<3, QZ>(v, S);
+        float EX[QZ], du[GZ];
+#pragma unroll
+        for (int z = 0; z < QZ; ++z) {
+            const float w = exp2_fast(S[z] * kw);
+            const float t1 = fmaf(Gc[z], ud[z], -GYc[z]);
+            const float t2 = fmaf(Gd[z], uc[z], -GYd[z]);
+            const float E = (negc2 * w) * (t1 + t2);
+            if (z >= RP && z < QZ - RP) {
+                gacc[z - RP] = fmaf(Gd[z], w, gacc[z - RP]);
+                dh = fmaf(w * S[z], t1, dh);
+                du[z - RP] = uc[z] - ud[z];
+            }
+            EX[z] = dpp_box<1>(E);  // x box across the 16-lane row
+        }
+        // the next offset's column values fly across phase 4 and the next phases 1-2
+        load_col(sn, ud, Gd, GYd);
+        float EZ[GZ];
+        box_tree<3, GZ>(EX, EZ);
+#pragma unroll
+        for (int z = 0; z < GZ; ++z) EZB[z * C::EZP + ezw] = EZ[z];
+        lds_barrier();
+        // phase 4: y box of the boxed E, the 2 boxE (u~(z) - u~(z+d)) term
+#pragma unroll
+        for (int z = 0; z < GZ; ++z) {
+            const float bE = (EZB[z * C::EZP + ezlo] + EZB[z * C::EZP + ezw]) + EZB[z * C::EZP + ezhi];
+            gacc[z] = fmaf(bE, du[z], gacc[z]);
+        }
+        sd = sn;
+    }
 
     // ---- gradient of the tile's extended-domain positions ----
     if (inT) {

@@ -262,7 +262,9 @@ __device__ __forceinline__ void rows_tile(const RowsTile &ta, float *lds)
     for (int q = 0; q < C::ROWS; ++q) {
         const int r = tid + q * NT;
         const int yp = r % C::EY, zp = r / C::EY;
-        p1ok[q] = zp < tzv + 2 * RP;
+        // z_ext grows with the lane index within a wave: the first lane decides
+        // for the whole wave, which keeps the branch scalar
+        p1ok[q] = __builtin_amdgcn_readfirstlane(zp < tzv + 2 * RP);
         // at offset (dy, dz) = (-RS, -RS): every scalar offset added below is >= 0
         rbase[q] = 4u * (unsigned)((a.z0 - RP + zp + C::R - RS) * pxy + (a.y0 - RP + yp + C::R - RS) * px + a.x0);
         sxw[q] = zp * C::SXP + yp * C::SXR;
@@ -281,7 +283,7 @@ __device__ __forceinline__ void rows_tile(const RowsTile &ta, float *lds)
         const int xv = it % C::NXV;
         const int zp2 = (it / C::NXV) % C::EZ;
         const int yr2 = it / (C::NXV * C::EZ);
-        p2ok[m] = zp2 < tzv + 2 * RP;
+        p2ok[m] = __builtin_amdgcn_readfirstlane(zp2 < tzv + 2 * RP);
         p2src[m] = zp2 * C::SXP + yr2 * C::RY * C::SXR + xv * C::VX;
         p2dst[m] = zp2 * C::SYP + yr2 * C::RY * 16 + xv * C::VX;
     }
@@ -295,7 +297,7 @@ __device__ __forceinline__ void rows_tile(const RowsTile &ta, float *lds)
         const int it = tid + k * NT;
         const int itc = ((C::P3N % NT == 0) || it < C::P3N) ? it : C::P3N - 1;
         const int x = itc % 16, y = (itc / 16) % C::TY, zr = itc / (16 * C::TY);
-        p3ok[k] = ((C::P3N % NT == 0) || it < C::P3N) && zr * RZ < tzv;
+        p3ok[k] = __builtin_amdgcn_readfirstlane(((C::P3N % NT == 0) || it < C::P3N) && zr * RZ < tzv);
         p3r[k] = zr * RZ * C::SYP + y * 16 + x;
         // candidate u~(x + d) at d = (-RS, -RS, -RS)
         cb[k] = 4u * (unsigned)((a.z0 + zr * RZ + C::R - RS) * pxy + (a.y0 + y + C::R - RS) * px + a.x0 + x + C::LP -
