@@ -13,9 +13,10 @@
 // Organisation (one workgroup of 256 threads per gradient tile T of
 // 14 x 14 x 10 positions; Q = T grown by r_p = 16 x 16 x 12 is where S, w
 // and E live; D = Q grown by r_p holds the squared differences):
-//   * a pre-pass writes a padded copy of u~ and a padded, interleaved (G, GY)
-//     array (zero outside the volume); the tile reads both with buffer loads
-//     whose per-lane offsets are fixed for the tile;
+//   * a pre-pass writes a padded copy of u~ and a padded, interleaved
+//     (G, GY, u~, 0) array ((G, GY) zero outside the volume); the tile reads
+//     both with buffer loads whose per-lane offsets are fixed for the tile
+//     (one 128-bit load per column position and offset);
 //   * per offset d (dz, dy, dx):
 //       phase 1  x box sums of squared differences on the 18 x 14 rows of D
 //                -> LDS; the centre samples of a row stay in registers, the
@@ -68,7 +69,7 @@ struct BwdRowsCfg {
 
 struct BwdRowsArgs {
     PadVol pu;       // padded u~: ext position (x, y, z) at (z+H)*pxy + (y+H)*px + (x+H)
-    PadVol pg;       // padded (G, GY), same geometry, float2 elements
+    PadVol pg;       // padded (G, GY, u~, 0), same geometry, float4 elements
     float *gext;     // gradient on the extended domain, (n + 2R)^3
     float *dh_part;  // one partial per workgroup
     int nx, ny, nz;
@@ -81,6 +82,12 @@ __device__ __forceinline__ float2 pad_load2(const PadRsrc &r, unsigned voff, int
 {
     const auto t = __builtin_amdgcn_raw_buffer_load_b64(r.r, (int)voff, soff, 0);
     return make_float2(__uint_as_float(t[0]), __uint_as_float(t[1]));
+}
+
+__device__ __forceinline__ float4 pad_load4(const PadRsrc &r, unsigned voff, int soff)
+{
+    const auto t = __builtin_amdgcn_raw_buffer_load_b128(r.r, (int)voff, soff, 0);
+    return make_float4(__uint_as_float(t[0]), __uint_as_float(t[1]), __uint_as_float(t[2]), __uint_as_float(t[3]));
 }
 
 template <int RS>
@@ -129,16 +136,15 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void nl
     const int xq = tid % 16, yq = tid / 16;
     const bool inT = xq >= RP && xq < 16 - RP && yq >= RP && yq < C::QY - RP;
     const int cx = Xt - RP + xq, cy = Yt - RP + yq, cz0 = Zt - RP;
-    const unsigned colU = 4u * (unsigned)pidx(cx - RS, cy - RS, cz0 - RS);
-    const unsigned colG = 8u * (unsigned)pidx(cx - RS, cy - RS, cz0 - RS);
+    const unsigned colG = 16u * (unsigned)pidx(cx - RS, cy - RS, cz0 - RS);
     const int s0 = RS * pxy + RS * px + RS;  // d = 0
     float uc[QZ], Gc[QZ], GYc[QZ];
 #pragma unroll
     for (int z = 0; z < QZ; ++z) {
-        uc[z] = pad_load(ru, colU, 4 * (s0 + z * pxy));
-        const float2 g = pad_load2(rg, colG, 8 * (s0 + z * pxy));
+        const float4 g = pad_load4(rg, colG, 16 * (s0 + z * pxy));
         Gc[z] = g.x;
         GYc[z] = g.y;
+        uc[z] = g.z;
     }
     const int sxyr = yq * 16 + xq;  // s_xy column base
     const int ezw = yq * 16 + xq;   // boxed-E column base
@@ -164,10 +170,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void nl
     auto load_col = [&](int sd, float *ud, float *Gd, float *GYd) {
 #pragma unroll
         for (int z = 0; z < QZ; ++z) {
-            ud[z] = pad_load(ru, colU, 4 * (sd + z * pxy));
-            const float2 g = pad_load2(rg, colG, 8 * (sd + z * pxy));
+            const float4 g = pad_load4(rg, colG, 16 * (sd + z * pxy));
             Gd[z] = g.x;
             GYd[z] = g.y;
+            ud[z] = g.z;
         }
     };
     // The shifted samples of this thread's phase-1 row for all dx of one
@@ -184,7 +190,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void nl
         const int sn = (o + 1 < NOFF) ? sd_next(sd, o) : sd;  // next offset (reloads the last one at the end)
         if (o % NW == 0) {  // dx = -RS: a new (dy, dz)
 #pragma unroll
-            for (int m = 0; m < NWIN; ++m) win[m] = pad_load(ru, rbase, 4 * (sd + m));
+            for (int m = 0; m < NWIN; m += 2) {  // row starts are 8-byte aligned (even padded x)
+                const float2 t = pad_load2(ru, rbase, 4 * (sd + m));
+                win[m] = t.x;
+                win[m + 1] = t.y;
+            }
         }
         // phase 1: x sums of squared differences on the rows of D
         {
@@ -268,10 +278,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void nl
     if (tid == 0) a.dh_part[blockIdx.x] = red[0];
 }
 
-// Padded u~ (reflect-101 on [-R, n+R), 0 beyond) and padded (G, GY) = (g/Z,
-// g*y/Z) inside the volume, (0, 0) outside; both with origin offset H.
+// Padded u~ (reflect-101 on [-R, n+R), 0 beyond) and the interleaved
+// (G, GY, u~, 0) with (G, GY) = (g/Z, g*y/Z) inside the volume and (0, 0)
+// outside; both with origin offset H.
 __global__ void bwd_pad(const float *u, const float *g, const float *y, const float *Z, int nx, int ny, int nz,
-                        float *pu, float2 *pg, int px, int py, int pz, int H, int R)
+                        float *pu, float4 *pg, int px, int py, int pz, int H, int R)
 {
     const long long n = (long long)px * py * pz;
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
@@ -291,7 +302,7 @@ __global__ void bwd_pad(const float *u, const float *g, const float *y, const fl
             }
         }
         pu[i] = v;
-        pg[i] = gg;
+        pg[i] = make_float4(gg.x, gg.y, v, 0.0f);
     }
 }
 
@@ -319,23 +330,23 @@ hipError_t bwd_rows_t(const float *u, const float *y, const float *Z, const floa
     const long long npad = (long long)px * py * pz;
     const long long ext = (nx + 2LL * C::R) * (ny + 2LL * C::R) * (nz + 2LL * C::R);
     const int nblocks = a.ntx * a.nty * a.ntz;
-    // 32-bit byte offsets into the (G, GY) copy (8 bytes per position)
-    if (8 * npad >= (1LL << 32)) return hipErrorInvalidValue;
+    // 32-bit byte offsets into the (G, GY, u~, 0) copy (16 bytes per position)
+    if (16 * npad >= (1LL << 32)) return hipErrorInvalidValue;
     const int P = 27;
     const double c = 1.0 / ((double)P * (double)h * (double)h);
     a.k = (float)weight_scale(P, h);
     a.negc2 = (float)(-2.0 * c);
     float *ws = nullptr;
-    // layout: (G, GY) copy | u~ copy | gradient on the extended domain | dL/dh partials
-    const size_t nfl = (size_t)(2 * npad + npad + ext + nblocks);
+    // layout: (G, GY, u~, 0) copy | u~ copy | gradient on the extended domain | dL/dh partials
+    const size_t nfl = (size_t)(4 * npad + npad + ext + nblocks);
     hipError_t e = hipMallocAsync((void **)&ws, sizeof(float) * nfl, s);
     if (e != hipSuccess) return e;
-    float2 *pg = reinterpret_cast<float2 *>(ws);
-    float *pu = ws + 2 * npad;
+    float4 *pg = reinterpret_cast<float4 *>(ws);
+    float *pu = ws + 4 * npad;
     a.gext = pu + npad;
     a.dh_part = a.gext + ext;
     a.pu = PadVol{pu, px, px * py, (unsigned)(4 * npad)};
-    a.pg = PadVol{reinterpret_cast<const float *>(pg), px, px * py, (unsigned)(8 * npad)};
+    a.pg = PadVol{reinterpret_cast<const float *>(pg), px, px * py, (unsigned)(16 * npad)};
     long long gb = (npad + 255) / 256;
     if (gb > 65536) gb = 65536;
     hipLaunchKernelGGL(bwd_pad, dim3((unsigned)gb), dim3(256), 0, s, u, g, y, Z, nx, ny, nz, pu, pg, px, py, pz, C::H,
