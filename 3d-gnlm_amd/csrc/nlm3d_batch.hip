@@ -43,7 +43,7 @@ struct BatchVol {
     float sigma;
     int variant;
     int ntx, nty;
-    // kVRows volumes: padded copy (filled by batch_pad inside the graph) and
+    // rows-variant volumes: padded copy (filled by batch_pad inside the graph) and
     // the Z accumulator (Zout, or plan workspace when the caller wants no Z)
     PadVol pv;
     float *Zacc;
@@ -72,12 +72,16 @@ template <int RP>
 __device__ __noinline__ void tile_slab(const Fwd3Tile &a, float *lds) { fwd3_slab_tile<RP>(a, lds, a.sigma); }
 template <class C>
 __device__ __noinline__ void tile_direct(const Fwd3Tile &a, float *lds) { fwd3_tile<C>(a, lds); }
-template <int RP, int RS>
-__device__ __noinline__ void tile_rows(const RowsTile &a, float *lds) { rows_tile<RowsCfg<RP, RS>>(a, lds); }
+// LIGHT instances are called from the three-waves-per-SIMD kernel only (a
+// separate function, so that the register limit of that kernel is theirs).
+template <int RP, int RS, bool SH, bool LIGHT>
+__device__ __noinline__ void tile_rows(const RowsTile &a, float *lds) { rows_tile<RowsCfg<RP, RS, SH>>(a, lds); }
 
-// kVRows tile: r_s is a template parameter of the tile body (its dx loop is
-// unrolled), so dispatch on it; the cases match nlm3d_rows.hip.
-template <int RP>
+// kVRows / kVRowsShort tile: r_s is a template parameter of the tile body (its
+// dx loop is unrolled), so dispatch on it; the cases match nlm3d_rows.hip.
+// LIGHT: only the bodies that fit three waves per SIMD (r_p = 1 rows, r_p = 2
+// short rows).
+template <int RP, bool LIGHT>
 __device__ __forceinline__ void run_rows_rp(const BatchVol &v, const BatchTile &t, float *lds)
 {
     RowsTile a;
@@ -89,31 +93,44 @@ __device__ __forceinline__ void run_rows_rp(const BatchVol &v, const BatchTile &
     a.nz = v.in.nz;
     a.k = v.k;
     a.x0 = t.tx * 16;
-    a.y0 = t.ty * fwd3_ty(kVRows, RP);
-    a.z0 = t.tz * fwd3_tz(kVRows, RP);
+    a.y0 = t.ty * fwd3_ty(v.variant, RP);
+    a.z0 = t.tz * fwd3_tz(v.variant, RP);
     if constexpr (RP == 1) {
         switch (v.rs) {
-        case 1: tile_rows<1, 1>(a, lds); break;
-        case 2: tile_rows<1, 2>(a, lds); break;
-        case 3: tile_rows<1, 3>(a, lds); break;
-        case 4: tile_rows<1, 4>(a, lds); break;
-        case 5: tile_rows<1, 5>(a, lds); break;
-        case 6: tile_rows<1, 6>(a, lds); break;
-        default: tile_rows<1, 7>(a, lds); break;
+        case 1: tile_rows<1, 1, false, LIGHT>(a, lds); break;
+        case 2: tile_rows<1, 2, false, LIGHT>(a, lds); break;
+        case 3: tile_rows<1, 3, false, LIGHT>(a, lds); break;
+        case 4: tile_rows<1, 4, false, LIGHT>(a, lds); break;
+        case 5: tile_rows<1, 5, false, LIGHT>(a, lds); break;
+        case 6: tile_rows<1, 6, false, LIGHT>(a, lds); break;
+        default: tile_rows<1, 7, false, LIGHT>(a, lds); break;
         }
     } else if constexpr (RP == 2) {
-        switch (v.rs) {
-        case 1: tile_rows<2, 1>(a, lds); break;
-        case 2: tile_rows<2, 2>(a, lds); break;
-        case 3: tile_rows<2, 3>(a, lds); break;
-        case 4: tile_rows<2, 4>(a, lds); break;
-        case 5: tile_rows<2, 5>(a, lds); break;
-        default: tile_rows<2, 6>(a, lds); break;
+        if (LIGHT || v.variant == kVRowsShort) {
+            switch (v.rs) {
+            case 1: tile_rows<2, 1, true, LIGHT>(a, lds); break;
+            case 2: tile_rows<2, 2, true, LIGHT>(a, lds); break;
+            case 3: tile_rows<2, 3, true, LIGHT>(a, lds); break;
+            case 4: tile_rows<2, 4, true, LIGHT>(a, lds); break;
+            case 5: tile_rows<2, 5, true, LIGHT>(a, lds); break;
+            default: tile_rows<2, 6, true, LIGHT>(a, lds); break;
+            }
+            return;
+        }
+        if constexpr (!LIGHT) {
+            switch (v.rs) {
+            case 1: tile_rows<2, 1, false, false>(a, lds); break;
+            case 2: tile_rows<2, 2, false, false>(a, lds); break;
+            case 3: tile_rows<2, 3, false, false>(a, lds); break;
+            case 4: tile_rows<2, 4, false, false>(a, lds); break;
+            case 5: tile_rows<2, 5, false, false>(a, lds); break;
+            default: tile_rows<2, 6, false, false>(a, lds); break;
+            }
         }
     }
 }
 
-// Padded copies of the kVRows volumes: blockIdx.y picks the volume.
+// Padded copies of the rows-variant volumes: blockIdx.y picks the volume.
 __global__ void batch_pad(const BatchVol *__restrict__ vols, const int *__restrict__ ids)
 {
     const BatchVol &v = vols[ids[blockIdx.y]];
@@ -144,8 +161,8 @@ __device__ __forceinline__ void run_tile_rp(const BatchVol &v, const BatchTile &
     a.rs = v.rs;
     a.k = v.k;
     a.sigma = v.sigma;
-    if (v.variant == kVRows) {
-        run_rows_rp<RP>(v, t, lds);
+    if (fwd3_is_rows(v.variant)) {
+        run_rows_rp<RP, false>(v, t, lds);
     } else if (v.variant == kVSlab2) {
         using C = SlabCfg<RP>;
         a.x0 = t.tx * C::TX;
@@ -171,9 +188,18 @@ __device__ __forceinline__ void run_tile_rp(const BatchVol &v, const BatchTile &
     }
 }
 
-__global__ __launch_bounds__(kFwdThreads) __attribute__((amdgpu_waves_per_eu(2))) void nlm3d_fwd_persistent(const BatchVol *__restrict__ vols,
-                                                                     const BatchTile *__restrict__ tiles,
-                                                                     const int *__restrict__ qstart, int *counters)
+// Volume v can run in the LIGHT kernel (three waves per SIMD).
+__host__ __device__ inline bool batch_light(int rp, int variant)
+{
+    return (rp == 1 && variant == kVRows) || (rp == 2 && variant == kVRowsShort);
+}
+
+// LIGHT: every volume of the plan satisfies batch_light; the kernel then holds
+// only those tile bodies and runs three waves per SIMD.
+template <bool LIGHT>
+__global__ __launch_bounds__(kFwdThreads) __attribute__((amdgpu_waves_per_eu(LIGHT ? 3 : 2))) void nlm3d_fwd_persistent(
+    const BatchVol *__restrict__ vols, const BatchTile *__restrict__ tiles, const int *__restrict__ qstart,
+    int *counters)
 {
     extern __shared__ __align__(16) float lds[];  // 128-bit LDS accesses in the tile bodies
     __shared__ int next;
@@ -199,11 +225,16 @@ __global__ __launch_bounds__(kFwdThreads) __attribute__((amdgpu_waves_per_eu(2))
         if (t < 0) break;
         const BatchTile tl = tiles[t];
         const BatchVol v = vols[tl.vol];
-        switch (v.rp) {
-        case 0: run_tile_rp<0>(v, tl, lds); break;
-        case 1: run_tile_rp<1>(v, tl, lds); break;
-        case 2: run_tile_rp<2>(v, tl, lds); break;
-        default: run_tile_rp<3>(v, tl, lds); break;
+        if constexpr (LIGHT) {
+            if (v.rp == 1) run_rows_rp<1, true>(v, tl, lds);
+            else run_rows_rp<2, true>(v, tl, lds);
+        } else {
+            switch (v.rp) {
+            case 0: run_tile_rp<0>(v, tl, lds); break;
+            case 1: run_tile_rp<1>(v, tl, lds); break;
+            case 2: run_tile_rp<2>(v, tl, lds); break;
+            default: run_tile_rp<3>(v, tl, lds); break;
+            }
         }
     }
 }
@@ -219,10 +250,11 @@ struct gnlm_batch_plan {
     BatchVol *d_vols = nullptr;
     BatchTile *d_tiles = nullptr;
     int *d_counter = nullptr;  // kNumXcd queue counters, then kNumXcd + 1 queue starts
-    int npad = 0;              // kVRows volumes
+    int npad = 0;              // kVRows / kVRowsShort volumes
     int *d_padids = nullptr;   // their indices
     long long padmax = 0;      // largest padded copy (floats)
     float *d_ws = nullptr;     // padded copies + Z accumulators
+    bool light = false;        // every volume fits the three-waves-per-SIMD kernel
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -259,9 +291,14 @@ hipError_t enqueue_batch(const gnlm_batch_plan *p, hipStream_t s)
                            (const BatchVol *)p->d_vols, (const int *)p->d_padids);
     }
     hipLaunchKernelGGL(reset_counters, dim3(1), dim3(64), 0, s, p->d_counter);
-    hipLaunchKernelGGL(nlm3d_fwd_persistent, dim3(p->nblocks), dim3(kFwdThreads), p->lds, s,
-                       (const BatchVol *)p->d_vols, (const BatchTile *)p->d_tiles,
-                       (const int *)(p->d_counter + kNumXcd), p->d_counter);
+    if (p->light)
+        hipLaunchKernelGGL(nlm3d_fwd_persistent<true>, dim3(p->nblocks), dim3(kFwdThreads), p->lds, s,
+                           (const BatchVol *)p->d_vols, (const BatchTile *)p->d_tiles,
+                           (const int *)(p->d_counter + kNumXcd), p->d_counter);
+    else
+        hipLaunchKernelGGL(nlm3d_fwd_persistent<false>, dim3(p->nblocks), dim3(kFwdThreads), p->lds, s,
+                           (const BatchVol *)p->d_vols, (const BatchTile *)p->d_tiles,
+                           (const int *)(p->d_counter + kNumXcd), p->d_counter);
     return hipGetLastError();
 }
 
@@ -305,7 +342,7 @@ int gnlm_batch_plan_create(const gnlm_batch_volume *vols, int n, gnlm_batch_plan
         b.Zacc = nullptr;
         b.pv = PadVol{nullptr, 0, 0, 0u};
         b.px = b.py = b.pz = b.lp = b.halo = 0;
-        if (b.variant == kVRows) {
+        if (fwd3_is_rows(b.variant)) {
             // geometry of the padded copy, as in nlm3d_rows.hip
             b.halo = v.r_p + v.r_s;
             b.lp = 4 * ((b.halo + 3) / 4);
@@ -354,13 +391,15 @@ int gnlm_batch_plan_create(const gnlm_batch_volume *vols, int n, gnlm_batch_plan
     p->nvol = n;
     p->ntiles = (int)sorted.size();
     p->lds = lds;
-    hipError_t e = hipFuncSetAttribute((const void *)nlm3d_fwd_persistent,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    p->light = true;
+    for (const BatchVol &b : hv) p->light = p->light && batch_light(b.rp, b.variant);
+    const void *kfn = p->light ? (const void *)nlm3d_fwd_persistent<true> : (const void *)nlm3d_fwd_persistent<false>;
+    hipError_t e = hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     int dev = 0, ncu = 0, occ = 0;
     if (e == hipSuccess) e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
     if (e == hipSuccess)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void *)nlm3d_fwd_persistent, kFwdThreads, lds);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kfn, kFwdThreads, lds);
     if (e == hipSuccess && occ < 1) e = hipErrorInvalidConfiguration;
     if (e == hipSuccess) p->nblocks = std::min(ncu * occ, p->ntiles);
     p->npad = (int)padids.size();
@@ -371,7 +410,7 @@ int gnlm_batch_plan_create(const gnlm_batch_volume *vols, int n, gnlm_batch_plan
         e = hipMemcpy(p->d_padids, padids.data(), sizeof(int) * padids.size(), hipMemcpyHostToDevice);
     for (int i = 0; i < n && e == hipSuccess; ++i) {
         BatchVol &b = hv[(size_t)i];
-        if (b.variant != kVRows) continue;
+        if (!fwd3_is_rows(b.variant)) continue;
         b.pv.p = p->d_ws + wsoff[(size_t)i];
         b.Zacc = b.Zout ? b.Zout : p->d_ws + zoff[(size_t)i];
     }

@@ -34,6 +34,7 @@
 // Nothing of size voxels x offsets is stored.  Every position of the
 // extended domain belongs to exactly one tile; dL/dh partials are summed per
 // lane, then per workgroup in a fixed order, so results are deterministic.
+#include <atomic>
 #include "gnlm_internal.h"
 #include "nlm3d_rows.h"
 #include "nlm3d_tile.h"
@@ -43,9 +44,10 @@ namespace {
 
 constexpr int NT = 256;
 
-template <int RS_>
+template <int RS_, bool SYM_>
 struct BwdRowsCfg {
     static constexpr int RP = 1, RS = RS_, W = 3, R = RP + RS;
+    static constexpr bool SYM = SYM_;
     static constexpr int H = 2 * R + RP;                  // halo of the padded copies (all axes)
     // gradient tile T, grown tile Q, phase-1 rows D
     static constexpr int QY = 16, QZ = 12;
@@ -64,13 +66,21 @@ struct BwdRowsCfg {
     static constexpr int SXR = 20, SXP = 400, SXN = SXP * DZ;
     static constexpr int SYP = 16 * QY + 16, SYN = SYP * DZ;
     static constexpr int EZP = 16 * QY, EZN = EZP * GZ;
-    static constexpr int LDS_FLOATS = SXN + 16 + SYN + EZN;  // + a spare s_x row
+    // SYM: scatter accumulator over T grown by RS in x and y and by RS towards
+    // +z (the visited half of the offsets has dz >= 0), [z][y][x]; the plane
+    // pitch is a multiple of 64 floats so that a thread's z pairs share one
+    // read2/write2 instruction
+    static constexpr int SCX = GX + 2 * RS, SCY = GY + 2 * RS, SCZ = GZ + RS;
+    static constexpr int SCP = (SCX * SCY + 63) / 64 * 64;
+    static constexpr int SCN = SYM ? SCP * SCZ : 0;
+    static constexpr int LDS_FLOATS = SXN + 16 + SYN + EZN + SCN;  // + a spare s_x row
 };
 
 struct BwdRowsArgs {
     PadVol pu;       // padded u~: ext position (x, y, z) at (z+H)*pxy + (y+H)*px + (x+H)
     PadVol pg;       // padded (G, GY, u~, 0), same geometry, float4 elements
-    float *gext;     // gradient on the extended domain, (n + 2R)^3
+    float *gext;     // gradient on the extended domain, (n + 2R)^3 (gather form)
+    float *gsc;      // symmetric form: each tile's scatter region (SCN floats per tile)
     float *dh_part;  // one partial per workgroup
     int nx, ny, nz;
     float k;         // w = exp2(S * k)
@@ -90,15 +100,16 @@ __device__ __forceinline__ float4 pad_load4(const PadRsrc &r, unsigned voff, int
     return make_float4(__uint_as_float(t[0]), __uint_as_float(t[1]), __uint_as_float(t[2]), __uint_as_float(t[3]));
 }
 
-template <int RS>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void nlm3d_bwd_rows(BwdRowsArgs a)
+template <int RS, bool SYM>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SYM ? 2 : 3))) void nlm3d_bwd_rows(BwdRowsArgs a)
 {
-    using C = BwdRowsCfg<RS>;
+    using C = BwdRowsCfg<RS, SYM>;
     constexpr int RP = C::RP, QZ = C::QZ, GZ = C::GZ;
     extern __shared__ __align__(16) float lds[];
     float *SX = lds;
     float *SXY = SX + C::SXN + 16;
     float *EZB = SXY + C::SYN;
+    float *SC = EZB + C::EZN;  // SYM only
     const int tid = threadIdx.x;
     const PadRsrc ru = pad_rsrc(a.pu);
     const PadRsrc rg = pad_rsrc(a.pg);
@@ -150,14 +161,22 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void nl
     const int ezw = yq * 16 + xq;   // boxed-E column base
     const int ezlo = (yq > 0 ? yq - 1 : 0) * 16 + xq;        // y neighbours (clamped in
     const int ezhi = (yq < C::QY - 1 ? yq + 1 : yq) * 16 + xq;  // the halo rows, unused)
+    // SYM visits only the offsets after d = 0 in (dz, dy, dx) order; d = 0
+    // itself contributes G(z) (w_0 = 1, u~(z) - u~(z) = 0)
     float gacc[GZ];
 #pragma unroll
-    for (int z = 0; z < GZ; ++z) gacc[z] = 0.0f;
+    for (int z = 0; z < GZ; ++z) gacc[z] = SYM ? Gc[z + RP] : 0.0f;
+    // sum of S E over the tile's positions: sum_d w S t1 = (1/2) sum_d S E / (-2c)
+    // summed over the whole extended domain (t2_d(x) = t1_-d(x+d))
     float dh = 0.0f;
+    // T-local scatter base: position (x_t, y_t, z_t = 0) of the accumulator
+    const int scb = (yq - RP + RS) * C::SCX + (xq - RP + RS);
+    if constexpr (SYM) {
+        for (int i = tid; i < C::SCN; i += NT) SC[i] = 0.0f;  // ordered by the loop's barriers
+    }
 
-    // offsets in (dz, dy, dx) order as one flat loop; sd(o) is the scalar
-    // offset of d relative to d = (-RS, -RS, -RS) (always >= 0)
     constexpr int NW = 2 * RS + 1, NOFF = NW * NW * NW;
+    constexpr int O0 = SYM ? (NOFF + 1) / 2 : 0;  // first offset visited
     // scalar offset of d relative to d = (-RS, -RS, -RS) (always >= 0), stepped
     // through the offsets in (dz, dy, dx) order
     auto sd_next = [&](int sd, int o) {  // offset index o -> o + 1
@@ -183,12 +202,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void nl
     float win[NWIN];
     // the column values at x + d, one offset ahead
     float ud[QZ], Gd[QZ], GYd[QZ];
-    load_col(0, ud, Gd, GYd);
+    int sd = SYM ? s0 + 1 : 0;  // d = (1, 0, 0) or (-RS, -RS, -RS)
+    int scd = 1;                // SYM: scatter offset of d, starting at d = (1, 0, 0)
+    load_col(sd, ud, Gd, GYd);
 
-    int sd = 0;
-    for (int o = 0; o < NOFF; ++o) {
+    for (int o = O0; o < NOFF; ++o) {
         const int sn = (o + 1 < NOFF) ? sd_next(sd, o) : sd;  // next offset (reloads the last one at the end)
-        if (o % NW == 0) {  // dx = -RS: a new (dy, dz)
+        if (o == O0 || o % NW == 0) {  // a new (dy, dz) (SYM starts at dx = 1: window[m] = sample at d + m)
 #pragma unroll
             for (int m = 0; m < NWIN; m += 2) {  // row starts are 8-byte aligned (even padded x)
                 const float2 t = pad_load2(ru, rbase, 4 * (sd + m));
@@ -224,7 +244,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void nl
         for (int j = 0; j < QZ + 2 * RP; ++j) v[j] = SXY[j * C::SYP + sxyr];
         float S[QZ];
         box_tree<3, QZ>(v, S);
-        float EX[QZ], du[GZ];
+        float EX[QZ], du[GZ], Gw[GZ];
 #pragma unroll
         for (int z = 0; z < QZ; ++z) {
             const float w = exp2_fast(S[z] * kw);
@@ -233,10 +253,17 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void nl
             const float E = (negc2 * w) * (t1 + t2);
             if (z >= RP && z < QZ - RP) {
                 gacc[z - RP] = fmaf(Gd[z], w, gacc[z - RP]);
-                dh = fmaf(w * S[z], t1, dh);
+                dh = fmaf(S[z], E, dh);
                 du[z - RP] = uc[z] - ud[z];
+                if constexpr (SYM) Gw[z - RP] = Gc[z] * w;
             }
-            EX[z] = dpp_box<1>(E);  // x box across the 16-lane row
+            // x box across the 16-lane row.  The empty asm makes E a value of
+            // its own: otherwise its product is fused into the box's first add
+            // (an fma whose addend is the shifted lane), and that add can no
+            // longer take the DPP operand itself -- one extra move per z.
+            float Eo = E;
+            asm("" : "+v"(Eo));
+            EX[z] = dpp_box<1>(Eo);
         }
         // the next offset's column values fly across phase 4 and the next phases 1-2
         load_col(sn, ud, Gd, GYd);
@@ -245,17 +272,56 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void nl
 #pragma unroll
         for (int z = 0; z < GZ; ++z) EZB[z * C::EZP + ezw] = EZ[z];
         lds_barrier();
-        // phase 4: y box of the boxed E, the 2 boxE (u~(z) - u~(z+d)) term
+        // phase 4: y box of the boxed E, the 2 boxE (u~(z) - u~(z+d)) term;
+        // SYM: the -d term of position x + d, G(x) w_d(x) + boxE_d(x) (u~(x+d) -
+        // u~(x)), added into the accumulator (one add per position and offset:
+        // the order of the adds is fixed)
+        if constexpr (SYM) {
+            float A[GZ];
 #pragma unroll
-        for (int z = 0; z < GZ; ++z) {
-            const float bE = (EZB[z * C::EZP + ezlo] + EZB[z * C::EZP + ezw]) + EZB[z * C::EZP + ezhi];
-            gacc[z] = fmaf(bE, du[z], gacc[z]);
+            for (int z = 0; z < GZ; ++z) {  // all reads first: the adds below may alias them
+                const float bE = (EZB[z * C::EZP + ezlo] + EZB[z * C::EZP + ezw]) + EZB[z * C::EZP + ezhi];
+                const float pz = bE * du[z];
+                gacc[z] = gacc[z] + pz;
+                A[z] = Gw[z] - pz;
+            }
+            // Plain read-add-write: within one offset the targets x + d of
+            // distinct positions are distinct, and the barriers between
+            // offsets order the updates (LDS float atomics cost ~12x as much).
+            if (inT) {
+                float *sc = SC + scb + scd;
+                float cur[GZ];
+#pragma unroll
+                for (int z = 0; z < GZ; ++z) cur[z] = sc[z * C::SCP];
+#pragma unroll
+                for (int z = 0; z < GZ; ++z) sc[z * C::SCP] = cur[z] + A[z];
+            }
+            // scatter offset of the next d: dz SCP + dy SCX + dx
+            const int ix = o % NW;
+            scd += ix < NW - 1 ? 1 : ((o / NW) % NW < NW - 1 ? C::SCX - (NW - 1) : C::SCP - (NW - 1) * C::SCX - (NW - 1));
+        } else {
+#pragma unroll
+            for (int z = 0; z < GZ; ++z) {
+                const float bE = (EZB[z * C::EZP + ezlo] + EZB[z * C::EZP + ezw]) + EZB[z * C::EZP + ezhi];
+                gacc[z] = fmaf(bE, du[z], gacc[z]);
+            }
         }
         sd = sn;
     }
 
-    // ---- gradient of the tile's extended-domain positions ----
-    if (inT) {
+    if constexpr (SYM) {
+        // the tile's own positions get their gathered terms, then the whole
+        // scatter region goes to the tile's slot; bwd_sym_combine sums the
+        // overlapping regions in a fixed order
+        __syncthreads();
+        if (inT) {
+#pragma unroll
+            for (int z = 0; z < GZ; ++z) SC[scb + z * C::SCP] += gacc[z];
+        }
+        __syncthreads();
+        float *dst = a.gsc + (size_t)t * C::SCN;
+        for (int i = tid; i < C::SCN; i += NT) dst[i] = SC[i];
+    } else if (inT) {  // ---- gradient of the tile's extended-domain positions ----
         const long long ex = a.nx + 2LL * C::R, ey = a.ny + 2LL * C::R, ez = a.nz + 2LL * C::R;
         const long long x = cx + C::R, y = cy + C::R;
         if (x < ex && y < ey) {
@@ -306,17 +372,61 @@ __global__ void bwd_pad(const float *u, const float *g, const float *y, const fl
     }
 }
 
+// Symmetric form: the gradient on the extended domain is the sum of the
+// scatter regions of the (up to 2 x 2 x 2) tiles that cover a position, added
+// in a fixed tile order.  Tile (tx, ty, tz) covers ext positions (xr, yr, zr)
+// (origin at -R) with lx = xr - tx GX + RS in [0, SCX), ly = yr - ty GY + RS
+// in [0, SCY), lz = zr - tz GZ in [0, SCZ), at lz SCP + ly SCX + lx of its slot.
 template <int RS>
-constexpr size_t bwd_rows_lds()
+__global__ void bwd_sym_combine(const float *__restrict__ gsc, float *__restrict__ gext, int ex, int ey, int ez,
+                                int ntx, int nty, int ntz)
 {
-    return BwdRowsCfg<RS>::LDS_FLOATS * sizeof(float);
+    using C = BwdRowsCfg<RS, true>;
+    const long long n = (long long)ex * ey * ez;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const int xr = (int)(i % ex);
+        const long long q = i / ex;
+        const int yr = (int)(q % ey), zr = (int)(q / ey);
+        const int tx1 = min((xr + RS) / C::GX, ntx - 1), ty1 = min((yr + RS) / C::GY, nty - 1),
+                  tz1 = min(zr / C::GZ, ntz - 1);
+        float acc = 0.0f;
+        for (int tz = max(tz1 - 2, 0); tz <= tz1; ++tz) {
+            const int lz = zr - tz * C::GZ;
+            if (lz >= C::SCZ) continue;
+            for (int ty = max(ty1 - 2, 0); ty <= ty1; ++ty) {
+                const int ly = yr - ty * C::GY + RS;
+                if (ly >= C::SCY) continue;
+                for (int tx = max(tx1 - 2, 0); tx <= tx1; ++tx) {
+                    const int lx = xr - tx * C::GX + RS;
+                    if (lx >= C::SCX) continue;
+                    const long long tile = tx + (long long)ntx * (ty + (long long)nty * tz);
+                    acc += gsc[tile * C::SCN + lz * C::SCP + ly * C::SCX + lx];
+                }
+            }
+        }
+        gext[i] = acc;
+    }
 }
 
+template <int RS, bool SYM>
+constexpr size_t bwd_rows_lds()
+{
+    return BwdRowsCfg<RS, SYM>::LDS_FLOATS * sizeof(float);
+}
+
+// The symmetric form where its LDS image lets two workgroups share a CU.
 template <int RS>
+constexpr bool bwd_rows_sym()
+{
+    return bwd_rows_lds<RS, true>() <= kLdsLimitBytes / 2;
+}
+
+template <int RS, bool SYM>
 hipError_t bwd_rows_t(const float *u, const float *y, const float *Z, const float *g, int nx, int ny, int nz,
                       float h, float *gu, float *gh, hipStream_t s)
 {
-    using C = BwdRowsCfg<RS>;
+    using C = BwdRowsCfg<RS, SYM>;
     BwdRowsArgs a;
     a.nx = nx;
     a.ny = ny;
@@ -338,13 +448,15 @@ hipError_t bwd_rows_t(const float *u, const float *y, const float *Z, const floa
     a.negc2 = (float)(-2.0 * c);
     float *ws = nullptr;
     // layout: (G, GY, u~, 0) copy | u~ copy | gradient on the extended domain | dL/dh partials
-    const size_t nfl = (size_t)(4 * npad + npad + ext + nblocks);
+    //         | scatter regions (SYM)
+    const size_t nfl = (size_t)(4 * npad + npad + ext + nblocks) + (size_t)nblocks * C::SCN;
     hipError_t e = hipMallocAsync((void **)&ws, sizeof(float) * nfl, s);
     if (e != hipSuccess) return e;
     float4 *pg = reinterpret_cast<float4 *>(ws);
     float *pu = ws + 4 * npad;
     a.gext = pu + npad;
     a.dh_part = a.gext + ext;
+    a.gsc = SYM ? a.dh_part + nblocks : nullptr;
     a.pu = PadVol{pu, px, px * py, (unsigned)(4 * npad)};
     a.pg = PadVol{reinterpret_cast<const float *>(pg), px, px * py, (unsigned)(16 * npad)};
     long long gb = (npad + 255) / 256;
@@ -353,29 +465,49 @@ hipError_t bwd_rows_t(const float *u, const float *y, const float *Z, const floa
                        C::R);
     e = hipGetLastError();
     if (e == hipSuccess) {
-        const size_t lds = bwd_rows_lds<RS>();
-        e = hipFuncSetAttribute((const void *)nlm3d_bwd_rows<RS>, hipFuncAttributeMaxDynamicSharedMemorySize,
+        const size_t lds = bwd_rows_lds<RS, SYM>();
+        e = hipFuncSetAttribute((const void *)nlm3d_bwd_rows<RS, SYM>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)lds);
         if (e == hipSuccess) {
-            hipLaunchKernelGGL(nlm3d_bwd_rows<RS>, dim3(nblocks), dim3(NT), lds, s, a);
+            hipLaunchKernelGGL((nlm3d_bwd_rows<RS, SYM>), dim3(nblocks), dim3(NT), lds, s, a);
             e = hipGetLastError();
         }
     }
+    if (SYM && e == hipSuccess) {
+        const int ex = nx + 2 * C::R, ey = ny + 2 * C::R, ez = nz + 2 * C::R;
+        long long cb = (ext + 255) / 256;
+        if (cb > 65536) cb = 65536;
+        hipLaunchKernelGGL(bwd_sym_combine<RS>, dim3((unsigned)cb), dim3(256), 0, s, a.gsc, a.gext, ex, ey, ez, a.ntx,
+                           a.nty, a.ntz);
+        e = hipGetLastError();
+    }
     if (e == hipSuccess) e = bwd3d_fold_launch(a.gext, gu, nx, ny, nz, C::R, s);
-    if (e == hipSuccess && gh) e = bwd3d_reduce_dh_launch(a.dh_part, nblocks, (float)(2.0 * c / (double)h), gh, s);
+    // the partials sum S E: dL/dh = (2c/h) sum_all-d w S t1 = -(1/(2h)) sum_all-d S E,
+    // and the symmetric form visits half of the offsets (the other half has the same sum)
+    if (e == hipSuccess && gh)
+        e = bwd3d_reduce_dh_launch(a.dh_part, nblocks, (float)((SYM ? -1.0 : -0.5) / (double)h), gh, s);
     const hipError_t e2 = hipFreeAsync(ws, s);
     return e != hipSuccess ? e : e2;
 }
 
 #define GNLM_BWD_ROWS_CASES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7)
 
+std::atomic<int> g_bwd_form{kBwdAuto};
+
 }  // namespace
+
+int bwd3d_set_form(int form)
+{
+    if (form < kBwdAuto || form > kBwdSymmetric) return -1;
+    g_bwd_form.store(form, std::memory_order_relaxed);
+    return 0;
+}
 
 bool bwd3d_rows_supported(int r_p, int r_s)
 {
     if (r_p != 1) return false;
 #define GNLM_BR_SUP(RS) \
-    if (r_s == RS) return bwd_rows_lds<RS>() <= kLdsLimitBytes;
+    if (r_s == RS) return bwd_rows_lds<RS, false>() <= kLdsLimitBytes && bwd_rows_lds<RS, true>() <= kLdsLimitBytes;
     GNLM_BWD_ROWS_CASES(GNLM_BR_SUP)
 #undef GNLM_BR_SUP
     return false;
@@ -385,8 +517,13 @@ hipError_t bwd3d_rows_launch(const float *u, const float *y, const float *Z, con
                              int r_p, int r_s, float h, float *gu, float *gh, hipStream_t s)
 {
     if (r_p != 1) return hipErrorInvalidValue;
-#define GNLM_BR_RUN(RS) \
-    if (r_s == RS) return bwd_rows_t<RS>(u, y, Z, g, nx, ny, nz, h, gu, gh, s);
+    const int form = g_bwd_form.load(std::memory_order_relaxed);
+#define GNLM_BR_RUN(RS)                                                                            \
+    if (r_s == RS) {                                                                               \
+        if (form == kBwdSymmetric || (form == kBwdAuto && bwd_rows_sym<RS>()))                     \
+            return bwd_rows_t<RS, true>(u, y, Z, g, nx, ny, nz, h, gu, gh, s);                     \
+        return bwd_rows_t<RS, false>(u, y, Z, g, nx, ny, nz, h, gu, gh, s);                        \
+    }
     GNLM_BWD_ROWS_CASES(GNLM_BR_RUN)
 #undef GNLM_BR_RUN
     return hipErrorInvalidValue;

@@ -104,7 +104,7 @@ hipError_t launch_cfg(const InDesc &in, float *out, float *Zout, int rs, float k
 template <int RP>
 hipError_t launch_rp(const InDesc &in, float *out, float *Zout, int rs, float k, int variant, hipStream_t s)
 {
-    if (variant == kVRows) return fwd3d_rows_launch(in, out, Zout, RP, rs, k, s);
+    if (fwd3_is_rows(variant)) return fwd3d_rows_launch(in, out, Zout, RP, rs, k, variant == kVRowsShort, s);
     if (variant == kVSlab) return launch_slab<RP, false>(in, out, Zout, rs, k, s);
     if (variant == kVSlab2) return launch_slab<RP, true>(in, out, Zout, rs, k, s);
     if (variant == kV16x16x8) return launch_cfg<Fwd3Cfg<RP, 16, 16, 8>>(in, out, Zout, rs, k, s);
@@ -114,7 +114,7 @@ hipError_t launch_rp(const InDesc &in, float *out, float *Zout, int rs, float k,
 template <int RP>
 size_t lds_rp(int rs, int variant)
 {
-    if (variant == kVRows) return fwd3d_rows_lds_bytes(RP, rs);
+    if (fwd3_is_rows(variant)) return fwd3d_rows_lds_bytes(RP, rs, variant == kVRowsShort);
     if (variant == kVSlab) return slab_lds_floats(RP, rs) * sizeof(float);
     if (variant == kVSlab2) return slab2_lds_floats(RP, rs) * sizeof(float);
     if (variant == kV16x16x8) return fwd3_lds_floats<Fwd3Cfg<RP, 16, 16, 8>>(rs) * sizeof(float);
@@ -137,7 +137,7 @@ size_t fwd3d_lds_bytes(int r_p, int r_s, int variant)
 bool fwd3d_variant_supported(int r_p, int r_s, int variant)
 {
     if (r_s < 0 || r_p < 0 || r_p > 3 || variant < 0 || variant >= kNumFwd3Variants) return false;
-    if (variant == kVRows) return fwd3d_rows_supported(r_p, r_s);
+    if (fwd3_is_rows(variant)) return fwd3d_rows_supported(r_p, r_s, variant == kVRowsShort);
     return fwd3d_lds_bytes(r_p, r_s, variant) <= kLdsLimitBytes;
 }
 

@@ -20,7 +20,7 @@
 //     (dy, dz) is loaded once (16 + 2 LP samples) and serves all 2 r_s + 1
 //     dx offsets from registers -- the dx shift is a compile-time register
 //     index inside a uniform switch.
-//   * phase 1: squared differences, x box sums (doubling tree) -> s_x rows
+//   * phase 1: squared differences, x box sums (shared pair sums) -> s_x rows
 //              (four 128-bit LDS stores per row);
 //     phase 2: (x vector, z_ext, run of y) items: y box sums -> s_xy;
 //     phase 3: (x, y, run of z) items: z box sums, w = exp2(S k), per-dz
@@ -34,7 +34,7 @@
 //     two floats per voxel per dz), which keeps the register file for the
 //     rows and partials.
 // Box sums add non-negative terms only (no running add/subtract windows), in
-// a pairwise tree instead of the CPU oracle's left-to-right chain, so the
+// shared pairwise trees instead of the CPU oracle's left-to-right chain, so the
 // results agree with the oracle to rounding (~1e-7 relative in S) rather
 // than bit for bit.
 #pragma once
@@ -58,16 +58,21 @@ struct PadVol {
     unsigned bytes;  // size of the copy (< 4 GiB: offsets are 32-bit)
 };
 
-template <int RP_, int RS_>
+// SHORT (r_p = 2 only): 16 x 12 x 12 tiles with one phase-1 row per thread
+// and 3-plane phase-3 runs -- a larger halo share, but about 100 fewer live
+// registers, so three waves per SIMD instead of two.
+template <int RP_, int RS_, bool SHORT_ = false>
 struct RowsCfg {
     static constexpr int RP = RP_, RS = RS_;
+    static constexpr bool SHORT = SHORT_;
+    static_assert(!SHORT || RP == 2, "short tiles are for r_p = 2");
     static constexpr int W = 2 * RP + 1;                 // patch width
     static constexpr int R = RP + RS;                    // halo of the copy in y, z
     static constexpr int LP = 4 * ((R + 3) / 4);         // halo of the copy in x (aligned)
     static constexpr int TX = 16;
     static constexpr int NS = TX + 2 * LP;               // samples per loaded row
     static constexpr int TY = RP == 1 ? 14 : 12;
-    static constexpr int TZ = RP == 1 ? 14 : 28;
+    static constexpr int TZ = RP == 1 ? 14 : (SHORT ? 12 : 28);
     static constexpr int EX = TX + 2 * RP, EY = TY + 2 * RP, EZ = TZ + 2 * RP;
     static constexpr int NROWS = EY * EZ;
     static constexpr int ROWS = NROWS / kRowsThreads;    // phase-1 rows per thread
@@ -80,7 +85,7 @@ struct RowsCfg {
     static constexpr int P2R = P2N / kRowsThreads;
     static_assert(TY % RY == 0 && P2N % kRowsThreads == 0, "whole phase-2 rounds");
     // phase 3 items: (x, y, run of RZ z)
-    static constexpr int RZ = 7;
+    static constexpr int RZ = SHORT ? 3 : 7;
     static constexpr int P3N = TX * TY * (TZ / RZ);
     static constexpr int P3R = (P3N + kRowsThreads - 1) / kRowsThreads;
     static_assert(TZ % RZ == 0, "whole z runs");
@@ -101,21 +106,31 @@ struct RowsCfg {
 // for them); address space 1 keeps them global.
 typedef __attribute__((address_space(1))) float gfloat;
 
-// s[i] = v[i] + v[i+1] + ... + v[i+W-1] for i < N, as a pairwise tree whose
-// pair sums are shared between neighbouring outputs (W = 3, 5, 7).
+// s[i] = v[i] + v[i+1] + ... + v[i+W-1] for i < N (W = 3, 5, 7), as trees of
+// pairwise sums (no running add/subtract windows).  Outputs
+// come in pairs (i, i+1), i even, that share their W-1 common terms m_i =
+// v[i+1] + ... + v[i+W-1]: s[i] = v[i] + m_i, s[i+1] = m_i + v[i+W].  m_i is
+// built from the pair sums p_j = v[j] + v[j+1] at odd j, which neighbouring
+// output pairs share again (W = 7: (p_{i+1} + p_{i+3}) + p_{i+5} for i = 0
+// mod 4, p_{i+1} + (p_{i+3} + p_{i+5}) for i = 2 mod 4, so the inner sums
+// are common too).  Cost per output: 1.5 adds (W = 3), 2 (W = 5), 2.25
+// (W = 7), against W - 1 for direct sums.
 template <int W, int N>
 __device__ __forceinline__ void box_tree(const float *v, float *s)
 {
     static_assert(W == 3 || W == 5 || W == 7, "patch widths 3, 5, 7");
-    constexpr int NP = N + W - 3;
+    constexpr int NP = N + W - 2;  // p_j for odd j < NP
     float p[NP];
 #pragma unroll
-    for (int i = 0; i < NP; ++i) p[i] = v[i] + v[i + 1];
+    for (int j = 1; j < NP; j += 2) p[j] = v[j] + v[j + 1];
 #pragma unroll
-    for (int i = 0; i < N; ++i) {
-        if constexpr (W == 3) s[i] = p[i] + v[i + 2];
-        else if constexpr (W == 5) s[i] = (p[i] + p[i + 2]) + v[i + 4];
-        else s[i] = ((p[i] + p[i + 2]) + p[i + 4]) + v[i + 6];
+    for (int i = 0; i < N; i += 2) {
+        float m;
+        if constexpr (W == 3) m = p[i + 1];
+        else if constexpr (W == 5) m = p[i + 1] + p[i + 3];
+        else m = (i % 4 == 0) ? (p[i + 1] + p[i + 3]) + p[i + 5] : p[i + 1] + (p[i + 3] + p[i + 5]);
+        s[i] = v[i] + m;
+        if (i + 1 < N) s[i + 1] = m + v[i + W];
     }
 }
 

@@ -17,11 +17,11 @@ struct RowsArgs {
     int ntx, nty, ntz;
 };
 
-template <int RP, int RS>
-__global__ __launch_bounds__(kRowsThreads) __attribute__((amdgpu_waves_per_eu(2))) void nlm3d_fwd_rows(RowsArgs a)
+template <int RP, int RS, bool SHORT>
+__global__ __launch_bounds__(kRowsThreads) __attribute__((amdgpu_waves_per_eu(RP == 1 || SHORT ? 3 : 2))) void nlm3d_fwd_rows(RowsArgs a)
 {
     extern __shared__ __align__(16) float lds[];
-    using C = RowsCfg<RP, RS>;
+    using C = RowsCfg<RP, RS, SHORT>;
     const int t = xcd_tile(blockIdx.x, a.ntx * a.nty * a.ntz);
     RowsTile tl;
     tl.v = a.v;
@@ -55,16 +55,16 @@ __global__ void rows_pad(InDesc in, float *dst, int px, int py, int pz, int lp, 
     }
 }
 
-template <int RP, int RS>
+template <int RP, int RS, bool SHORT>
 constexpr size_t rows_lds()
 {
-    return RowsCfg<RP, RS>::LDS_FLOATS * sizeof(float);
+    return RowsCfg<RP, RS, SHORT>::LDS_FLOATS * sizeof(float);
 }
 
-template <int RP, int RS>
+template <int RP, int RS, bool SHORT>
 hipError_t rows_launch_t(const InDesc &in, float *out, float *Zout, float k, hipStream_t s)
 {
-    using C = RowsCfg<RP, RS>;
+    using C = RowsCfg<RP, RS, SHORT>;
     RowsArgs a;
     a.nx = in.nx;
     a.ny = in.ny;
@@ -94,12 +94,12 @@ hipError_t rows_launch_t(const InDesc &in, float *out, float *Zout, float k, hip
     hipLaunchKernelGGL(rows_pad, dim3((unsigned)g), dim3(256), 0, s, in, ws, px, py, pz, C::LP, C::R);
     e = hipGetLastError();
     if (e == hipSuccess) {
-        const size_t lds = rows_lds<RP, RS>();
-        e = hipFuncSetAttribute((const void *)nlm3d_fwd_rows<RP, RS>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)lds);
+        const size_t lds = rows_lds<RP, RS, SHORT>();
+        e = hipFuncSetAttribute((const void *)nlm3d_fwd_rows<RP, RS, SHORT>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e == hipSuccess) {
-            hipLaunchKernelGGL((nlm3d_fwd_rows<RP, RS>), dim3(a.ntx * a.nty * a.ntz), dim3(kRowsThreads), lds, s,
-                               a);
+            hipLaunchKernelGGL((nlm3d_fwd_rows<RP, RS, SHORT>), dim3(a.ntx * a.nty * a.ntz), dim3(kRowsThreads), lds,
+                               s, a);
             e = hipGetLastError();
         }
     }
@@ -108,36 +108,39 @@ hipError_t rows_launch_t(const InDesc &in, float *out, float *Zout, float k, hip
 }
 
 // Instantiated radii: r_p in {1, 2}; r_s up to 7 (r_p = 1) or 6 (r_p = 2),
-// so that the loaded rows stay at <= 32 samples.
-#define GNLM_ROWS_CASES(X) \
-    X(1, 1) X(1, 2) X(1, 3) X(1, 4) X(1, 5) X(1, 6) X(1, 7) X(2, 1) X(2, 2) X(2, 3) X(2, 4) X(2, 5) X(2, 6)
+// so that the loaded rows stay at <= 32 samples; short tiles for r_p = 2.
+#define GNLM_ROWS_CASES(X)                                                                                       \
+    X(1, 1, false) X(1, 2, false) X(1, 3, false) X(1, 4, false) X(1, 5, false) X(1, 6, false) X(1, 7, false)     \
+    X(2, 1, false) X(2, 2, false) X(2, 3, false) X(2, 4, false) X(2, 5, false) X(2, 6, false) X(2, 1, true)      \
+    X(2, 2, true) X(2, 3, true) X(2, 4, true) X(2, 5, true) X(2, 6, true)
 
 }  // namespace
 
-bool fwd3d_rows_supported(int r_p, int r_s)
+bool fwd3d_rows_supported(int r_p, int r_s, bool short_tiles)
 {
-#define GNLM_ROWS_SUP(RP, RS) \
-    if (r_p == RP && r_s == RS) return rows_lds<RP, RS>() <= kLdsLimitBytes;
+#define GNLM_ROWS_SUP(RP, RS, SH) \
+    if (r_p == RP && r_s == RS && short_tiles == SH) return rows_lds<RP, RS, SH>() <= kLdsLimitBytes;
     GNLM_ROWS_CASES(GNLM_ROWS_SUP)
 #undef GNLM_ROWS_SUP
     return false;
 }
 
-size_t fwd3d_rows_lds_bytes(int r_p, int r_s)
+size_t fwd3d_rows_lds_bytes(int r_p, int r_s, bool short_tiles)
 {
-#define GNLM_ROWS_LDS(RP, RS) \
-    if (r_p == RP && r_s == RS) return rows_lds<RP, RS>();
+#define GNLM_ROWS_LDS(RP, RS, SH) \
+    if (r_p == RP && r_s == RS && short_tiles == SH) return rows_lds<RP, RS, SH>();
     GNLM_ROWS_CASES(GNLM_ROWS_LDS)
 #undef GNLM_ROWS_LDS
     return (size_t)-1;
 }
 
-hipError_t fwd3d_rows_launch(const InDesc &in, float *out, float *Zout, int r_p, int r_s, float k, hipStream_t s)
+hipError_t fwd3d_rows_launch(const InDesc &in, float *out, float *Zout, int r_p, int r_s, float k, bool short_tiles,
+                             hipStream_t s)
 {
     // a caller-padded input must carry the whole halo the copy needs
     if (in.pad > 0 && in.pad < r_p + r_s) return hipErrorInvalidValue;
-#define GNLM_ROWS_RUN(RP, RS) \
-    if (r_p == RP && r_s == RS) return rows_launch_t<RP, RS>(in, out, Zout, k, s);
+#define GNLM_ROWS_RUN(RP, RS, SH) \
+    if (r_p == RP && r_s == RS && short_tiles == SH) return rows_launch_t<RP, RS, SH>(in, out, Zout, k, s);
     GNLM_ROWS_CASES(GNLM_ROWS_RUN)
 #undef GNLM_ROWS_RUN
     return hipErrorInvalidValue;

@@ -78,8 +78,9 @@ def _extend(L):
     L.gnlm_reset_tuning.restype = None
     for name in ("gnlm_batch_plan_create", "gnlm_batch_run", "gnlm_batch_run_eager", "gnlm_batch_plan_info",
                  "gnlm_batch_plan_destroy", "gnlm_autotune", "gnlm_tuned_variant", "gnlm_set_variant",
-                 "gnlm_num_variants"):
+                 "gnlm_num_variants", "gnlm_set_backward_form"):
         getattr(L, name).restype = i
+    L.gnlm_set_backward_form.argtypes = [i]
 
 
 class BatchVolume(ctypes.Structure):
@@ -148,7 +149,14 @@ def reset_tuning():
     lib().gnlm_reset_tuning()
 
 
-VARIANT_16x16x16, VARIANT_16x16x8, VARIANT_SLAB, VARIANT_SLAB2, VARIANT_ROWS = 0, 1, 2, 3, 4
+VARIANT_16x16x16, VARIANT_16x16x8, VARIANT_SLAB, VARIANT_SLAB2, VARIANT_ROWS, VARIANT_ROWS_SHORT = 0, 1, 2, 3, 4, 5
+
+BWD_AUTO, BWD_GATHER, BWD_SYMMETRIC = 0, 1, 2
+
+
+def set_backward_form(form):
+    """Process-wide form of the r_p = 1 adjoint (gnlm_set_backward_form)."""
+    check(lib().gnlm_set_backward_form(int(form)), "gnlm_set_backward_form")
 
 
 def check(status, where=""):

@@ -90,10 +90,24 @@ int gnlm2d_forward_ex(const float *d_in, int in_pad, long long in_sy, float *d_o
  * writes d_gin = dL/d(input) (reflect-101 copies folded back onto their
  * source voxels) and, if d_gh is not NULL, one float dL/dh.  Patch distances
  * are recomputed; nothing of size voxels x offsets is stored.  Results are
- * deterministic.  Returns GNLM_EUNSUPPORTED for r_p > 2 or when the tile's
- * LDS image exceeds 160 KiB (r_p = 1: r_s <= 7; r_p = 2: r_s <= 5). */
+ * deterministic.  Scratch (padded copies, the gradient on the extended
+ * domain [-R, n+R)^3 with R = r_p + r_s) comes from a stream-ordered
+ * workspace released on `stream`.  Returns GNLM_EUNSUPPORTED for r_p > 2 or
+ * when the tile's LDS image exceeds 160 KiB (r_p = 1: r_s <= 7; r_p = 2:
+ * r_s <= 5). */
 int gnlm3d_backward(const float *d_in, const float *d_out, const float *d_Z, const float *d_gout, int nx,
                     int ny, int nz, int r_p, int r_s, float h, float *d_gin, float *d_gh, void *stream);
+
+/* Form of the r_p = 1 adjoint (no counterpart in the reference), process-
+ * wide: GNLM_BWD_GATHER visits every offset d and gathers its terms;
+ * GNLM_BWD_SYMMETRIC visits half of them and, since S_d(x) = S_-d(x + d),
+ * scatters each one's mirror term to x + d through an LDS accumulator (half
+ * the arithmetic, a larger LDS image); GNLM_BWD_AUTO (default) takes the
+ * symmetric form where two of its workgroups fit a CU (r_s <= 4).  Both give
+ * the gradient to FP32 rounding; each is deterministic.  GNLM_EINVAL for an
+ * unknown form. */
+enum { GNLM_BWD_AUTO = 0, GNLM_BWD_GATHER = 1, GNLM_BWD_SYMMETRIC = 2 };
+int gnlm_set_backward_form(int form);
 
 /* ---- batched persistent forward --------------------------------------
  * No counterpart in the reference (its launcher processes one compile-time
@@ -143,8 +157,14 @@ void gnlm_reset_tuning(void);                        /* forget all choices  */
  * identical to the generic path and the CPU oracle's FP32 order),
  * 2 = "slab" (16 x (16-2r_p) x (32-2r_p), DPP y sums, direct x/z sums,
  * pre-scaled input; agrees with the others to FP32 rounding), 3 = "slab2"
- * (variant 2 with 64/128-bit LDS accesses; bit-identical to 2).  Untuned
- * default: 3 where it fits, else 2, else the first direct variant. */
+ * (variant 2 with 64/128-bit LDS accesses; bit-identical to 2), 4 = "rows"
+ * (r_p 1 with r_s <= 7, r_p 2 with r_s <= 6: a padded aligned copy of the
+ * input in a stream-ordered workspace, per-row register reuse across dx,
+ * pairwise-tree box sums; agrees with the others to FP32 rounding),
+ * 5 = "rows, short tiles" (variant 4 on 16 x 12 x 12 tiles, r_p 2 only:
+ * fewer registers per thread, more workgroups per CU).
+ * Untuned default: 5, else 4 where they are instantiated, else 3 where it
+ * fits, else 2, else the first direct variant. */
 
 /* uint8 -> float32 element conversion (n elements), for 8-bit inputs such as
  * the reference's (3DNLM/main.cu:7620 copies unsigned char volumes). */

@@ -1,5 +1,6 @@
 """GPU adjoint: dL/d(input) and dL/dh.
 
+Both forms of the r_p = 1 adjoint (gather, symmetric) are run.
 Tolerances:
   * vs the oracle's FP64 chain-rule adjoint on the same input and dL/dy:
     max |gpu - ref| <= 1e-4 * max |ref| for dL/du, and relative 1e-4 for dL/dh.
@@ -27,18 +28,31 @@ def _gpu_grad(dev, u, g, rp, rs, h):
     return gu.cpu().numpy().astype(np.float64), float(gh.item())
 
 
+# the two forms of the r_p = 1 adjoint (gather over every offset; half of the
+# offsets with the mirror terms scattered), forced for the whole test
+@pytest.fixture(params=[gnlm.BWD_GATHER, gnlm.BWD_SYMMETRIC], ids=["gather", "symmetric"])
+def form(request):
+    gnlm.set_backward_form(request.param)
+    yield request.param
+    gnlm.set_backward_form(gnlm.BWD_AUTO)
+
+
 @pytest.mark.parametrize("shape,rp,rs,h", [
     ((7, 8, 9), 1, 2, 0.3),
     ((10, 12, 11), 1, 3, 0.25),
     ((6, 9, 8), 0, 2, 0.3),
     ((12, 12, 12), 1, 4, 0.2),
     ((3, 17, 18), 1, 2, 0.3),   # thin volume: reflect-101 folds many positions
+    ((26, 31, 29), 1, 4, 0.2),  # several tiles per axis: scatter regions overlap
+    ((14, 15, 13), 1, 5, 0.25),
+    ((9, 10, 11), 1, 7, 0.3),   # largest instantiated r_s
+    ((5, 4, 23), 1, 1, 0.3),
 ])
-def test_backward_matches_oracle(dev, shape, rp, rs, h):
+def test_backward_matches_oracle(dev, form, shape, rp, rs, h):
     rng = np.random.default_rng(sum(shape) * 7 + rs)
     u = rng.random(shape).astype(np.float32)
     g = rng.standard_normal(shape).astype(np.float32)
-    print(f"case bwd shape={shape} rp={rp} rs={rs}", flush=True)
+    print(f"case bwd form={form} shape={shape} rp={rp} rs={rs}", flush=True)
     gu, gh = _gpu_grad(dev, u, g, rp, rs, h)
     ref_gu, ref_gh = oracle.nlm3d_grad_f64(u.astype(np.float64), g.astype(np.float64), rp, rs, h)
     scale = np.abs(ref_gu).max()
@@ -46,13 +60,32 @@ def test_backward_matches_oracle(dev, shape, rp, rs, h):
     assert abs(gh - ref_gh) <= 1e-4 * max(abs(ref_gh), 1e-8), (gh, ref_gh)
 
 
-def test_backward_deterministic(dev):
+def test_backward_deterministic(dev, form):
     rng = np.random.default_rng(1)
     u = rng.random((20, 22, 24)).astype(np.float32)
     g = rng.standard_normal(u.shape).astype(np.float32)
     a = _gpu_grad(dev, u, g, 1, 3, 0.2)
     b = _gpu_grad(dev, u, g, 1, 3, 0.2)
     assert np.array_equal(a[0], b[0]) and a[1] == b[1]
+
+
+def test_backward_forms_agree(dev):
+    """Gather and symmetric forms on a 40^3 volume (r_s 4): the same gradient to
+    FP32 rounding (different summation orders)."""
+    rng = np.random.default_rng(5)
+    u = rng.random((40, 40, 40)).astype(np.float32)
+    g = rng.standard_normal(u.shape).astype(np.float32)
+    try:
+        gnlm.set_backward_form(gnlm.BWD_GATHER)
+        a = _gpu_grad(dev, u, g, 1, 4, 0.15)
+        gnlm.set_backward_form(gnlm.BWD_SYMMETRIC)
+        b = _gpu_grad(dev, u, g, 1, 4, 0.15)
+    finally:
+        gnlm.set_backward_form(gnlm.BWD_AUTO)
+    scale = np.abs(a[0]).max()
+    print("forms: max |diff| / max |g|", np.abs(a[0] - b[0]).max() / scale, "dh", a[1], b[1])
+    assert np.abs(a[0] - b[0]).max() <= 2e-5 * scale
+    assert abs(a[1] - b[1]) <= 2e-5 * abs(a[1])
 
 
 def _crop24():

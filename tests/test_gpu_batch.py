@@ -98,9 +98,9 @@ def test_autotune_variants(dev, rp, rs):
         direct = [outs[k] for k in (gnlm.VARIANT_16x16x16, gnlm.VARIANT_16x16x8) if k in outs]
         for o in direct[1:]:
             assert np.array_equal(o, direct[0])
-        for sv in (gnlm.VARIANT_SLAB, gnlm.VARIANT_SLAB2):
+        for sv in (gnlm.VARIANT_SLAB, gnlm.VARIANT_SLAB2, gnlm.VARIANT_ROWS, gnlm.VARIANT_ROWS_SHORT):
             if sv in outs:
-                assert max_rel(outs[sv], direct[0]) <= 1e-5
+                assert max_rel(outs[sv], direct[0]) <= 1e-5, sv
     finally:
         gnlm.reset_tuning()
 
@@ -110,10 +110,12 @@ def test_batch_slab_and_direct_variants(dev):
     the table selects."""
     vols = _volumes(dev, 6, 3)
     try:
-        for var in (gnlm.VARIANT_16x16x16, gnlm.VARIANT_SLAB, gnlm.VARIANT_SLAB2):
+        for var in (gnlm.VARIANT_16x16x16, gnlm.VARIANT_SLAB, gnlm.VARIANT_SLAB2, gnlm.VARIANT_ROWS,
+                    gnlm.VARIANT_ROWS_SHORT):
             for rp in (1, 2):
                 for rs in (3, 5):
-                    gnlm.set_variant(rp, rs, var)
+                    # short tiles exist for r_p = 2 only
+                    gnlm.set_variant(rp, rs, gnlm.VARIANT_ROWS if (var, rp) == (gnlm.VARIANT_ROWS_SHORT, 1) else var)
             outs = [torch.empty_like(u) for u, *_ in vols]
             plan = gnlm.BatchPlan([(u, o, rp, rs, h) for (u, rp, rs, h), o in zip(vols, outs)])
             plan.run()

@@ -136,18 +136,27 @@ ROWS_CASES = [
     ((5, 6, 7), 1, 3, 0.3),       # dims smaller than the halo
     ((1, 20, 20), 1, 2, 0.3),     # single plane
 ]
+# short 16 x 12 x 12 tiles (r_p = 2 only)
+ROWS_SHORT_CASES = [
+    ((28, 24, 24), 2, 5, 0.1),    # whole tiles
+    ((37, 29, 33), 2, 5, 0.2),    # partial tiles on every axis
+    ((17, 19, 23), 2, 1, 0.25),
+    ((12, 12, 12), 2, 6, 0.3),
+    ((9, 8, 7), 2, 3, 0.3),       # smaller than one tile
+]
 
 
-@pytest.mark.parametrize("shape,rp,rs,h", ROWS_CASES)
-def test_3d_rows_variant_matches_oracle(dev, shape, rp, rs, h):
-    """Padded-copy "rows" variant: pairwise-tree box sums, so it agrees with
+@pytest.mark.parametrize("shape,rp,rs,h,variant",
+                         [c + ("rows",) for c in ROWS_CASES] + [c + ("short",) for c in ROWS_SHORT_CASES])
+def test_3d_rows_variant_matches_oracle(dev, shape, rp, rs, h, variant):
+    """Padded-copy "rows" variants: pairwise-tree box sums, so they agree with
     the oracle's left-to-right order to rounding (<= 1e-5 relative, uniform
     random input, against the FP32 path and FP64)."""
     rng = np.random.default_rng(sum(shape) + 17 * rp + rs)
     u = rng.random(shape, dtype=np.float32)
-    print(f"case rows shape={shape} rp={rp} rs={rs}", flush=True)
+    print(f"case {variant} shape={shape} rp={rp} rs={rs}", flush=True)
     try:
-        gnlm.set_variant(rp, rs, gnlm.VARIANT_ROWS)
+        gnlm.set_variant(rp, rs, gnlm.VARIANT_ROWS if variant == "rows" else gnlm.VARIANT_ROWS_SHORT)
         y = _run3(dev, u, rp, rs, h, gnlm.PATH_TILED)
     finally:
         gnlm.reset_tuning()
