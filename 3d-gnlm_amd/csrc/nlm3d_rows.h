@@ -59,8 +59,9 @@ struct PadVol {
 };
 
 // SHORT (r_p = 2 only): 16 x 12 x 12 tiles with one phase-1 row per thread
-// and 3-plane phase-3 runs -- a larger halo share, but about 100 fewer live
-// registers, so three waves per SIMD instead of two.
+// and 3-plane phase-3 runs -- a larger halo share, but half the live
+// registers: the per-volume kernel runs four waves per SIMD (128 VGPRs; its
+// few spills sit outside the offset loops) instead of two.
 template <int RP_, int RS_, bool SHORT_ = false>
 struct RowsCfg {
     static constexpr int RP = RP_, RS = RS_;

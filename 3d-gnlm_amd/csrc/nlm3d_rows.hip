@@ -18,7 +18,7 @@ struct RowsArgs {
 };
 
 template <int RP, int RS, bool SHORT>
-__global__ __launch_bounds__(kRowsThreads) __attribute__((amdgpu_waves_per_eu(RP == 1 || SHORT ? 3 : 2))) void nlm3d_fwd_rows(RowsArgs a)
+__global__ __launch_bounds__(kRowsThreads) __attribute__((amdgpu_waves_per_eu(SHORT ? 4 : (RP == 1 ? 3 : 2)))) void nlm3d_fwd_rows(RowsArgs a)
 {
     extern __shared__ __align__(16) float lds[];
     using C = RowsCfg<RP, RS, SHORT>;
