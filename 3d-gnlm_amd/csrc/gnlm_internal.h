@@ -29,6 +29,10 @@ hipError_t fwd3d_tiled_launch(const InDesc &in, float *out, float *Zout, int r_p
 // ---- "rows" variant (nlm3d_rows.hip) ---------------------------------------
 // r_p in {1, 2} with r_s up to 7 / 6; pads the input into a stream-ordered
 // workspace first (caller-padded input needs pad >= r_p + r_s).
+// Large patches (r_p 4-8): 16 x (32 - 2 r_p)^2 tiles on 1024 threads.
+bool fwd3d_rows_big_supported(int r_p, int r_s);
+size_t fwd3d_rows_big_lds_bytes(int r_p, int r_s);
+hipError_t fwd3d_rows_big_launch(const InDesc &in, float *out, float *Zout, int r_p, int r_s, float k, hipStream_t s);
 // short_tiles selects the 16 x 12 x 12 tiles (r_p = 2 only).
 bool fwd3d_rows_supported(int r_p, int r_s, bool short_tiles);
 size_t fwd3d_rows_lds_bytes(int r_p, int r_s, bool short_tiles);
@@ -61,8 +65,8 @@ hipError_t bwd3d_rows_launch(const float *u, const float *y, const float *Z, con
 // shared tail kernels: fold the extended-domain gradient onto the volume
 // through the reflect-101 preimages; fixed-order sum of dL/dh partials x scale
 // Form of the r_p = 1 adjoint: gather over all offsets, or the symmetric
-// form (half of the offsets, each also scattered to x + d); auto picks the
-// symmetric form where two of its workgroups fit a CU's LDS.
+// form (half of the offsets, each also scattered to x + d); auto is the
+// symmetric form, the faster one at every instantiated r_s on MI355X.
 enum { kBwdAuto = 0, kBwdGather = 1, kBwdSymmetric = 2 };
 int bwd3d_set_form(int form);  // -1 for an unknown form
 hipError_t bwd3d_fold_launch(const float *gext, float *gu, int nx, int ny, int nz, int R, hipStream_t s);

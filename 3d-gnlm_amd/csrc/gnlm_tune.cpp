@@ -30,6 +30,7 @@ int first_supported(int r_p, int r_s)
 {
     if (fwd3d_variant_supported(r_p, r_s, kVRowsShort)) return kVRowsShort;
     if (fwd3d_variant_supported(r_p, r_s, kVRows)) return kVRows;
+    if (fwd3d_variant_supported(r_p, r_s, kVRowsBig)) return kVRowsBig;
     if (fwd3d_variant_supported(r_p, r_s, kVSlab2)) return kVSlab2;
     if (fwd3d_variant_supported(r_p, r_s, kVSlab)) return kVSlab;
     for (int v = 0; v < kNumFwd3Variants; ++v)

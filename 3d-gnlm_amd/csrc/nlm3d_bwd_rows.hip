@@ -415,13 +415,6 @@ constexpr size_t bwd_rows_lds()
     return BwdRowsCfg<RS, SYM>::LDS_FLOATS * sizeof(float);
 }
 
-// The symmetric form where its LDS image lets two workgroups share a CU.
-template <int RS>
-constexpr bool bwd_rows_sym()
-{
-    return bwd_rows_lds<RS, true>() <= kLdsLimitBytes / 2;
-}
-
 template <int RS, bool SYM>
 hipError_t bwd_rows_t(const float *u, const float *y, const float *Z, const float *g, int nx, int ny, int nz,
                       float h, float *gu, float *gh, hipStream_t s)
@@ -520,7 +513,7 @@ hipError_t bwd3d_rows_launch(const float *u, const float *y, const float *Z, con
     const int form = g_bwd_form.load(std::memory_order_relaxed);
 #define GNLM_BR_RUN(RS)                                                                            \
     if (r_s == RS) {                                                                               \
-        if (form == kBwdSymmetric || (form == kBwdAuto && bwd_rows_sym<RS>()))                     \
+        if (form != kBwdGather)                                                                    \
             return bwd_rows_t<RS, true>(u, y, Z, g, nx, ny, nz, h, gu, gh, s);                     \
         return bwd_rows_t<RS, false>(u, y, Z, g, nx, ny, nz, h, gu, gh, s);                        \
     }

@@ -67,6 +67,7 @@ struct RowsCfg {
     static constexpr int RP = RP_, RS = RS_;
     static constexpr bool SHORT = SHORT_;
     static_assert(!SHORT || RP == 2, "short tiles are for r_p = 2");
+    static constexpr int NT = kRowsThreads;              // threads per workgroup
     static constexpr int W = 2 * RP + 1;                 // patch width
     static constexpr int R = RP + RS;                    // halo of the copy in y, z
     static constexpr int LP = 4 * ((R + 3) / 4);         // halo of the copy in x (aligned)
@@ -100,6 +101,49 @@ struct RowsCfg {
     static constexpr int LDS_FLOATS = SXN + SYN;
 };
 
+// BIG (r_p 4-8: the large patches of the reference's default configuration,
+// NS = 17, and of its sweeps, NS = 13-17): 16 x (32 - 2 r_p) x (32 - 2 r_p)
+// tiles on 1024 threads, one phase-1 row of the 32 x 32 grown rows per thread
+// (at most 128 VGPRs: four waves per SIMD); phase-2 items are single columns
+// (x, z_ext, half of the y range).
+template <int RP_, int RS_>
+struct RowsBigCfg {
+    static constexpr int RP = RP_, RS = RS_;
+    static_assert(RP >= 4 && RP <= 8, "big tiles are for r_p 4-8");
+    static constexpr bool SHORT = false;
+    static constexpr int NT = 1024;
+    static constexpr int W = 2 * RP + 1;
+    static constexpr int R = RP + RS;
+    static constexpr int LP = 4 * ((R + 3) / 4);
+    static constexpr int TX = 16;
+    static constexpr int NS = TX + 2 * LP;
+    static constexpr int EY = 32, EZ = 32;
+    static constexpr int TY = EY - 2 * RP, TZ = EZ - 2 * RP;
+    static constexpr int EX = TX + 2 * RP;
+    static constexpr int NROWS = EY * EZ;
+    static constexpr int ROWS = NROWS / NT;
+    static_assert(ROWS == 1, "one phase-1 row per thread");
+    static constexpr int VX = 1;
+    static constexpr int RY = TY / 2;
+    static constexpr int NXV = TX / VX;
+    static constexpr int P2N = NXV * EZ * (TY / RY);
+    static constexpr int P2R = P2N / NT;
+    static_assert(TY % RY == 0 && P2N % NT == 0, "whole phase-2 rounds");
+    // z runs: (x, y, run) items fill the 1024 threads as far as TZ allows
+    static constexpr int RZ = RP == 4 ? 6 : (RP == 5 ? 11 : (RP == 6 ? 4 : (RP == 7 ? 6 : 4)));
+    static constexpr int P3N = TX * TY * (TZ / RZ);
+    static constexpr int P3R = (P3N + NT - 1) / NT;
+    static_assert(TZ % RZ == 0, "whole z runs");
+    // LDS: s_x [z_ext][y_ext][x] (row pitch 20, plane pitch 656 = 16 mod 64),
+    //      s_xy [z_ext][y][x] (plane pitch = 16 mod 64)
+    static constexpr int SXR = 20;
+    static constexpr int SXP = SXR * EY + 16;
+    static constexpr int SXN = SXP * EZ;
+    static constexpr int SYP = (16 * TY - 16 + 63) / 64 * 64 + 16;
+    static constexpr int SYN = SYP * EZ;
+    static constexpr int LDS_FLOATS = SXN + SYN;
+};
+
 // Global-address-space view of the output buffers.  When the tile body is
 // a called function (the persistent batched kernel) its pointers arrive
 // generic and memory operations through them become flat operations, which
@@ -116,10 +160,50 @@ typedef __attribute__((address_space(1))) float gfloat;
 // mod 4, p_{i+1} + (p_{i+3} + p_{i+5}) for i = 2 mod 4, so the inner sums
 // are common too).  Cost per output: 1.5 adds (W = 3), 2 (W = 5), 2.25
 // (W = 7), against W - 1 for direct sums.
+// Any odd W (9 ... 17): the same output pairs, with m_i built from blocks of
+// 2^b pair sums B_b[k] = B_{b-1}[k] + B_{b-1}[k + 2^b] (k odd), one block per
+// set bit of (W-1)/2; every level is shared by all output pairs, so a box
+// costs about 3 adds per output for W = 13 ... 17.
+template <int W, int N>
+__device__ __forceinline__ void box_tree_big(const float *v, float *s)
+{
+    static_assert(W % 2 == 1 && W >= 3 && W <= 31, "odd patch widths up to 31");
+    constexpr int M = (W - 1) / 2;  // pair sums in m_i
+    constexpr int NV = N + W - 1;   // inputs
+    float B[4][NV];                 // B[b][k]: sum of v[k .. k + 2^(b+1) - 1]
+#pragma unroll
+    for (int k = 1; k + 1 < NV; k += 2) B[0][k] = v[k] + v[k + 1];
+#pragma unroll
+    for (int b = 1; b < 4; ++b) {
+#pragma unroll
+        for (int k = 1; k + (2 << b) <= NV; k += 2) B[b][k] = B[b - 1][k] + B[b - 1][k + (1 << b)];
+    }
+#pragma unroll
+    for (int i = 0; i < N; i += 2) {
+        float m = 0.0f;
+        int off = i + 1;
+        bool first = true;
+#pragma unroll
+        for (int b = 3; b >= 0; --b) {
+            if ((M >> b) & 1) {
+                m = first ? B[b][off] : m + B[b][off];
+                first = false;
+                off += 2 << b;
+            }
+        }
+        s[i] = v[i] + m;
+        if (i + 1 < N) s[i + 1] = m + v[i + W];
+    }
+}
+
 template <int W, int N>
 __device__ __forceinline__ void box_tree(const float *v, float *s)
 {
-    static_assert(W == 3 || W == 5 || W == 7, "patch widths 3, 5, 7");
+    if constexpr (W > 7) {
+        box_tree_big<W, N>(v, s);
+        return;
+    }
+    static_assert(W == 3 || W == 5 || W == 7 || W > 7, "odd patch widths");
     constexpr int NP = N + W - 2;  // p_j for odd j < NP
     float p[NP];
 #pragma unroll
@@ -226,7 +310,9 @@ __device__ __forceinline__ void rows_phase2(const float *src, float *dst)
     float v[C::VX][NIN];
 #pragma unroll
     for (int j = 0; j < NIN; ++j) {
-        if constexpr (C::VX == 4) {
+        if constexpr (C::VX == 1) {
+            v[0][j] = src[j * C::SXR];
+        } else if constexpr (C::VX == 4) {
             const float4 t = *reinterpret_cast<const float4 *>(src + j * C::SXR);
             v[0][j] = t.x;
             v[1][j] = t.y;
@@ -243,7 +329,9 @@ __device__ __forceinline__ void rows_phase2(const float *src, float *dst)
     for (int c = 0; c < C::VX; ++c) box_tree<C::W, C::RY>(v[c], s[c]);
 #pragma unroll
     for (int o = 0; o < C::RY; ++o) {
-        if constexpr (C::VX == 4)
+        if constexpr (C::VX == 1)
+            dst[o * 16] = s[0][o];
+        else if constexpr (C::VX == 4)
             *reinterpret_cast<float4 *>(dst + o * 16) = make_float4(s[0][o], s[1][o], s[2][o], s[3][o]);
         else
             *reinterpret_cast<float2 *>(dst + o * 16) = make_float2(s[0][o], s[1][o]);
@@ -254,7 +342,7 @@ template <class C>
 __device__ __forceinline__ void rows_tile(const RowsTile &ta, float *lds)
 {
     const RowsTile a = ta;  // registers: no reloads through a generic pointer in the loops
-    constexpr int RP = C::RP, RS = C::RS, RZ = C::RZ, NT = kRowsThreads;
+    constexpr int RP = C::RP, RS = C::RS, RZ = C::RZ, NT = C::NT;
     float *SX = lds;
     float *SXY = lds + C::SXN;
     const int tid = threadIdx.x;

@@ -145,11 +145,17 @@ def set_variant(r_p, r_s, variant):
     check(lib().gnlm_set_variant(int(r_p), int(r_s), int(variant)), "gnlm_set_variant")
 
 
+def tuned_variant(r_p, r_s):
+    """The variant the tiled 3D forward uses for (r_p, r_s); -1 if none fits."""
+    return int(lib().gnlm_tuned_variant(int(r_p), int(r_s)))
+
+
 def reset_tuning():
     lib().gnlm_reset_tuning()
 
 
-VARIANT_16x16x16, VARIANT_16x16x8, VARIANT_SLAB, VARIANT_SLAB2, VARIANT_ROWS, VARIANT_ROWS_SHORT = 0, 1, 2, 3, 4, 5
+(VARIANT_16x16x16, VARIANT_16x16x8, VARIANT_SLAB, VARIANT_SLAB2, VARIANT_ROWS, VARIANT_ROWS_SHORT,
+ VARIANT_ROWS_BIG) = 0, 1, 2, 3, 4, 5, 6
 
 BWD_AUTO, BWD_GATHER, BWD_SYMMETRIC = 0, 1, 2
 

@@ -40,6 +40,12 @@ WORKLOADS = {
     "nlm3d_96x96x48_p3_s7": dict(dims=(96, 96, 48), n_ell=12, ax=(4.0, 20.0), sigma=0.05, rp=1, rs=3, h=0.05,
                                  seed=20261204),
     "nlm2d_2048_p5_s11": dict(dims2=(2048, 2048), sigma=0.08, rp=2, rs=5, h=0.08, seed=20261205),
+    # the reference's own 2D geometry ("2D NLM/11x11/main.cu": patch 11, search
+    # 21, uint8 input, h = PARAMETER = 10 on 0-255), the one with a published
+    # figure: 50.90 ms = 82.4 Mpixels/s at 2048^2 on an sm_86 GPU
+    # ("2D NLM/11x11/resource-11.txt:807")
+    "nlm2d_2048_p11_s21_u8": dict(dims2=(2048, 2048), sigma=0.08, rp=5, rs=10, h=10.0, seed=20261205, u8=True,
+                                  published=82.4),
 }
 
 
@@ -192,6 +198,54 @@ def bench_grad(args):
     }
 
 
+def bench_sweep(args):
+    """The reference's parameter sweep (3DNLM/run.sh:25-34): a 512 x 512 8-bit
+    image stacked `depth` times (3DNLM/nlm.cpp:53-59), SS in {11, 13}, NS in
+    {13, 15, 17}, depth in {16, 32, 64, 128}, h = PARAMETER = 10 on 0-255.
+    The image is synthetic (a 512^2 phantom quantised to 8 bits).  Each point
+    is timed with HIP events over `steps` forwards after `warmup`."""
+    import torch
+    import gnlm
+    dev = torch.device("cuda:0")
+    img, _ = gnlm.phantom2d(512, 512, sigma=0.05, seed=20261209)
+    img = np.clip(np.rint(img * 255.0), 0, 255).astype(np.float32)
+    steps, warmup = max(1, min(args.steps, 3)), max(0, min(args.warmup, 1))
+    points = []
+    t_all = time.perf_counter()
+    for depth in (16, 32, 64, 128):
+        vol = torch.from_numpy(np.ascontiguousarray(np.broadcast_to(img, (depth, 512, 512)))).to(dev)
+        out = torch.empty_like(vol)
+        for ss in (11, 13):
+            for ns in (13, 15, 17):
+                rs, rp = ss // 2, ns // 2
+                for _ in range(warmup):
+                    gnlm.nlm3d(vol, rp, rs, 10.0, out=out)
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                torch.cuda.synchronize()
+                e0.record()
+                for _ in range(steps):
+                    gnlm.nlm3d(vol, rp, rs, 10.0, out=out)
+                e1.record()
+                torch.cuda.synchronize()
+                ms = e0.elapsed_time(e1) / steps
+                nvox = 512 * 512 * depth
+                points.append({"SS": ss, "NS": ns, "depth": depth, "variant": gnlm.tuned_variant(rp, rs),
+                               "ms": round(ms, 3), "Mvoxels_s": round(nvox / ms / 1e3, 1),
+                               "Gpairs_s": round(nvox * ss ** 3 / ms / 1e6, 1)})
+                print(f"sweep SS={ss} NS={ns} depth={depth}: {ms:.2f} ms", file=sys.stderr, flush=True)
+    total = sum(p["ms"] for p in points)
+    return {
+        "metric": "Mvoxels/s 3D NLM 512^2 x depth, SS 11-13, NS 13-17 (run.sh sweep), all points",
+        "value": round(sum(512 * 512 * p["depth"] for p in points) / total / 1e3, 1),
+        "unit": "Mvoxels/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": round(total, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "nlm3d_sweep_512", "h": 10.0, "input": "512^2 8-bit phantom stacked along z"},
+        "points": points, "wall_s": round(time.perf_counter() - t_all, 2),
+    }
+
+
 def pmc_pass(args):
     """One launch of the workload's kernels, then a device copy of a known
     size (64 Mi floats = 256 MiB read, 256 MiB written) whose FETCH_SIZE and
@@ -233,7 +287,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="nlm3d_256_p5_s11",
-                    choices=sorted(WORKLOADS) + ["nlm3d_batch64", "nlm3d_grad_8x128"])
+                    choices=sorted(WORKLOADS) + ["nlm3d_batch64", "nlm3d_grad_8x128", "nlm3d_sweep_512"])
     ap.add_argument("--cpu-planes", type=int, default=256,
                     help="z-planes in the CPU baseline sample (clamped to the volume; 256 = all of the headline "
                          "volume, about 13 s on 16 cores)")
@@ -259,6 +313,9 @@ def main():
     if args.workload == "nlm3d_grad_8x128":
         print(json.dumps(bench_grad(args)))
         return
+    if args.workload == "nlm3d_sweep_512":
+        print(json.dumps(bench_sweep(args)))
+        return
     wl = WORKLOADS[args.workload]
     dev = torch.device("cuda:0")
     rp, rs, h = wl["rp"], wl["rs"], wl["h"]
@@ -279,6 +336,16 @@ def main():
         run = lambda t, o: gnlm.nlm2d(t, rp, rs, h)  # noqa: E731
         unit_vox = "Mpixels/s"
     nvox = nx * ny * nz
+    if wl.get("u8"):
+        # 8-bit input resident in HBM; each step converts it and filters it
+        u = np.clip(np.rint(u * 255.0), 0, 255).astype(np.float32)
+        u8 = torch.from_numpy(u.astype(np.uint8)).to(dev)
+        f32 = torch.empty(u.shape, dtype=torch.float32, device=dev)
+
+        def run(t, o):
+            gnlm.check(gnlm.lib().gnlm_u8_to_f32(u8.data_ptr(), f32.data_ptr(), f32.numel(),
+                                                 torch.cuda.current_stream(dev).cuda_stream), "gnlm_u8_to_f32")
+            return gnlm.nlm2d(f32, rp, rs, h)
     ud = torch.from_numpy(u).to(dev)
     out = torch.empty_like(ud)
     torch.cuda.synchronize()
@@ -338,7 +405,9 @@ def main():
                   "max_rel_floor_1e-3_range": float((d / np.maximum(np.abs(ref), floor)).max())}
 
     line = {
-        "metric": "Mvoxels/s 3D NLM 256^3 p5 s11 FP32" if nz > 1 else "Mpixels/s 2D NLM",
+        "metric": ((f"Mvoxels/s 3D NLM {nx}^3" if nx == ny == nz else f"Mvoxels/s 3D NLM {nx}x{ny}x{nz}") +
+                   f" p{2 * rp + 1} s{2 * rs + 1} FP32" if nz > 1 else
+                   (f"Mpixels/s 2D NLM {nx}^2 p{2 * rp + 1} s{2 * rs + 1}" + (" uint8" if wl.get("u8") else ""))),
         "value": round(value, 3),
         "unit": unit_vox,
         "n_gpus": 1,
@@ -347,7 +416,7 @@ def main():
         "ms_per_step": round(ms_step, 4),
         "higher_is_better": True,
         "scaling": "weak",
-        "vs_baseline": None,
+        "vs_baseline": round(value / wl["published"], 4) if wl.get("published") else None,
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": args.workload, "dims": [nx, ny, nz], "r_p": rp, "r_s": rs, "h": h,

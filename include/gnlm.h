@@ -102,8 +102,8 @@ int gnlm3d_backward(const float *d_in, const float *d_out, const float *d_Z, con
  * wide: GNLM_BWD_GATHER visits every offset d and gathers its terms;
  * GNLM_BWD_SYMMETRIC visits half of them and, since S_d(x) = S_-d(x + d),
  * scatters each one's mirror term to x + d through an LDS accumulator (half
- * the arithmetic, a larger LDS image); GNLM_BWD_AUTO (default) takes the
- * symmetric form where two of its workgroups fit a CU (r_s <= 4).  Both give
+ * the arithmetic, a larger LDS image); GNLM_BWD_AUTO (default) is the
+ * symmetric form, the faster one at every r_s (1.1-1.9x).  Both give
  * the gradient to FP32 rounding; each is deterministic.  GNLM_EINVAL for an
  * unknown form. */
 enum { GNLM_BWD_AUTO = 0, GNLM_BWD_GATHER = 1, GNLM_BWD_SYMMETRIC = 2 };
@@ -162,9 +162,12 @@ void gnlm_reset_tuning(void);                        /* forget all choices  */
  * input in a stream-ordered workspace, per-row register reuse across dx,
  * pairwise-tree box sums; agrees with the others to FP32 rounding),
  * 5 = "rows, short tiles" (variant 4 on 16 x 12 x 12 tiles, r_p 2 only:
- * fewer registers per thread, more workgroups per CU).
- * Untuned default: 5, else 4 where they are instantiated, else 3 where it
- * fits, else 2, else the first direct variant. */
+ * fewer registers per thread, more workgroups per CU), 6 = "rows, big tiles"
+ * (r_p 4-8 with r_s <= 7, the large patches of the reference's default
+ * NS = 17: 16 x (32 - 2 r_p)^2 tiles on 1024 threads; per-volume launches
+ * only, as batch plans take r_p <= 3).
+ * Untuned default: 5, else 4, else 6 where they are instantiated, else 3
+ * where it fits, else 2, else the first direct variant. */
 
 /* uint8 -> float32 element conversion (n elements), for 8-bit inputs such as
  * the reference's (3DNLM/main.cu:7620 copies unsigned char volumes). */

@@ -58,8 +58,9 @@ def test_invalid_arguments_rejected_without_gpu():
     assert L.gnlm3d_forward_ex(fake, 2, 0, 0, fake, None, 8, 8, 8, 1, 2, 0.1, 0, None) == gnlm.EINVAL
     # unknown path
     assert L.gnlm3d_forward_ex(fake, 0, 0, 0, fake, None, 8, 8, 8, 1, 2, 0.1, 7, None) == gnlm.EINVAL
-    # tiled path for radii it does not implement
-    assert L.gnlm3d_forward_ex(fake, 0, 0, 0, fake, None, 8, 8, 8, 8, 6, 0.1, 1, None) == gnlm.EUNSUPPORTED
+    # tiled path for radii it does not implement (patch radius 9, search radius 8 at patch radius 8)
+    assert L.gnlm3d_forward_ex(fake, 0, 0, 0, fake, None, 8, 8, 8, 9, 6, 0.1, 1, None) == gnlm.EUNSUPPORTED
+    assert L.gnlm3d_forward_ex(fake, 0, 0, 0, fake, None, 8, 8, 8, 8, 8, 0.1, 1, None) == gnlm.EUNSUPPORTED
     # even window sizes in the drop-in
     buf = (ctypes.c_ubyte * 16)()
     out = (ctypes.c_float * 16)()

@@ -167,6 +167,48 @@ def test_3d_rows_variant_matches_oracle(dev, shape, rp, rs, h, variant):
     assert max_rel(y, ref64) <= TOL, report(y, ref64)
 
 
+# large patches (variant 6): 16 x (32 - 2 r_p)^2 tiles on 1024 threads
+ROWS_BIG_CASES = [
+    ((24, 22, 21), 8, 6, 0.3),    # the reference's default NS = 17, SS = 13
+    ((40, 36, 33), 4, 3, 0.3),    # several tiles per axis, partial tiles
+    ((33, 35, 21), 6, 5, 0.3),    # NS = 13, SS = 11 (the reference's sweep)
+    ((17, 19, 23), 5, 7, 0.25),   # largest instantiated r_s
+    ((18, 16, 19), 7, 2, 0.3),
+    ((10, 9, 8), 8, 1, 0.3),      # smaller than one tile: reflection folds more than once
+]
+
+
+@pytest.mark.parametrize("shape,rp,rs,h", ROWS_BIG_CASES)
+def test_3d_rows_big_variant_matches_oracle(dev, shape, rp, rs, h):
+    """Big-tile rows variant (r_p 4-8): shared-pair trees for patch widths up
+    to 17; <= 1e-5 relative against the separable FP64 oracle and the CPU
+    FP32 path (brute force is too slow at 17^3 patches)."""
+    rng = np.random.default_rng(sum(shape) + 31 * rp + rs)
+    u = rng.random(shape, dtype=np.float32)
+    print(f"case rows-big shape={shape} rp={rp} rs={rs}", flush=True)
+    try:
+        gnlm.set_variant(rp, rs, gnlm.VARIANT_ROWS_BIG)
+        y = _run3(dev, u, rp, rs, h, gnlm.PATH_TILED)
+    finally:
+        gnlm.reset_tuning()
+    ref64 = oracle.nlm3d_separable_f64(u.astype(np.float64), rp, rs, h)
+    ref32 = oracle.nlm3d_separable_f32(u, rp, rs, h)
+    print("rows-big vs fp32", report(y, ref32), "vs fp64", report(y, ref64))
+    assert max_rel(y, ref64) <= TOL, report(y, ref64)
+    assert max_rel(y, ref32) <= TOL, report(y, ref32)
+
+
+def test_3d_large_patch_default_path_is_tiled(dev):
+    """The reference's default geometry (SS = 13, NS = 17) takes the tiled
+    path by default and agrees with the any-radius generic path."""
+    rng = np.random.default_rng(4)
+    u = rng.random((20, 21, 22), dtype=np.float32)
+    a = _run3(dev, u, 8, 6, 0.3)
+    b = _run3(dev, u, 8, 6, 0.3, gnlm.PATH_GENERIC)
+    assert gnlm.tuned_variant(8, 6) == gnlm.VARIANT_ROWS_BIG
+    assert max_rel(a, b) <= TOL, report(a, b)
+
+
 def test_3d_rows_padded_equals_reflect(dev):
     """A caller-padded input (the drop-in layout) gives the same bits as the
     reflect-on-read input through the rows variant."""
