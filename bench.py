@@ -39,6 +39,10 @@ WORKLOADS = {
                              seed=20261206),
     "nlm3d_96x96x48_p3_s7": dict(dims=(96, 96, 48), n_ell=12, ax=(4.0, 20.0), sigma=0.05, rp=1, rs=3, h=0.05,
                                  seed=20261204),
+    # the reference's default configuration (3DNLM/config.h:4-9: 512 x 512 x 128,
+    # SS 13, NS 17; 8-bit input, h = PARAMETER = 10 on 0-255, 3DNLM/main.cu:13)
+    "nlm3d_512x512x128_p17_s13_u8": dict(dims=(512, 512, 128), n_ell=64, ax=(5.0, 40.0), sigma=0.05, rp=8, rs=6,
+                                         h=10.0, seed=20261210, u8=True),
     "nlm2d_2048_p5_s11": dict(dims2=(2048, 2048), sigma=0.08, rp=2, rs=5, h=0.08, seed=20261205),
     # the reference's own 2D geometry ("2D NLM/11x11/main.cu": patch 11, search
     # 21, uint8 input, h = PARAMETER = 10 on 0-255), the one with a published
@@ -268,6 +272,8 @@ def pmc_pass(args):
             nx, ny, nz = wl["dims"]
             u, _ = gnlm.phantom3d(nx, ny, nz, n_ellipsoids=wl["n_ell"], axis_lo=wl["ax"][0], axis_hi=wl["ax"][1],
                                   background=0.05, sigma=wl["sigma"], seed=wl["seed"])
+            if wl.get("u8"):
+                u = np.clip(np.rint(u * 255.0), 0, 255).astype(np.float32)
             gnlm.nlm3d(torch.from_numpy(u).to(dev), wl["rp"], wl["rs"], wl["h"])
         else:
             u, _ = gnlm.phantom2d(*wl["dims2"], sigma=wl["sigma"], seed=wl["seed"])
@@ -288,9 +294,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="nlm3d_256_p5_s11",
                     choices=sorted(WORKLOADS) + ["nlm3d_batch64", "nlm3d_grad_8x128", "nlm3d_sweep_512"])
-    ap.add_argument("--cpu-planes", type=int, default=256,
-                    help="z-planes in the CPU baseline sample (clamped to the volume; 256 = all of the headline "
-                         "volume, about 13 s on 16 cores)")
+    ap.add_argument("--cpu-planes", type=int, default=0,
+                    help="z-planes in the CPU baseline sample (0: as many as about 15 s of CPU work on 16 cores "
+                         "allows -- all 256 planes of the headline volume)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--pmc-pass", action="store_true",
                     help="one launch of the workload plus a 256 MiB calibration copy (for rocprofv3 --pmc)")
@@ -345,7 +351,7 @@ def main():
         def run(t, o):
             gnlm.check(gnlm.lib().gnlm_u8_to_f32(u8.data_ptr(), f32.data_ptr(), f32.numel(),
                                                  torch.cuda.current_stream(dev).cuda_stream), "gnlm_u8_to_f32")
-            return gnlm.nlm2d(f32, rp, rs, h)
+            return gnlm.nlm3d(f32, rp, rs, h, out=o) if nz > 1 else gnlm.nlm2d(f32, rp, rs, h)
     ud = torch.from_numpy(u).to(dev)
     out = torch.empty_like(ud)
     torch.cuda.synchronize()
@@ -381,7 +387,10 @@ def main():
         yc = y.cpu().numpy()
         threads = _threads()
         if nz > 1:
-            nplanes = max(1, min(args.cpu_planes, nz))
+            # cost per plane ~ voxels x offsets x patch width (the separable path);
+            # 256 planes of the headline volume (256^2, s11, p5) take about 13 s
+            budget = 1.3e11 / (nx * ny * (2 * rs + 1) ** 3 * (2 * rp + 1))
+            nplanes = max(1, min(args.cpu_planes if args.cpu_planes > 0 else int(budget), nz))
             z0 = nz // 2 - nplanes // 2
             tc = time.perf_counter()
             ref = oracle.nlm3d_separable_f32(u, rp, rs, h, z0=z0, z1=z0 + nplanes, nthreads=threads)

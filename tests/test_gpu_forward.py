@@ -198,6 +198,25 @@ def test_3d_rows_big_variant_matches_oracle(dev, shape, rp, rs, h):
     assert max_rel(y, ref32) <= TOL, report(y, ref32)
 
 
+def test_3d_rows_big_padded_equals_reflect(dev):
+    """A caller-padded input (the drop-in layout) gives the same bits as the
+    reflect-on-read input through the big-tile variant."""
+    rng = np.random.default_rng(12)
+    u = rng.random((21, 26, 35), dtype=np.float32)
+    rp, rs, h = 6, 5, 0.3
+    R = rp + rs
+    up = np.pad(u, R, mode="reflect")
+    try:
+        gnlm.set_variant(rp, rs, gnlm.VARIANT_ROWS_BIG)
+        a = _run3(dev, u, rp, rs, h, gnlm.PATH_TILED)
+        t = torch.from_numpy(np.ascontiguousarray(up)).to(dev)
+        nz, ny, nx = u.shape
+        b = gnlm.nlm3d_padded(t, R, nx, ny, nz, rp, rs, h, gnlm.PATH_TILED).cpu().numpy()
+    finally:
+        gnlm.reset_tuning()
+    assert np.array_equal(a, b), report(b, a)
+
+
 def test_3d_large_patch_default_path_is_tiled(dev):
     """The reference's default geometry (SS = 13, NS = 17) takes the tiled
     path by default and agrees with the any-radius generic path."""
