@@ -86,9 +86,12 @@ struct RowsCfg {
     static constexpr int P2N = NXV * EZ * (TY / RY);
     static constexpr int P2R = P2N / kRowsThreads;
     static_assert(TY % RY == 0 && P2N % kRowsThreads == 0, "whole phase-2 rounds");
-    // phase 3 items: (x, y, run of RZ z)
+    // phase 3 items: (group of V3 x columns, y, run of RZ z); the short tiles
+    // read their columns four at a time (128-bit LDS reads: the phase's LDS
+    // time halves, and 192 items leave a fourth wave free)
     static constexpr int RZ = SHORT ? 3 : 7;
-    static constexpr int P3N = TX * TY * (TZ / RZ);
+    static constexpr int V3 = SHORT ? 4 : 1;
+    static constexpr int P3N = (TX / V3) * TY * (TZ / RZ);
     static constexpr int P3R = (P3N + kRowsThreads - 1) / kRowsThreads;
     static_assert(TZ % RZ == 0, "whole z runs");
     // LDS: s_x [z_ext][y_ext][x] (row pitch 20, plane pitch 20*16+16),
@@ -131,6 +134,7 @@ struct RowsBigCfg {
     static_assert(TY % RY == 0 && P2N % NT == 0, "whole phase-2 rounds");
     // z runs: (x, y, run) items fill the 1024 threads as far as TZ allows
     static constexpr int RZ = RP == 4 ? 6 : (RP == 5 ? 11 : (RP == 6 ? 4 : (RP == 7 ? 6 : 4)));
+    static constexpr int V3 = 1;
     static constexpr int P3N = TX * TY * (TZ / RZ);
     static constexpr int P3R = (P3N + NT - 1) / NT;
     static_assert(TZ % RZ == 0, "whole z runs");
@@ -392,7 +396,8 @@ __device__ __forceinline__ void rows_tile(const RowsTile &ta, float *lds)
         p2dst[m] = zp2 * C::SYP + yr2 * C::RY * 16 + xv * C::VX;
     }
 
-    // ---- phase-3 items ----
+    // ---- phase-3 items: (x group of V3 columns, y, z run) ----
+    constexpr int V3 = C::V3, NXG = 16 / V3;
     int p3r[C::P3R];
     unsigned cb[C::P3R];
     bool p3ok[C::P3R];
@@ -400,7 +405,8 @@ __device__ __forceinline__ void rows_tile(const RowsTile &ta, float *lds)
     for (int k = 0; k < C::P3R; ++k) {
         const int it = tid + k * NT;
         const int itc = ((C::P3N % NT == 0) || it < C::P3N) ? it : C::P3N - 1;
-        const int x = itc % 16, y = (itc / 16) % C::TY, zr = itc / (16 * C::TY);
+        const int x = (itc % NXG) * V3, y = (itc / NXG) % C::TY, zr = itc / (NXG * C::TY);
+        // items past P3N (whole waves for the short tiles) skip the phase
         p3ok[k] = __builtin_amdgcn_readfirstlane(((C::P3N % NT == 0) || it < C::P3N) && zr * RZ < tzv);
         p3r[k] = zr * RZ * C::SYP + y * 16 + x;
         // candidate u~(x + d) at d = (-RS, -RS, -RS)
@@ -409,14 +415,16 @@ __device__ __forceinline__ void rows_tile(const RowsTile &ta, float *lds)
     }
 
     for (int dz = -RS; dz <= RS; ++dz) {
-        float Zd[C::P3R][RZ], Yd[C::P3R][RZ];
+        float Zd[C::P3R][V3][RZ], Yd[C::P3R][V3][RZ];
 #pragma unroll
         for (int k = 0; k < C::P3R; ++k)
 #pragma unroll
-            for (int o = 0; o < RZ; ++o) {
-                Zd[k][o] = 0.0f;
-                Yd[k][o] = 0.0f;
-            }
+            for (int c = 0; c < V3; ++c)
+#pragma unroll
+                for (int o = 0; o < RZ; ++o) {
+                    Zd[k][c][o] = 0.0f;
+                    Yd[k][c][o] = 0.0f;
+                }
         for (int dy = -RS; dy <= RS; ++dy) {
             // scalar offsets are unsigned in the buffer address: count from d = -RS
             const int soffp = (dz + RS) * pxy + (dy + RS) * px;
@@ -429,11 +437,24 @@ __device__ __forceinline__ void rows_tile(const RowsTile &ta, float *lds)
                 constexpr int DX = decltype(dxc)::value;
                 rows_phase1<C, C::LP - RP + DX>(cen, sh, SX, sxw, p1ok);
                 // candidate samples u~(x + d), in flight across the barriers
-                float cand[C::P3R][RZ];
+                float cand[C::P3R][V3][RZ];
 #pragma unroll
                 for (int o = 0; o < RZ; ++o)
 #pragma unroll
-                    for (int k = 0; k < C::P3R; ++k) cand[k][o] = pad_load(rs, cb[k], 4 * (soffp + DX + RS + o * pxy));
+                    for (int k = 0; k < C::P3R; ++k) {
+                        if constexpr (V3 == 4) {
+                            // the group's four samples in one load (dword-aligned: the
+                            // shift dx moves the group by single samples)
+                            const auto t = __builtin_amdgcn_raw_buffer_load_b128(
+                                rs.r, (int)cb[k], 4 * (soffp + DX + RS + o * pxy), 0);
+#pragma unroll
+                            for (int c = 0; c < 4; ++c) cand[k][c][o] = __uint_as_float(t[c]);
+                        } else {
+#pragma unroll
+                            for (int c = 0; c < V3; ++c)
+                                cand[k][c][o] = pad_load(rs, cb[k] + 4u * c, 4 * (soffp + DX + RS + o * pxy));
+                        }
+                    }
                 lds_barrier();
 #pragma unroll
                 for (int m = 0; m < C::P2R; ++m)
@@ -442,16 +463,34 @@ __device__ __forceinline__ void rows_tile(const RowsTile &ta, float *lds)
 #pragma unroll
                 for (int k = 0; k < C::P3R; ++k) {
                     if (p3ok[k]) {
-                        float v[RZ + 2 * RP];
+                        float v[V3][RZ + 2 * RP];
 #pragma unroll
-                        for (int j = 0; j < RZ + 2 * RP; ++j) v[j] = SXY[p3r[k] + j * C::SYP];
-                        float S[RZ];
-                        box_tree<C::W, RZ>(v, S);
+                        for (int j = 0; j < RZ + 2 * RP; ++j) {
+                            const float *src = SXY + p3r[k] + j * C::SYP;
+                            if constexpr (V3 == 4) {
+                                const float4 t = *reinterpret_cast<const float4 *>(src);
+                                v[0][j] = t.x;
+                                v[1][j] = t.y;
+                                v[2][j] = t.z;
+                                v[3][j] = t.w;
+                            } else if constexpr (V3 == 2) {
+                                const float2 t = *reinterpret_cast<const float2 *>(src);
+                                v[0][j] = t.x;
+                                v[1][j] = t.y;
+                            } else {
+                                v[0][j] = src[0];
+                            }
+                        }
 #pragma unroll
-                        for (int o = 0; o < RZ; ++o) {
-                            const float w = exp2_fast(S[o] * kw);
-                            Zd[k][o] = Zd[k][o] + w;
-                            Yd[k][o] = fmaf(w, cand[k][o], Yd[k][o]);
+                        for (int c = 0; c < V3; ++c) {
+                            float S[RZ];
+                            box_tree<C::W, RZ>(v[c], S);
+#pragma unroll
+                            for (int o = 0; o < RZ; ++o) {
+                                const float w = exp2_fast(S[o] * kw);
+                                Zd[k][c][o] = Zd[k][c][o] + w;
+                                Yd[k][c][o] = fmaf(w, cand[k][c][o], Yd[k][c][o]);
+                            }
                         }
                     }
                 }
@@ -461,20 +500,25 @@ __device__ __forceinline__ void rows_tile(const RowsTile &ta, float *lds)
 #pragma unroll
         for (int k = 0; k < C::P3R; ++k) {
             const int it = tid + k * NT;
-            const int x = it % 16, y = (it / 16) % C::TY, zr = it / (16 * C::TY);
-            const int gx = a.x0 + x, gy = a.y0 + y, gz = a.z0 + zr * RZ;
-            if (!p3ok[k] || it >= C::P3N || gx >= a.nx || gy >= a.ny) continue;
+            const int x = (it % NXG) * V3, y = (it / NXG) % C::TY, zr = it / (NXG * C::TY);
+            const int gy = a.y0 + y, gz = a.z0 + zr * RZ;
+            if (!p3ok[k] || it >= C::P3N || gy >= a.ny) continue;
 #pragma unroll
-            for (int o = 0; o < RZ; ++o) {
-                if (gz + o >= a.nz) continue;
-                const int oi = gx + a.nx * (gy + a.ny * (gz + o));
-                float Z = Zd[k][o], Y = Yd[k][o];
-                if (dz != -RS) {
-                    Z = Zacc[oi] + Z;
-                    Y = out[oi] + Y;
+            for (int c = 0; c < V3; ++c) {
+                const int gx = a.x0 + x + c;
+                if (gx >= a.nx) continue;
+#pragma unroll
+                for (int o = 0; o < RZ; ++o) {
+                    if (gz + o >= a.nz) continue;
+                    const int oi = gx + a.nx * (gy + a.ny * (gz + o));
+                    float Z = Zd[k][c][o], Y = Yd[k][c][o];
+                    if (dz != -RS) {
+                        Z = Zacc[oi] + Z;
+                        Y = out[oi] + Y;
+                    }
+                    Zacc[oi] = Z;
+                    out[oi] = (dz == RS) ? Y / Z : Y;
                 }
-                Zacc[oi] = Z;
-                out[oi] = (dz == RS) ? Y / Z : Y;
             }
         }
     }
