@@ -244,6 +244,19 @@ __device__ __forceinline__ PadRsrc pad_rsrc(const PadVol &v)
     return PadRsrc{__builtin_amdgcn_make_buffer_rsrc(p, (short)0, (int)nb, 0x00020000)};
 }
 
+// A resource over any buffer of `bytes` bytes (uniform base and size).
+__device__ __forceinline__ PadRsrc buf_rsrc(const void *base, unsigned bytes)
+{
+    const unsigned long long b = (unsigned long long)base;
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)b);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(b >> 32));
+    void *p = (void *)(((unsigned long long)hi << 32) | lo);
+    return PadRsrc{__builtin_amdgcn_make_buffer_rsrc(p, (short)0, (int)__builtin_amdgcn_readfirstlane(bytes),
+                                                     0x00020000)};
+}
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
 __device__ __forceinline__ float pad_load(const PadRsrc &r, unsigned voff, int soff)
 {
     return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r.r, (int)voff, soff, 0));
@@ -414,6 +427,13 @@ __device__ __forceinline__ void rows_tile(const RowsTile &ta, float *lds)
                                 RS);
     }
 
+    // V3 == 4: the flush of whole groups uses 128-bit buffer accesses
+    const long long nvox = (long long)a.nx * a.ny * a.nz;
+    const bool vec_ok = V3 == 4 && (((unsigned long long)a.out | (unsigned long long)a.Zacc) & 15) == 0 &&
+                        (a.nx & 3) == 0 && nvox < (1LL << 30);
+    const unsigned vbytes = vec_ok ? (unsigned)(4 * nvox) : 0u;
+    const PadRsrc ro = buf_rsrc(a.out, vbytes), rz = buf_rsrc(a.Zacc, vbytes);
+
     for (int dz = -RS; dz <= RS; ++dz) {
         float Zd[C::P3R][V3][RZ], Yd[C::P3R][V3][RZ];
 #pragma unroll
@@ -503,6 +523,42 @@ __device__ __forceinline__ void rows_tile(const RowsTile &ta, float *lds)
             const int x = (it % NXG) * V3, y = (it / NXG) % C::TY, zr = it / (NXG * C::TY);
             const int gy = a.y0 + y, gz = a.z0 + zr * RZ;
             if (!p3ok[k] || it >= C::P3N || gy >= a.ny) continue;
+            if constexpr (V3 == 4) {
+                // a whole group at a 16-byte aligned position: one 128-bit
+                // access per buffer and plane (vec_ok: aligned buffers, rows of
+                // a multiple of 4 samples, byte offsets within 32 bits)
+                if (vec_ok && a.x0 + x + 3 < a.nx) {
+#pragma unroll
+                    for (int o = 0; o < RZ; ++o) {
+                        if (gz + o >= a.nz) continue;
+                        const int boff = 4 * (a.x0 + x + a.nx * (gy + a.ny * (gz + o)));
+                        float Z[4], Y[4];
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) {
+                            Z[c] = Zd[k][c][o];
+                            Y[c] = Yd[k][c][o];
+                        }
+                        if (dz != -RS) {
+                            const u32x4 zt = __builtin_amdgcn_raw_buffer_load_b128(rz.r, boff, 0, 0);
+                            const u32x4 yt = __builtin_amdgcn_raw_buffer_load_b128(ro.r, boff, 0, 0);
+#pragma unroll
+                            for (int c = 0; c < 4; ++c) {
+                                Z[c] = __uint_as_float(zt[c]) + Z[c];
+                                Y[c] = __uint_as_float(yt[c]) + Y[c];
+                            }
+                        }
+                        u32x4 zs, ys;
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) {
+                            zs[c] = __float_as_uint(Z[c]);
+                            ys[c] = __float_as_uint((dz == RS) ? Y[c] / Z[c] : Y[c]);
+                        }
+                        __builtin_amdgcn_raw_buffer_store_b128(zs, rz.r, boff, 0, 0);
+                        __builtin_amdgcn_raw_buffer_store_b128(ys, ro.r, boff, 0, 0);
+                    }
+                    continue;
+                }
+            }
 #pragma unroll
             for (int c = 0; c < V3; ++c) {
                 const int gx = a.x0 + x + c;

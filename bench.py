@@ -387,9 +387,10 @@ def main():
         yc = y.cpu().numpy()
         threads = _threads()
         if nz > 1:
-            # cost per plane ~ voxels x offsets x patch width (the separable path);
-            # 256 planes of the headline volume (256^2, s11, p5) take about 13 s
-            budget = 1.3e11 / (nx * ny * (2 * rs + 1) ** 3 * (2 * rp + 1))
+            # cost per plane ~ voxels x offsets x patch width^2.5 (measured on the
+            # separable path: 256 planes of the headline volume, 256^2 s11 p5,
+            # take about 13 s; 13 planes of 512^2 s13 p17 took 109 s)
+            budget = 1.44e12 / (nx * ny * (2 * rs + 1) ** 3 * (2 * rp + 1) ** 2.5)
             nplanes = max(1, min(args.cpu_planes if args.cpu_planes > 0 else int(budget), nz))
             z0 = nz // 2 - nplanes // 2
             tc = time.perf_counter()
