@@ -44,9 +44,10 @@ namespace {
 
 constexpr int NT = 256;
 
-template <int RS_, bool SYM_>
+template <int RP_, int RS_, bool SYM_>
 struct BwdRowsCfg {
-    static constexpr int RP = 1, RS = RS_, W = 3, R = RP + RS;
+    static constexpr int RP = RP_, RS = RS_, W = 2 * RP + 1, R = RP + RS;
+    static_assert(RP == 1 || RP == 2, "patch radius 1 or 2");
     static constexpr bool SYM = SYM_;
     static constexpr int H = 2 * R + RP;                  // halo of the padded copies (all axes)
     // gradient tile T, grown tile Q, phase-1 rows D
@@ -57,13 +58,17 @@ struct BwdRowsCfg {
     static constexpr int TX = 16;                         // x outputs per phase-1 row (Q's width)
     static constexpr int EX = TX + 2 * RP;                // centre samples per row
     static constexpr int NROWS = DY * DZ;
-    static_assert(NROWS <= NT, "one phase-1 row per thread");
+    // one phase-1 row per thread; rows beyond NT (r_p = 2: 320 rows) are cut
+    // into x quarters (4 outputs), one quarter per thread
+    static constexpr int NR1 = NROWS < NT ? NROWS : NT;
+    static constexpr int NRX = NROWS - NR1;
+    static_assert(NRX == 0 || NRX * 4 == NT, "the extra rows fill one quarter-row round");
     static constexpr int VX = 2, RY = 8, NXV = TX / VX;
     static constexpr int P2N = NXV * DZ * (QY / RY);
     static_assert(P2N <= NT, "one phase-2 item per thread");
     // LDS: s_x [z_d][y_d][x] (row pitch 20, plane pitch 400 == 16 mod 64),
     //      s_xy [z_d][y_q][x] (plane pitch 272), boxed E [z_t][y_q][x]
-    static constexpr int SXR = 20, SXP = 400, SXN = SXP * DZ;
+    static constexpr int SXR = 20, SXP = (SXR * DY - 16 + 63) / 64 * 64 + 16, SXN = SXP * DZ;
     static constexpr int SYP = 16 * QY + 16, SYN = SYP * DZ;
     static constexpr int EZP = 16 * QY, EZN = EZP * GZ;
     // SYM: scatter accumulator over T grown by RS in x and y and by RS towards
@@ -100,10 +105,10 @@ __device__ __forceinline__ float4 pad_load4(const PadRsrc &r, unsigned voff, int
     return make_float4(__uint_as_float(t[0]), __uint_as_float(t[1]), __uint_as_float(t[2]), __uint_as_float(t[3]));
 }
 
-template <int RS, bool SYM>
+template <int RP_, int RS, bool SYM>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SYM ? 2 : 3))) void nlm3d_bwd_rows(BwdRowsArgs a)
 {
-    using C = BwdRowsCfg<RS, SYM>;
+    using C = BwdRowsCfg<RP_, RS, SYM>;
     constexpr int RP = C::RP, QZ = C::QZ, GZ = C::GZ;
     extern __shared__ __align__(16) float lds[];
     float *SX = lds;
@@ -135,6 +140,18 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SYM ? 2 : 3)
     float cen[C::EX];
 #pragma unroll
     for (int i = 0; i < C::EX; ++i) cen[i] = pad_load(ru, rbase, 4 * (RS * pxy + RS * px + RS + i));
+    // quarter of an extra row (rows NR1 ... NROWS - 1): 4 x outputs from 4 + 2 RP centres
+    constexpr int EXQ = 4 + 2 * RP;
+    const int xr = C::NR1 + tid / 4, xj = tid % 4;
+    const int ydx = xr % C::DY, zdx = xr / C::DY;
+    const unsigned rbasex =
+        C::NRX > 0 ? 4u * (unsigned)pidx(Xt - RP - C::R + 4 * xj, Yt - 2 * RP + ydx - RS, Zt - 2 * RP + zdx - RS) : 0u;
+    const int sxwx = zdx * C::SXP + ydx * C::SXR + 4 * xj;
+    float cenx[EXQ];
+    if constexpr (C::NRX > 0) {
+#pragma unroll
+        for (int i = 0; i < EXQ; ++i) cenx[i] = pad_load(ru, rbasex, 4 * (RS * pxy + RS * px + RS + i));
+    }
 
     // ---- phase-2 item ----
     const bool p2ok = tid < C::P2N;
@@ -159,8 +176,18 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SYM ? 2 : 3)
     }
     const int sxyr = yq * 16 + xq;  // s_xy column base
     const int ezw = yq * 16 + xq;   // boxed-E column base
-    const int ezlo = (yq > 0 ? yq - 1 : 0) * 16 + xq;        // y neighbours (clamped in
-    const int ezhi = (yq < C::QY - 1 ? yq + 1 : yq) * 16 + xq;  // the halo rows, unused)
+    int ezr[C::W];  // y neighbours yq - RP ... yq + RP (clamped in the halo rows, unused there)
+#pragma unroll
+    for (int j = 0; j < C::W; ++j) {
+        const int yy = yq - RP + j;
+        ezr[j] = (yy < 0 ? 0 : (yy >= C::QY ? C::QY - 1 : yy)) * 16 + xq;
+    }
+    // y box of the boxed E at plane z (pairwise tree over the 2 RP + 1 rows)
+    auto ybox = [&](int z) {
+        const float *e = EZB + z * C::EZP;
+        if constexpr (RP == 1) return (e[ezr[0]] + e[ezr[1]]) + e[ezr[2]];
+        else return ((e[ezr[0]] + e[ezr[1]]) + e[ezr[2]]) + (e[ezr[3]] + e[ezr[4]]);
+    };
     // SYM visits only the offsets after d = 0 in (dz, dy, dx) order; d = 0
     // itself contributes G(z) (w_0 = 1, u~(z) - u~(z) = 0)
     float gacc[GZ];
@@ -198,8 +225,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SYM ? 2 : 3)
     // The shifted samples of this thread's phase-1 row for all dx of one
     // (dy, dz): loaded once (a lane reads its own row, so these loads touch a
     // cache line per lane) and slid by one sample per dx in registers.
-    constexpr int NWIN = C::EX + 2 * RS;
+    constexpr int NWIN = C::EX + 2 * RS, NWINX = EXQ + 2 * RS;
     float win[NWIN];
+    float winx[NWINX];
     // the column values at x + d, one offset ahead
     float ud[QZ], Gd[QZ], GYd[QZ];
     int sd = SYM ? s0 + 1 : 0;  // d = (1, 0, 0) or (-RS, -RS, -RS)
@@ -215,6 +243,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SYM ? 2 : 3)
                 win[m] = t.x;
                 win[m + 1] = t.y;
             }
+            if constexpr (C::NRX > 0) {
+#pragma unroll
+                for (int m = 0; m < NWINX; m += 2) {
+                    const float2 t = pad_load2(ru, rbasex, 4 * (sd + m));
+                    winx[m] = t.x;
+                    winx[m + 1] = t.y;
+                }
+            }
         }
         // phase 1: x sums of squared differences on the rows of D
         {
@@ -225,15 +261,30 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SYM ? 2 : 3)
                 qv[i] = e * e;
             }
             float sx[C::TX];
-            box_tree<3, C::TX>(qv, sx);
+            box_tree<C::W, C::TX>(qv, sx);
             float4 *dst = reinterpret_cast<float4 *>(SX + sxw);
 #pragma unroll
             for (int j = 0; j < C::TX / 4; ++j)
                 dst[j] = make_float4(sx[4 * j], sx[4 * j + 1], sx[4 * j + 2], sx[4 * j + 3]);
+            if constexpr (C::NRX > 0) {
+                float qx[EXQ];
+#pragma unroll
+                for (int i = 0; i < EXQ; ++i) {
+                    const float e = cenx[i] - winx[i];
+                    qx[i] = e * e;
+                }
+                float s4[4];
+                box_tree<C::W, 4>(qx, s4);
+                *reinterpret_cast<float4 *>(SX + sxwx) = make_float4(s4[0], s4[1], s4[2], s4[3]);
+            }
         }
-        // slide the window to the next dx
+        // slide the windows to the next dx
 #pragma unroll
         for (int m = 0; m < NWIN - 1; ++m) win[m] = win[m + 1];
+        if constexpr (C::NRX > 0) {
+#pragma unroll
+            for (int m = 0; m < NWINX - 1; ++m) winx[m] = winx[m + 1];
+        }
         lds_barrier();
         // phase 2: y sums
         if (p2ok) rows_phase2<C>(SX + p2src, SXY + p2dst);
@@ -243,7 +294,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SYM ? 2 : 3)
 #pragma unroll
         for (int j = 0; j < QZ + 2 * RP; ++j) v[j] = SXY[j * C::SYP + sxyr];
         float S[QZ];
-        box_tree<3, QZ>(v, S);
+        box_tree<C::W, QZ>(v, S);
         float EX[QZ], du[GZ], Gw[GZ];
 #pragma unroll
         for (int z = 0; z < QZ; ++z) {
@@ -263,12 +314,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SYM ? 2 : 3)
             // longer take the DPP operand itself -- one extra move per z.
             float Eo = E;
             asm("" : "+v"(Eo));
-            EX[z] = dpp_box<1>(Eo);
+            EX[z] = dpp_box<RP>(Eo);
         }
         // the next offset's column values fly across phase 4 and the next phases 1-2
         load_col(sn, ud, Gd, GYd);
         float EZ[GZ];
-        box_tree<3, GZ>(EX, EZ);
+        box_tree<C::W, GZ>(EX, EZ);
 #pragma unroll
         for (int z = 0; z < GZ; ++z) EZB[z * C::EZP + ezw] = EZ[z];
         lds_barrier();
@@ -280,7 +331,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SYM ? 2 : 3)
             float A[GZ];
 #pragma unroll
             for (int z = 0; z < GZ; ++z) {  // all reads first: the adds below may alias them
-                const float bE = (EZB[z * C::EZP + ezlo] + EZB[z * C::EZP + ezw]) + EZB[z * C::EZP + ezhi];
+                const float bE = ybox(z);
                 const float pz = bE * du[z];
                 gacc[z] = gacc[z] + pz;
                 A[z] = Gw[z] - pz;
@@ -302,7 +353,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SYM ? 2 : 3)
         } else {
 #pragma unroll
             for (int z = 0; z < GZ; ++z) {
-                const float bE = (EZB[z * C::EZP + ezlo] + EZB[z * C::EZP + ezw]) + EZB[z * C::EZP + ezhi];
+                const float bE = ybox(z);
                 gacc[z] = fmaf(bE, du[z], gacc[z]);
             }
         }
@@ -377,11 +428,11 @@ __global__ void bwd_pad(const float *u, const float *g, const float *y, const fl
 // in a fixed tile order.  Tile (tx, ty, tz) covers ext positions (xr, yr, zr)
 // (origin at -R) with lx = xr - tx GX + RS in [0, SCX), ly = yr - ty GY + RS
 // in [0, SCY), lz = zr - tz GZ in [0, SCZ), at lz SCP + ly SCX + lx of its slot.
-template <int RS>
+template <int RP, int RS>
 __global__ void bwd_sym_combine(const float *__restrict__ gsc, float *__restrict__ gext, int ex, int ey, int ez,
                                 int ntx, int nty, int ntz)
 {
-    using C = BwdRowsCfg<RS, true>;
+    using C = BwdRowsCfg<RP, RS, true>;
     const long long n = (long long)ex * ey * ez;
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
          i += (long long)gridDim.x * blockDim.x) {
@@ -409,17 +460,17 @@ __global__ void bwd_sym_combine(const float *__restrict__ gsc, float *__restrict
     }
 }
 
-template <int RS, bool SYM>
+template <int RP, int RS, bool SYM>
 constexpr size_t bwd_rows_lds()
 {
-    return BwdRowsCfg<RS, SYM>::LDS_FLOATS * sizeof(float);
+    return BwdRowsCfg<RP, RS, SYM>::LDS_FLOATS * sizeof(float);
 }
 
-template <int RS, bool SYM>
+template <int RP, int RS, bool SYM>
 hipError_t bwd_rows_t(const float *u, const float *y, const float *Z, const float *g, int nx, int ny, int nz,
                       float h, float *gu, float *gh, hipStream_t s)
 {
-    using C = BwdRowsCfg<RS, SYM>;
+    using C = BwdRowsCfg<RP, RS, SYM>;
     BwdRowsArgs a;
     a.nx = nx;
     a.ny = ny;
@@ -435,7 +486,7 @@ hipError_t bwd_rows_t(const float *u, const float *y, const float *Z, const floa
     const int nblocks = a.ntx * a.nty * a.ntz;
     // 32-bit byte offsets into the (G, GY, u~, 0) copy (16 bytes per position)
     if (16 * npad >= (1LL << 32)) return hipErrorInvalidValue;
-    const int P = 27;
+    const int P = C::W * C::W * C::W;
     const double c = 1.0 / ((double)P * (double)h * (double)h);
     a.k = (float)weight_scale(P, h);
     a.negc2 = (float)(-2.0 * c);
@@ -458,11 +509,11 @@ hipError_t bwd_rows_t(const float *u, const float *y, const float *Z, const floa
                        C::R);
     e = hipGetLastError();
     if (e == hipSuccess) {
-        const size_t lds = bwd_rows_lds<RS, SYM>();
-        e = hipFuncSetAttribute((const void *)nlm3d_bwd_rows<RS, SYM>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)lds);
+        const size_t lds = bwd_rows_lds<RP, RS, SYM>();
+        e = hipFuncSetAttribute((const void *)nlm3d_bwd_rows<RP, RS, SYM>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e == hipSuccess) {
-            hipLaunchKernelGGL((nlm3d_bwd_rows<RS, SYM>), dim3(nblocks), dim3(NT), lds, s, a);
+            hipLaunchKernelGGL((nlm3d_bwd_rows<RP, RS, SYM>), dim3(nblocks), dim3(NT), lds, s, a);
             e = hipGetLastError();
         }
     }
@@ -470,8 +521,8 @@ hipError_t bwd_rows_t(const float *u, const float *y, const float *Z, const floa
         const int ex = nx + 2 * C::R, ey = ny + 2 * C::R, ez = nz + 2 * C::R;
         long long cb = (ext + 255) / 256;
         if (cb > 65536) cb = 65536;
-        hipLaunchKernelGGL(bwd_sym_combine<RS>, dim3((unsigned)cb), dim3(256), 0, s, a.gsc, a.gext, ex, ey, ez, a.ntx,
-                           a.nty, a.ntz);
+        hipLaunchKernelGGL((bwd_sym_combine<RP, RS>), dim3((unsigned)cb), dim3(256), 0, s, a.gsc, a.gext, ex, ey, ez,
+                           a.ntx, a.nty, a.ntz);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = bwd3d_fold_launch(a.gext, gu, nx, ny, nz, C::R, s);
@@ -483,7 +534,9 @@ hipError_t bwd_rows_t(const float *u, const float *y, const float *Z, const floa
     return e != hipSuccess ? e : e2;
 }
 
-#define GNLM_BWD_ROWS_CASES(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7)
+#define GNLM_BWD_ROWS_CASES(X)                                                                \
+    X(1, 1) X(1, 2) X(1, 3) X(1, 4) X(1, 5) X(1, 6) X(1, 7) X(2, 1) X(2, 2) X(2, 3) X(2, 4) X(2, 5) \
+    X(2, 6) X(2, 7)
 
 std::atomic<int> g_bwd_form{kBwdAuto};
 
@@ -498,9 +551,9 @@ int bwd3d_set_form(int form)
 
 bool bwd3d_rows_supported(int r_p, int r_s)
 {
-    if (r_p != 1) return false;
-#define GNLM_BR_SUP(RS) \
-    if (r_s == RS) return bwd_rows_lds<RS, false>() <= kLdsLimitBytes && bwd_rows_lds<RS, true>() <= kLdsLimitBytes;
+#define GNLM_BR_SUP(RP, RS)                                                             \
+    if (r_p == RP && r_s == RS)                                                         \
+        return bwd_rows_lds<RP, RS, false>() <= kLdsLimitBytes && bwd_rows_lds<RP, RS, true>() <= kLdsLimitBytes;
     GNLM_BWD_ROWS_CASES(GNLM_BR_SUP)
 #undef GNLM_BR_SUP
     return false;
@@ -509,13 +562,12 @@ bool bwd3d_rows_supported(int r_p, int r_s)
 hipError_t bwd3d_rows_launch(const float *u, const float *y, const float *Z, const float *g, int nx, int ny, int nz,
                              int r_p, int r_s, float h, float *gu, float *gh, hipStream_t s)
 {
-    if (r_p != 1) return hipErrorInvalidValue;
     const int form = g_bwd_form.load(std::memory_order_relaxed);
-#define GNLM_BR_RUN(RS)                                                                            \
-    if (r_s == RS) {                                                                               \
+#define GNLM_BR_RUN(RP, RS)                                                                        \
+    if (r_p == RP && r_s == RS) {                                                                  \
         if (form != kBwdGather)                                                                    \
-            return bwd_rows_t<RS, true>(u, y, Z, g, nx, ny, nz, h, gu, gh, s);                     \
-        return bwd_rows_t<RS, false>(u, y, Z, g, nx, ny, nz, h, gu, gh, s);                        \
+            return bwd_rows_t<RP, RS, true>(u, y, Z, g, nx, ny, nz, h, gu, gh, s);                 \
+        return bwd_rows_t<RP, RS, false>(u, y, Z, g, nx, ny, nz, h, gu, gh, s);                    \
     }
     GNLM_BWD_ROWS_CASES(GNLM_BR_RUN)
 #undef GNLM_BR_RUN

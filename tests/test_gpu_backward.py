@@ -47,6 +47,10 @@ def form(request):
     ((14, 15, 13), 1, 5, 0.25),
     ((9, 10, 11), 1, 7, 0.3),   # largest instantiated r_s
     ((5, 4, 23), 1, 1, 0.3),
+    ((12, 12, 12), 2, 3, 0.2),  # patch radius 2: 20 x 16 phase-1 rows, quarter rows
+    ((26, 23, 21), 2, 4, 0.25), # several tiles per axis
+    ((9, 10, 11), 2, 7, 0.3),   # largest instantiated r_s
+    ((14, 13, 6), 2, 1, 0.3),
 ])
 def test_backward_matches_oracle(dev, form, shape, rp, rs, h):
     rng = np.random.default_rng(sum(shape) * 7 + rs)
@@ -70,16 +74,24 @@ def test_backward_deterministic(dev, form):
 
 
 def test_backward_forms_agree(dev):
-    """Gather and symmetric forms on a 40^3 volume (r_s 4): the same gradient to
-    FP32 rounding (different summation orders)."""
-    rng = np.random.default_rng(5)
+    """Gather and symmetric forms on a 40^3 volume: the same gradient to FP32
+    rounding (different summation orders)."""
+    _forms_agree(dev, 1, 4)
+
+
+def test_backward_forms_agree_rp2(dev):
+    _forms_agree(dev, 2, 3)
+
+
+def _forms_agree(dev, rp, rs):
+    rng = np.random.default_rng(5 + rp)
     u = rng.random((40, 40, 40)).astype(np.float32)
     g = rng.standard_normal(u.shape).astype(np.float32)
     try:
         gnlm.set_backward_form(gnlm.BWD_GATHER)
-        a = _gpu_grad(dev, u, g, 1, 4, 0.15)
+        a = _gpu_grad(dev, u, g, rp, rs, 0.15)
         gnlm.set_backward_form(gnlm.BWD_SYMMETRIC)
-        b = _gpu_grad(dev, u, g, 1, 4, 0.15)
+        b = _gpu_grad(dev, u, g, rp, rs, 0.15)
     finally:
         gnlm.set_backward_form(gnlm.BWD_AUTO)
     scale = np.abs(a[0]).max()
