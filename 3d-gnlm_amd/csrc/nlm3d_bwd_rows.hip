@@ -47,7 +47,7 @@ constexpr int NT = 256;
 template <int RP_, int RS_, bool SYM_>
 struct BwdRowsCfg {
     static constexpr int RP = RP_, RS = RS_, W = 2 * RP + 1, R = RP + RS;
-    static_assert(RP == 1 || RP == 2, "patch radius 1 or 2");
+    static_assert(RP >= 0 && RP <= 2, "patch radius 0, 1 or 2");
     static constexpr bool SYM = SYM_;
     static constexpr int H = 2 * R + RP;                  // halo of the padded copies (all axes)
     // gradient tile T, grown tile Q, phase-1 rows D
@@ -185,7 +185,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SYM ? 2 : 3)
     // y box of the boxed E at plane z (pairwise tree over the 2 RP + 1 rows)
     auto ybox = [&](int z) {
         const float *e = EZB + z * C::EZP;
-        if constexpr (RP == 1) return (e[ezr[0]] + e[ezr[1]]) + e[ezr[2]];
+        if constexpr (RP == 0) return e[ezr[0]];
+        else if constexpr (RP == 1) return (e[ezr[0]] + e[ezr[1]]) + e[ezr[2]];
         else return ((e[ezr[0]] + e[ezr[1]]) + e[ezr[2]]) + (e[ezr[3]] + e[ezr[4]]);
     };
     // SYM visits only the offsets after d = 0 in (dz, dy, dx) order; d = 0
@@ -535,8 +536,8 @@ hipError_t bwd_rows_t(const float *u, const float *y, const float *Z, const floa
 }
 
 #define GNLM_BWD_ROWS_CASES(X)                                                                \
-    X(1, 1) X(1, 2) X(1, 3) X(1, 4) X(1, 5) X(1, 6) X(1, 7) X(2, 1) X(2, 2) X(2, 3) X(2, 4) X(2, 5) \
-    X(2, 6) X(2, 7)
+    X(0, 1) X(0, 2) X(0, 3) X(0, 4) X(0, 5) X(0, 6) X(0, 7) X(1, 1) X(1, 2) X(1, 3) X(1, 4) X(1, 5) \
+    X(1, 6) X(1, 7) X(2, 1) X(2, 2) X(2, 3) X(2, 4) X(2, 5) X(2, 6) X(2, 7)
 
 std::atomic<int> g_bwd_form{kBwdAuto};
 

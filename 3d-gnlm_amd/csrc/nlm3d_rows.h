@@ -203,11 +203,16 @@ __device__ __forceinline__ void box_tree_big(const float *v, float *s)
 template <int W, int N>
 __device__ __forceinline__ void box_tree(const float *v, float *s)
 {
+    if constexpr (W == 1) {  // single-sample patches
+#pragma unroll
+        for (int i = 0; i < N; ++i) s[i] = v[i];
+        return;
+    }
     if constexpr (W > 7) {
         box_tree_big<W, N>(v, s);
         return;
     }
-    static_assert(W == 3 || W == 5 || W == 7 || W > 7, "odd patch widths");
+    static_assert(W % 2 == 1, "odd patch widths");
     constexpr int NP = N + W - 2;  // p_j for odd j < NP
     float p[NP];
 #pragma unroll

@@ -93,9 +93,9 @@ int gnlm2d_forward_ex(const float *d_in, int in_pad, long long in_sy, float *d_o
  * deterministic.  Scratch (padded copies, the gradient on the extended
  * domain [-R, n+R)^3 with R = r_p + r_s) comes from a stream-ordered
  * workspace released on `stream`.  Returns GNLM_EUNSUPPORTED for r_p > 2 or
- * radii beyond the instantiated ones (r_p = 1 or 2: r_s <= 7; r_p = 0: as
- * the LDS image of its tile allows).  r_p = 1 and 2 run the rows-organised
- * kernels in the form gnlm_set_backward_form selects. */
+ * radii beyond the instantiated ones (r_s <= 7 for r_p 0, 1, 2; beyond that
+ * as the LDS image of the fallback tile allows).  They run the
+ * rows-organised kernels in the form gnlm_set_backward_form selects. */
 int gnlm3d_backward(const float *d_in, const float *d_out, const float *d_Z, const float *d_gout, int nx,
                     int ny, int nz, int r_p, int r_s, float h, float *d_gin, float *d_gh, void *stream);
 

@@ -51,6 +51,7 @@ def form(request):
     ((26, 23, 21), 2, 4, 0.25), # several tiles per axis
     ((9, 10, 11), 2, 7, 0.3),   # largest instantiated r_s
     ((14, 13, 6), 2, 1, 0.3),
+    ((20, 21, 19), 0, 5, 0.3),  # single-voxel patches, several tiles
 ])
 def test_backward_matches_oracle(dev, form, shape, rp, rs, h):
     rng = np.random.default_rng(sum(shape) * 7 + rs)
