@@ -413,6 +413,30 @@ def main():
         parity = {"max_abs_vs_cpu_fp32": float(d.max()),
                   "max_rel_vs_cpu_fp32": float((d / np.maximum(np.abs(ref), 1e-6)).max()),
                   "max_rel_floor_1e-3_range": float((d / np.maximum(np.abs(ref), floor)).max())}
+        # the FP64 re-run of the same definition (separable FP64 path) on a
+        # slab of the sample (at most 16 planes) or the whole image
+        # the FP64 re-run of the same definition on part of the sample: 3D, the
+        # separable FP64 path on (at most) 16 planes; 2D, FP64 brute force on a
+        # 128^2 window (computed on the window grown by R = r_p + r_s, whose
+        # interior sees exactly the samples the full image gives it)
+        if nz > 1:
+            n64 = min(16, nplanes)
+            ref64 = oracle.nlm3d_separable_f64(u, rp, rs, h, z0=z0, z1=z0 + n64, nthreads=threads)
+            got64 = yc[z0:z0 + n64]
+            what = f"{n64} planes from z = {z0}"
+        else:
+            R, m = rp + rs, 128
+            y0, x0 = max(0, ny // 2 - m // 2), max(0, nx // 2 - m // 2)
+            ya, xa = max(0, y0 - R), max(0, x0 - R)
+            crop = u[ya:min(ny, y0 + m + R), xa:min(nx, x0 + m + R)].astype(np.float64)
+            full = oracle.nlm2d_brute(crop, rp, rs, h)
+            ref64 = full[y0 - ya:y0 - ya + m, x0 - xa:x0 - xa + m]
+            got64 = yc[y0:y0 + m, x0:x0 + m]
+            what = f"the {m}^2 window at ({x0}, {y0})"
+        d64 = np.abs(got64.astype(np.float64) - ref64)
+        parity["max_rel_vs_fp64"] = float((d64 / np.maximum(np.abs(ref64), 1e-6)).max())
+        parity["max_abs_vs_fp64"] = float(d64.max())
+        parity["fp64_sample"] = what
 
     line = {
         "metric": ((f"Mvoxels/s 3D NLM {nx}^3" if nx == ny == nz else f"Mvoxels/s 3D NLM {nx}x{ny}x{nz}") +
