@@ -261,6 +261,41 @@ __device__ __forceinline__ PadRsrc buf_rsrc(const void *base, unsigned bytes)
 }
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
+// V consecutive floats (V = 2 or 4) through a buffer resource, as raw bits.
+template <int V>
+__device__ __forceinline__ void buf_load_v(const PadRsrc &r, unsigned voff, int soff, unsigned *dst)
+{
+    if constexpr (V == 4) {
+        const u32x4 t = __builtin_amdgcn_raw_buffer_load_b128(r.r, (int)voff, soff, 0);
+        dst[0] = t[0];
+        dst[1] = t[1];
+        dst[2] = t[2];
+        dst[3] = t[3];
+    } else {
+        const u32x2 t = __builtin_amdgcn_raw_buffer_load_b64(r.r, (int)voff, soff, 0);
+        dst[0] = t[0];
+        dst[1] = t[1];
+    }
+}
+template <int V>
+__device__ __forceinline__ void buf_store_v(const PadRsrc &r, unsigned voff, const unsigned *src)
+{
+    if constexpr (V == 4) {
+        u32x4 t;
+        t[0] = src[0];
+        t[1] = src[1];
+        t[2] = src[2];
+        t[3] = src[3];
+        __builtin_amdgcn_raw_buffer_store_b128(t, r.r, (int)voff, 0, 0);
+    } else {
+        u32x2 t;
+        t[0] = src[0];
+        t[1] = src[1];
+        __builtin_amdgcn_raw_buffer_store_b64(t, r.r, (int)voff, 0, 0);
+    }
+}
 
 __device__ __forceinline__ float pad_load(const PadRsrc &r, unsigned voff, int soff)
 {
@@ -432,10 +467,10 @@ __device__ __forceinline__ void rows_tile(const RowsTile &ta, float *lds)
                                 RS);
     }
 
-    // V3 == 4: the flush of whole groups uses 128-bit buffer accesses
+    // V3 > 1: the flush of whole groups uses 64/128-bit buffer accesses
     const long long nvox = (long long)a.nx * a.ny * a.nz;
-    const bool vec_ok = V3 == 4 && (((unsigned long long)a.out | (unsigned long long)a.Zacc) & 15) == 0 &&
-                        (a.nx & 3) == 0 && nvox < (1LL << 30);
+    const bool vec_ok = V3 > 1 && (((unsigned long long)a.out | (unsigned long long)a.Zacc) & (4 * V3 - 1)) == 0 &&
+                        (a.nx % V3) == 0 && nvox < (1LL << 30);
     const unsigned vbytes = vec_ok ? (unsigned)(4 * nvox) : 0u;
     const PadRsrc ro = buf_rsrc(a.out, vbytes), rz = buf_rsrc(a.Zacc, vbytes);
 
@@ -467,13 +502,13 @@ __device__ __forceinline__ void rows_tile(const RowsTile &ta, float *lds)
                 for (int o = 0; o < RZ; ++o)
 #pragma unroll
                     for (int k = 0; k < C::P3R; ++k) {
-                        if constexpr (V3 == 4) {
-                            // the group's four samples in one load (dword-aligned: the
+                        if constexpr (V3 > 1) {
+                            // the group's samples in one load (dword-aligned: the
                             // shift dx moves the group by single samples)
-                            const auto t = __builtin_amdgcn_raw_buffer_load_b128(
-                                rs.r, (int)cb[k], 4 * (soffp + DX + RS + o * pxy), 0);
+                            unsigned t[V3];
+                            buf_load_v<V3>(rs, cb[k], 4 * (soffp + DX + RS + o * pxy), t);
 #pragma unroll
-                            for (int c = 0; c < 4; ++c) cand[k][c][o] = __uint_as_float(t[c]);
+                            for (int c = 0; c < V3; ++c) cand[k][c][o] = __uint_as_float(t[c]);
                         } else {
 #pragma unroll
                             for (int c = 0; c < V3; ++c)
@@ -528,38 +563,39 @@ __device__ __forceinline__ void rows_tile(const RowsTile &ta, float *lds)
             const int x = (it % NXG) * V3, y = (it / NXG) % C::TY, zr = it / (NXG * C::TY);
             const int gy = a.y0 + y, gz = a.z0 + zr * RZ;
             if (!p3ok[k] || it >= C::P3N || gy >= a.ny) continue;
-            if constexpr (V3 == 4) {
-                // a whole group at a 16-byte aligned position: one 128-bit
-                // access per buffer and plane (vec_ok: aligned buffers, rows of
-                // a multiple of 4 samples, byte offsets within 32 bits)
-                if (vec_ok && a.x0 + x + 3 < a.nx) {
+            if constexpr (V3 > 1) {
+                // a whole group at an aligned position: one 64/128-bit access per
+                // buffer and plane (vec_ok: aligned buffers, rows of a multiple of
+                // V3 samples, byte offsets within 32 bits)
+                if (vec_ok && a.x0 + x + V3 - 1 < a.nx) {
 #pragma unroll
                     for (int o = 0; o < RZ; ++o) {
                         if (gz + o >= a.nz) continue;
-                        const int boff = 4 * (a.x0 + x + a.nx * (gy + a.ny * (gz + o)));
-                        float Z[4], Y[4];
+                        const unsigned boff = 4u * (unsigned)(a.x0 + x + a.nx * (gy + a.ny * (gz + o)));
+                        float Z[V3], Y[V3];
 #pragma unroll
-                        for (int c = 0; c < 4; ++c) {
+                        for (int c = 0; c < V3; ++c) {
                             Z[c] = Zd[k][c][o];
                             Y[c] = Yd[k][c][o];
                         }
                         if (dz != -RS) {
-                            const u32x4 zt = __builtin_amdgcn_raw_buffer_load_b128(rz.r, boff, 0, 0);
-                            const u32x4 yt = __builtin_amdgcn_raw_buffer_load_b128(ro.r, boff, 0, 0);
+                            unsigned zt[V3], yt[V3];
+                            buf_load_v<V3>(rz, boff, 0, zt);
+                            buf_load_v<V3>(ro, boff, 0, yt);
 #pragma unroll
-                            for (int c = 0; c < 4; ++c) {
+                            for (int c = 0; c < V3; ++c) {
                                 Z[c] = __uint_as_float(zt[c]) + Z[c];
                                 Y[c] = __uint_as_float(yt[c]) + Y[c];
                             }
                         }
-                        u32x4 zs, ys;
+                        unsigned zs[V3], ys[V3];
 #pragma unroll
-                        for (int c = 0; c < 4; ++c) {
+                        for (int c = 0; c < V3; ++c) {
                             zs[c] = __float_as_uint(Z[c]);
                             ys[c] = __float_as_uint((dz == RS) ? Y[c] / Z[c] : Y[c]);
                         }
-                        __builtin_amdgcn_raw_buffer_store_b128(zs, rz.r, boff, 0, 0);
-                        __builtin_amdgcn_raw_buffer_store_b128(ys, ro.r, boff, 0, 0);
+                        buf_store_v<V3>(rz, boff, zs);
+                        buf_store_v<V3>(ro, boff, ys);
                     }
                     continue;
                 }
