@@ -90,7 +90,7 @@ struct RowsCfg {
     // read their columns four at a time (128-bit LDS reads: the phase's LDS
     // time halves, and 192 items leave a fourth wave free)
     static constexpr int RZ = SHORT ? 3 : 7;
-    static constexpr int V3 = SHORT ? 4 : 1;
+    static constexpr int V3 = SHORT ? 4 : (RP == 1 ? 2 : 1);
     static constexpr int P3N = (TX / V3) * TY * (TZ / RZ);
     static constexpr int P3R = (P3N + kRowsThreads - 1) / kRowsThreads;
     static_assert(TZ % RZ == 0, "whole z runs");
