@@ -11,23 +11,26 @@
 //     buffer: rows start on 16-byte boundaries and every sample a tile can
 //     touch exists, so the kernel reads rows with 128-bit global loads and
 //     no border logic.  The tile's working set (~100 KiB) stays in L2.
-//   * One workgroup (256 threads, 4 wave64) per 16 x TY x TZ tile; LDS holds
-//     only two small partial-sum buffers (~70 KiB for r_p = 2), so two
-//     workgroups share a CU and hide each other's barriers and latencies.
+//   * One workgroup (256 threads; 1024 for the large-patch tiles) per
+//     16 x TY x TZ tile; LDS holds only two partial-sum buffers (34 KiB for
+//     the short r_p = 2 tiles, up to 135 KiB for the large-patch ones), so up
+//     to four workgroups share a CU and hide each other's barriers.
 //   * Offsets are visited dz, dy, dx.  Phase-1 threads own whole rows
 //     (y_ext, z_ext) of the tile grown by r_p: the centre samples of a row
 //     stay in registers for the whole kernel, and the shifted row for one
 //     (dy, dz) is loaded once (16 + 2 LP samples) and serves all 2 r_s + 1
-//     dx offsets from registers -- the dx shift is a compile-time register
-//     index inside a uniform switch.
+//     dx offsets from registers -- dx is unrolled at compile time, so the
+//     shift is a static register index.
 //   * phase 1: squared differences, x box sums (shared pair sums) -> s_x rows
 //              (four 128-bit LDS stores per row);
 //     phase 2: (x vector, z_ext, run of y) items: y box sums -> s_xy;
-//     phase 3: (x, y, run of z) items: z box sums, w = exp2(S k), per-dz
-//              partial sums in registers; the candidate samples u~(x+d)
-//              come straight from the padded copy (L1/L2 hits).
-//     Two barriers per offset; every LDS access pattern is conflict-free
-//     for 64 banks (row pitches of 20 floats, plane paddings of 16 floats).
+//     phase 3: (group of V3 = 1, 2 or 4 x columns, y, run of z) items: z box
+//              sums, w = exp2(S k), per-dz partial sums in registers; the
+//              candidate samples u~(x+d) come straight from the padded copy
+//              (L1/L2 hits), one 32/64/128-bit load per group and plane.
+//     Two barriers per offset; the phase-1 and phase-2 patterns are
+//     conflict-free for 64 banks (row pitches of 20 floats, plane paddings
+//     of 16 floats).
 //   * Weights accumulate in two levels (partial over the (dy, dx) offsets of
 //     one dz, then the total over dz) as in the other variants; the totals
 //     live in the output buffers between dz steps (one read-modify-write of
