@@ -21,8 +21,9 @@
 //     (dy, dz) is loaded once (16 + 2 LP samples) and serves all 2 r_s + 1
 //     dx offsets from registers -- dx is unrolled at compile time, so the
 //     shift is a static register index.
-//   * phase 1: squared differences, x box sums (shared pair sums) -> s_x rows
-//              (four 128-bit LDS stores per row);
+//   * phase 1: differences, x box sums of their squares (shared pair sums,
+//              half of the squares folded into fused multiply-adds) -> s_x
+//              rows (four 128-bit LDS stores per row);
 //     phase 2: (x vector, z_ext, run of y) items: y box sums -> s_xy;
 //     phase 3: (group of V3 = 1, 2 or 4 x columns, y, run of z) items: z box
 //              sums, w = exp2(S k), per-dz partial sums in registers; the
@@ -203,6 +204,67 @@ __device__ __forceinline__ void box_tree_big(const float *v, float *s)
     }
 }
 
+// s[i] = e[i]^2 + ... + e[i+W-1]^2 for i < N: box_tree over the squares of
+// e, with the squares of the even-indexed samples folded into the sums as
+// fused multiply-adds (each such square feeds one pair sum and one output, both
+// of which take it as fma(e, e, .)); only the odd-indexed squares are formed,
+// which saves (N + W - 1) / 2 multiplies against squaring first.
+template <int W, int N>
+__device__ __forceinline__ void box_tree_sq(const float *e, float *s)
+{
+    static_assert(W % 2 == 1 && W <= 31, "odd patch widths up to 31");
+    constexpr int NV = N + W - 1;  // inputs
+    if constexpr (W == 1) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) s[i] = e[i] * e[i];
+        return;
+    }
+    float vo[NV];  // squares at odd indices
+#pragma unroll
+    for (int k = 1; k < NV; k += 2) vo[k] = e[k] * e[k];
+    if constexpr (W <= 7) {
+        constexpr int NP = N + W - 2;  // p_j for odd j < NP
+        float p[NP];
+#pragma unroll
+        for (int j = 1; j < NP; j += 2) p[j] = fmaf(e[j + 1], e[j + 1], vo[j]);
+#pragma unroll
+        for (int i = 0; i < N; i += 2) {
+            float m;
+            if constexpr (W == 3) m = p[i + 1];
+            else if constexpr (W == 5) m = p[i + 1] + p[i + 3];
+            else m = (i % 4 == 0) ? (p[i + 1] + p[i + 3]) + p[i + 5] : p[i + 1] + (p[i + 3] + p[i + 5]);
+            s[i] = fmaf(e[i], e[i], m);
+            if (i + 1 < N) s[i + 1] = m + vo[i + W];
+        }
+    } else {
+        constexpr int M = (W - 1) / 2;
+        float B[4][NV];
+#pragma unroll
+        for (int k = 1; k + 1 < NV; k += 2) B[0][k] = fmaf(e[k + 1], e[k + 1], vo[k]);
+#pragma unroll
+        for (int b = 1; b < 4; ++b) {
+#pragma unroll
+            for (int k = 1; k + (2 << b) <= NV; k += 2) B[b][k] = B[b - 1][k] + B[b - 1][k + (1 << b)];
+        }
+#pragma unroll
+        for (int i = 0; i < N; i += 2) {
+            float m = 0.0f;
+            int off = i + 1;
+            bool first = true;
+#pragma unroll
+            for (int b = 3; b >= 0; --b) {
+                if ((M >> b) & 1) {
+                    m = first ? B[b][off] : m + B[b][off];
+                    first = false;
+                    off += 2 << b;
+                }
+            }
+            s[i] = fmaf(e[i], e[i], m);
+            if (i + 1 < N) s[i + 1] = m + vo[i + W];
+        }
+    }
+}
+
 template <int W, int N>
 __device__ __forceinline__ void box_tree(const float *v, float *s)
 {
@@ -339,14 +401,11 @@ __device__ __forceinline__ void rows_phase1(const float (&cen)[C::ROWS][C::EX], 
 #pragma unroll
     for (int q = 0; q < C::ROWS; ++q) {
         if (!ok[q]) continue;
-        float qv[C::EX];
+        float e[C::EX];
 #pragma unroll
-        for (int i = 0; i < C::EX; ++i) {
-            const float e = cen[q][i] - sh[q][SH + i];
-            qv[i] = e * e;
-        }
+        for (int i = 0; i < C::EX; ++i) e[i] = cen[q][i] - sh[q][SH + i];
         float s[C::TX];
-        box_tree<C::W, C::TX>(qv, s);
+        box_tree_sq<C::W, C::TX>(e, s);
         float4 *dst = reinterpret_cast<float4 *>(SX + sxw[q]);
 #pragma unroll
         for (int j = 0; j < C::TX / 4; ++j) dst[j] = make_float4(s[4 * j], s[4 * j + 1], s[4 * j + 2], s[4 * j + 3]);
