@@ -51,12 +51,17 @@ __device__ __forceinline__ float exp2_fast(float x) { return __builtin_amdgcn_ex
 // blockIdx -> tile index, grouping consecutive tiles on one XCD.  The
 // dispatcher hands workgroup b to XCD (b % 8); tiles that share halo data
 // are adjacent in tile order, so giving each XCD a contiguous run of tiles
-// keeps their shared input in that XCD's L2.
-__device__ __forceinline__ int xcd_tile(int b, int nblocks)
+// keeps their shared input in that XCD's L2.  The first nmain tiles (rounded
+// down to a multiple of 8) are split into 8 such runs; the tiles after them
+// keep their order and are dealt round robin -- a launch passes the count of
+// its full-cost tiles, so that cheaper tiles at the end of the tile order (a
+// partial last layer) are spread over all XCDs and fill the last round
+// instead of all landing in one XCD's run.
+__device__ __forceinline__ int xcd_tile(int b, int nmain)
 {
     constexpr int NXCD = 8;
-    if (nblocks % NXCD != 0) return b;
-    return (b % NXCD) * (nblocks / NXCD) + (b / NXCD);
+    const int q = nmain / NXCD;
+    return b < NXCD * q ? (b % NXCD) * q + (b / NXCD) : b;
 }
 
 }  // namespace gnlm

@@ -20,11 +20,14 @@ struct RowsArgs {
     int ntx, nty, ntz;
 };
 
-// The tile of workgroup blockIdx.x (tiles of one XCD are contiguous).
+// The tile of workgroup blockIdx.x (tiles of one XCD are contiguous).  Tiles
+// run z-layer by z-layer; a partial last layer (nz not a multiple of TZ) skips
+// the work of its missing planes, so it is dealt over the XCDs last.
 template <class C>
 __device__ __forceinline__ void rows_kernel_body(const RowsArgs &a, float *lds)
 {
-    const int t = xcd_tile(blockIdx.x, a.ntx * a.nty * a.ntz);
+    const int nfull = a.ntx * a.nty * (a.ntz - (a.nz % C::TZ != 0 ? 1 : 0));
+    const int t = xcd_tile(blockIdx.x, nfull);
     RowsTile tl;
     tl.v = a.v;
     tl.out = a.out;
