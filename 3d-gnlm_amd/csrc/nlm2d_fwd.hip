@@ -11,16 +11,23 @@
 //   * per offset, phase 1 forms x-direction patch-row sums (s_x) for the
 //     tile's 32 + 2 r_p rows in segments of 16 outputs; each thread keeps its
 //     segment's centre samples in registers for the whole offset loop and
-//     reads only the shifted samples from LDS.  s_x is stored column-major
-//     with an odd pitch so both its writes and the column reads are
-//     conflict-free;
+//     reads only the shifted samples from LDS.  The sums are shared pairwise
+//     trees over the differences (half of the squares folded into fused
+//     multiply-adds), about 3 operations per output for any nb instead of
+//     nb.  s_x is stored column-major with an odd pitch so both its writes
+//     and the column reads are conflict-free; it is double-buffered, so one
+//     barrier per offset suffices;
 //   * phase 2: a thread owns 8 consecutive rows of one column, loads the
-//     8 + 2 r_p s_x values it needs once, sums each output's window directly
-//     in registers, and accumulates Z and Y in registers.
-// Sums follow the canonical order (x sums fused with fma, then y sums left to
-// right; weights summed per dy, then over dy), identical to the CPU oracle and
-// the generic path.
+//     8 + 2 r_p s_x values it needs once, forms the 8 window sums as shared
+//     trees in registers, and accumulates Z and Y in registers.
+// The weights are summed per dy, then over dy, as in the CPU oracle and the
+// generic path.  The patch sums add the same non-negative terms in a
+// different order (trees instead of left to right), so results agree with
+// those to rounding (~1e-7 relative in S); for integer-valued input (the
+// uint8 images of the reference) every S is an exact integer in FP32 and the
+// results are identical bit for bit.
 #include "gnlm_internal.h"
+#include "nlm_box.h"
 
 namespace gnlm {
 namespace {
@@ -130,14 +137,11 @@ __global__ __launch_bounds__(NT) void nlm2d_fwd_tiled(Fwd2Args a)
                     float e[G::ES];
 #pragma unroll
                     for (int i = 0; i < G::ES; ++i) e[i] = cen[q][i] - sh[i];
+                    float sx[G::SEG];
+                    box_tree_sq<2 * RP + 1, G::SEG>(e, sx);
                     float *dst = SX + (sg * G::SEG) * G::EYP + ry;
 #pragma unroll
-                    for (int j = 0; j < G::SEG; ++j) {
-                        float acc = 0.0f;
-#pragma unroll
-                        for (int i = 0; i <= 2 * RP; ++i) acc = fmaf(e[j + i], e[j + i], acc);
-                        dst[j * G::EYP] = acc;
-                    }
+                    for (int j = 0; j < G::SEG; ++j) dst[j * G::EYP] = sx[j];
                 }
             }
             __syncthreads();
@@ -147,12 +151,11 @@ __global__ __launch_bounds__(NT) void nlm2d_fwd_tiled(Fwd2Args a)
 #pragma unroll
             for (int j = 0; j < G::YR + 2 * RP; ++j) col[j] = src[j];
             const float *cand = U + candbase + doff;
+            float S[G::YR];
+            box_tree<2 * RP + 1, G::YR>(col, S);
 #pragma unroll
             for (int m = 0; m < G::YR; ++m) {
-                float S = 0.0f;
-#pragma unroll
-                for (int j = 0; j <= 2 * RP; ++j) S = S + col[m + j];
-                const float w = exp2_fast(S * a.k);
+                const float w = exp2_fast(S[m] * a.k);
                 Zd[m] = Zd[m] + w;
                 Yd[m] = fmaf(w, cand[m * LX], Yd[m]);
             }

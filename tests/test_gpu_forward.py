@@ -327,11 +327,19 @@ def test_2d_matches_oracle(dev, shape, rp, rs, h):
 
 @pytest.mark.parametrize("shape,rp,rs,h", CASES2[:3])
 def test_2d_tiled_equals_generic(dev, shape, rp, rs, h):
+    """The tiled kernel sums patches in trees, the generic path left to right:
+    on integer-valued input (the reference's uint8 images, nb <= 11) every
+    patch distance is an exact FP32 integer in both, so the results are
+    identical bit for bit; on general input they agree to rounding."""
     rng = np.random.default_rng(3)
+    u8 = rng.integers(0, 256, shape).astype(np.float32)
+    a = _run2(dev, u8, rp, rs, 255.0 * h, gnlm.PATH_TILED)
+    b = _run2(dev, u8, rp, rs, 255.0 * h, gnlm.PATH_GENERIC)
+    assert np.array_equal(a, b), report(a, b)
     u = rng.random(shape, dtype=np.float32)
     a = _run2(dev, u, rp, rs, h, gnlm.PATH_TILED)
     b = _run2(dev, u, rp, rs, h, gnlm.PATH_GENERIC)
-    assert np.array_equal(a, b), report(a, b)
+    assert max_rel(a, b) <= 2e-6, report(a, b)
 
 
 def test_2d_config2(dev):
