@@ -277,6 +277,8 @@ def pmc_pass(args):
             gnlm.nlm3d(torch.from_numpy(u).to(dev), wl["rp"], wl["rs"], wl["h"])
         else:
             u, _ = gnlm.phantom2d(*wl["dims2"], sigma=wl["sigma"], seed=wl["seed"])
+            if wl.get("u8"):
+                u = np.clip(np.rint(u * 255.0), 0, 255).astype(np.float32)
             gnlm.nlm2d(torch.from_numpy(u).to(dev), wl["rp"], wl["rs"], wl["h"])
     a = torch.empty(64 * 1024 * 1024, device=dev)
     b = torch.empty_like(a)
