@@ -389,18 +389,41 @@ def main():
         yc = y.cpu().numpy()
         threads = _threads()
         if nz > 1:
-            # cost per plane ~ voxels x offsets x patch width^2.5 (measured on the
-            # separable path: 256 planes of the headline volume, 256^2 s11 p5,
-            # take about 13 s; 13 planes of 512^2 s13 p17 took 109 s)
-            budget = 1.44e12 / (nx * ny * (2 * rs + 1) ** 3 * (2 * rp + 1) ** 2.5)
-            nplanes = max(1, min(args.cpu_planes if args.cpu_planes > 0 else int(budget), nz))
-            z0 = nz // 2 - nplanes // 2
+            # Cost of the separable CPU path on 16 threads: about
+            # 7e-12 s x (x y voxels) x (planes + 2 r_p) x offsets x (2 r_p + 1)^2.4
+            # (256 planes of the headline volume, 256^2 s11 p5, take 9-13 s; 2
+            # planes of 512^2 s13 p17 took 81 s).  Take as many planes as about
+            # 15 s allow; when even one plane of the full x-y extent would take
+            # longer, one plane of a centred x-y window instead, computed on the
+            # window grown by R = r_p + r_s in every axis (its interior sees
+            # exactly the samples the full volume gives it).
+            R = rp + rs
+            per = 7e-12 * (2 * rs + 1) ** 3 * (2 * rp + 1) ** 2.4
+            nplanes = int(15.0 / (per * nx * ny)) - 2 * rp
+            wx, wy = nx, ny
+            if nplanes < 1:
+                nplanes = 1
+                side = int((15.0 / (per * (1 + 2 * rp))) ** 0.5)
+                w = max(32, (side - 2 * R) // 16 * 16)
+                wx, wy = (w if w + 2 * R <= nx else nx), (w if w + 2 * R <= ny else ny)
+            nplanes = max(1, min(args.cpu_planes if args.cpu_planes > 0 else nplanes, nz))
+            z0, y0, x0 = nz // 2 - nplanes // 2, (ny - wy) // 2, (nx - wx) // 2
+            # the crop: the sample grown by R, clipped to the volume (a clipped
+            # face is the volume's own face, where reflect-101 applies as usual)
+            cz0, cy0, cx0 = (max(0, z0 - R), max(0, y0 - R), max(0, x0 - R)) if (wx, wy) != (nx, ny) else (0, 0, 0)
+            cz1, cy1, cx1 = ((min(nz, z0 + nplanes + R), min(ny, y0 + wy + R), min(nx, x0 + wx + R))
+                             if (wx, wy) != (nx, ny) else (nz, ny, nx))
+            crop = np.ascontiguousarray(u[cz0:cz1, cy0:cy1, cx0:cx1])
+            win = (slice(None), slice(y0 - cy0, y0 - cy0 + wy), slice(x0 - cx0, x0 - cx0 + wx))
             tc = time.perf_counter()
-            ref = oracle.nlm3d_separable_f32(u, rp, rs, h, z0=z0, z1=z0 + nplanes, nthreads=threads)
+            ref = oracle.nlm3d_separable_f32(crop, rp, rs, h, z0=z0 - cz0, z1=z0 - cz0 + nplanes,
+                                             nthreads=threads)[win]
             tcpu = time.perf_counter() - tc
-            got = yc[z0:z0 + nplanes]
-            sample = f"z-planes [{z0},{z0 + nplanes}) of the same volume ({nplanes * nx * ny} voxels)"
-            svox = nplanes * nx * ny
+            got = yc[z0:z0 + nplanes, y0:y0 + wy, x0:x0 + wx]
+            svox = nplanes * wx * wy
+            sample = (f"z-planes [{z0},{z0 + nplanes}) of the same volume ({svox} voxels)" if (wx, wy) == (nx, ny)
+                      else f"the {wx} x {wy} window at ({x0}, {y0}) of z-plane {z0} ({svox} voxels; the CPU "
+                      f"path runs on the window grown by {R})")
         else:
             tc = time.perf_counter()
             ref = oracle.nlm2d_separable_f32(u, rp, rs, h, nthreads=threads)
@@ -423,9 +446,10 @@ def main():
         # interior sees exactly the samples the full image gives it)
         if nz > 1:
             n64 = min(16, nplanes)
-            ref64 = oracle.nlm3d_separable_f64(u, rp, rs, h, z0=z0, z1=z0 + n64, nthreads=threads)
-            got64 = yc[z0:z0 + n64]
-            what = f"{n64} planes from z = {z0}"
+            ref64 = oracle.nlm3d_separable_f64(crop, rp, rs, h, z0=z0 - cz0, z1=z0 - cz0 + n64, nthreads=threads)[win]
+            got64 = got[:n64]
+            what = (f"{n64} planes from z = {z0}" if (wx, wy) == (nx, ny)
+                    else f"the same {wx} x {wy} window of z-plane {z0}")
         else:
             R, m = rp + rs, 128
             y0, x0 = max(0, ny // 2 - m // 2), max(0, nx // 2 - m // 2)
