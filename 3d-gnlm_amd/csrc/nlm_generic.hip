@@ -9,8 +9,10 @@
 //   pass Y : s_xy(x,y,z) = sum_j s_x         on  y in [0,ny), z extended
 //   pass ZW: S = sum_k s_xy;  w = exp2(S k);  Zd += w;  Yd = fma(w, u~(x+d), Yd)
 //   after each dz (3D) / dy (2D): Z += Zd, Y += Yd (two-level accumulation)
-// The arithmetic order is the canonical one shared with the tiled kernels and
-// the CPU oracle, so all paths agree bit for bit on the patch distances.
+// The arithmetic order is the canonical one of the CPU oracle and of the 3D
+// direct-sum tiles (variants 0 and 1), which agree with this path bit for bit
+// on the patch distances; the rows tiles and the 2D tile sum in pairwise
+// trees and agree to rounding (bit for bit on integer-valued input).
 // The passes are HBM/L2-bound (about 30 B per voxel per offset) and each
 // offset costs three launches; the path trades speed for generality.
 #include "gnlm_internal.h"
