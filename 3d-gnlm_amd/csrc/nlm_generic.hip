@@ -12,7 +12,8 @@
 // The arithmetic order is the canonical one of the CPU oracle and of the 3D
 // direct-sum tiles (variants 0 and 1), which agree with this path bit for bit
 // on the patch distances; the rows tiles and the 2D tile sum in pairwise
-// trees and agree to rounding (bit for bit on integer-valued input).
+// trees and agree to rounding (bit for bit on 8-bit integer-valued input while
+// P * 255^2 < 2^24, where every partial sum is an exact FP32 integer).
 // The passes are HBM/L2-bound (about 30 B per voxel per offset) and each
 // offset costs three launches; the path trades speed for generality.
 #include "gnlm_internal.h"
