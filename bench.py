@@ -16,8 +16,9 @@ peak = 157.3 TFLOP/s (MI355X FP32 vector).  HBM traffic per launch comes
 from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs kept in profiles/.
 
 cpu_baseline: the CPU FP32 path of the oracle (separable, OpenMP), timed on
-this host's cores on a bounded sample of the same volume (a slab of z-planes);
-the sample's outputs are also compared with the GPU's.
+this host's cores on a bounded sample of the same volume (a slab of z-planes,
+or for large patches one x-y window of a plane); the sample's outputs are also
+compared with the GPU's.
 """
 import argparse
 import json
@@ -61,6 +62,40 @@ def _traffic_from_profiles(name):
             return json.load(f).get(name)
     except (OSError, ValueError):
         return None
+
+
+def cpu_sample(nx, ny, nz, rp, rs, planes=0, budget_s=15.0):
+    """The bounded CPU-baseline sample of a 3D workload.
+
+    Cost of the separable CPU path on 16 threads: about 7e-12 s x (x y voxels)
+    x (planes + 2 r_p) x offsets x (2 r_p + 1)^2.4 (256 planes of the headline
+    volume, 256^2 s11 p5, take 9-13 s; 2 planes of 512^2 s13 p17 took 81 s).
+    Takes as many centred z-planes as budget_s allows (or `planes` if > 0).
+    When even one plane of the full x-y extent would take longer, it takes one
+    plane of a centred x-y window instead; the CPU path then runs on the
+    window grown by R = r_p + r_s in every axis (clipped to the volume: a
+    clipped face is the volume's own, where reflect-101 applies as usual), so
+    its interior sees exactly the samples the full volume gives it.
+
+    Returns (z0, nplanes, y0, wy, x0, wx) of the sample and the crop
+    (cz0, cz1, cy0, cy1, cx0, cx1) the CPU path runs on."""
+    R = rp + rs
+    per = 7e-12 * (2 * rs + 1) ** 3 * (2 * rp + 1) ** 2.4
+    nplanes = int(budget_s / (per * nx * ny)) - 2 * rp
+    wx, wy = nx, ny
+    if nplanes < 1:
+        nplanes = 1
+        side = int((budget_s / (per * (1 + 2 * rp))) ** 0.5)
+        w = max(32, (side - 2 * R) // 16 * 16)
+        wx, wy = (w if w + 2 * R <= nx else nx), (w if w + 2 * R <= ny else ny)
+    nplanes = max(1, min(planes if planes > 0 else nplanes, nz))
+    z0, y0, x0 = nz // 2 - nplanes // 2, (ny - wy) // 2, (nx - wx) // 2
+    if (wx, wy) == (nx, ny):
+        crop = (0, nz, 0, ny, 0, nx)
+    else:
+        crop = (max(0, z0 - R), min(nz, z0 + nplanes + R), max(0, y0 - R), min(ny, y0 + wy + R), max(0, x0 - R),
+                min(nx, x0 + wx + R))
+    return (z0, nplanes, y0, wy, x0, wx), crop
 
 
 def _threads():
@@ -389,30 +424,9 @@ def main():
         yc = y.cpu().numpy()
         threads = _threads()
         if nz > 1:
-            # Cost of the separable CPU path on 16 threads: about
-            # 7e-12 s x (x y voxels) x (planes + 2 r_p) x offsets x (2 r_p + 1)^2.4
-            # (256 planes of the headline volume, 256^2 s11 p5, take 9-13 s; 2
-            # planes of 512^2 s13 p17 took 81 s).  Take as many planes as about
-            # 15 s allow; when even one plane of the full x-y extent would take
-            # longer, one plane of a centred x-y window instead, computed on the
-            # window grown by R = r_p + r_s in every axis (its interior sees
-            # exactly the samples the full volume gives it).
+            (z0, nplanes, y0, wy, x0, wx), (cz0, cz1, cy0, cy1, cx0, cx1) = cpu_sample(nx, ny, nz, rp, rs,
+                                                                                      args.cpu_planes)
             R = rp + rs
-            per = 7e-12 * (2 * rs + 1) ** 3 * (2 * rp + 1) ** 2.4
-            nplanes = int(15.0 / (per * nx * ny)) - 2 * rp
-            wx, wy = nx, ny
-            if nplanes < 1:
-                nplanes = 1
-                side = int((15.0 / (per * (1 + 2 * rp))) ** 0.5)
-                w = max(32, (side - 2 * R) // 16 * 16)
-                wx, wy = (w if w + 2 * R <= nx else nx), (w if w + 2 * R <= ny else ny)
-            nplanes = max(1, min(args.cpu_planes if args.cpu_planes > 0 else nplanes, nz))
-            z0, y0, x0 = nz // 2 - nplanes // 2, (ny - wy) // 2, (nx - wx) // 2
-            # the crop: the sample grown by R, clipped to the volume (a clipped
-            # face is the volume's own face, where reflect-101 applies as usual)
-            cz0, cy0, cx0 = (max(0, z0 - R), max(0, y0 - R), max(0, x0 - R)) if (wx, wy) != (nx, ny) else (0, 0, 0)
-            cz1, cy1, cx1 = ((min(nz, z0 + nplanes + R), min(ny, y0 + wy + R), min(nx, x0 + wx + R))
-                             if (wx, wy) != (nx, ny) else (nz, ny, nx))
             crop = np.ascontiguousarray(u[cz0:cz1, cy0:cy1, cx0:cx1])
             win = (slice(None), slice(y0 - cy0, y0 - cy0 + wy), slice(x0 - cx0, x0 - cx0 + wx))
             tc = time.perf_counter()

@@ -1,0 +1,45 @@
+"""The CPU-baseline sample of bench.py (bench.cpu_sample), on the CPU.
+
+For large patches the sample is one x-y window of a plane, computed by the
+CPU path on the window grown by R = r_p + r_s.  The window's outputs must be
+exactly those of the CPU path run on the whole volume.  The separable path
+reads only samples within R of an output, and a clipped face of the crop is
+the volume's own face.
+"""
+import numpy as np
+
+import bench
+import oracle
+
+
+def test_headline_sample_is_the_whole_volume():
+    (z0, n, y0, wy, x0, wx), crop = bench.cpu_sample(256, 256, 256, 2, 5)
+    assert (z0, n, y0, wy, x0, wx) == (0, 256, 0, 256, 0, 256)
+    assert crop == (0, 256, 0, 256, 0, 256)
+
+
+def test_large_patch_sample_is_a_window():
+    nx, ny, nz, rp, rs = 512, 512, 128, 8, 6
+    (z0, n, y0, wy, x0, wx), (cz0, cz1, cy0, cy1, cx0, cx1) = bench.cpu_sample(nx, ny, nz, rp, rs)
+    R = rp + rs
+    assert n == 1 and wx == wy and wx % 16 == 0 and wx < nx
+    assert (cz0, cz1) == (z0 - R, z0 + 1 + R)
+    assert (cy0, cy1, cx0, cx1) == (y0 - R, y0 + wy + R, x0 - R, x0 + wx + R)
+    assert 0 <= cy0 and cy1 <= ny and 0 <= cx0 and cx1 <= nx
+
+
+def test_window_equals_whole_volume_result():
+    rng = np.random.default_rng(21)
+    nz, ny, nx, rp, rs, h = 30, 40, 44, 2, 3, 0.2
+    u = rng.random((nz, ny, nx), dtype=np.float32)
+    # a tiny budget forces the window: 32 wide in x; y (40 < 32 + 2R) stays whole
+    (z0, n, y0, wy, x0, wx), (cz0, cz1, cy0, cy1, cx0, cx1) = bench.cpu_sample(nx, ny, nz, rp, rs, budget_s=1e-6)
+    assert n == 1 and wx == 32 and wy == ny
+    crop = np.ascontiguousarray(u[cz0:cz1, cy0:cy1, cx0:cx1])
+    win = (slice(None), slice(y0 - cy0, y0 - cy0 + wy), slice(x0 - cx0, x0 - cx0 + wx))
+    got = oracle.nlm3d_separable_f32(crop, rp, rs, h, z0=z0 - cz0, z1=z0 - cz0 + n)[win]
+    ref = oracle.nlm3d_separable_f32(u, rp, rs, h, z0=z0, z1=z0 + n)[:, y0:y0 + wy, x0:x0 + wx]
+    assert np.array_equal(got, ref)
+    got64 = oracle.nlm3d_separable_f64(crop, rp, rs, h, z0=z0 - cz0, z1=z0 - cz0 + n)[win]
+    ref64 = oracle.nlm3d_separable_f64(u, rp, rs, h, z0=z0, z1=z0 + n)[:, y0:y0 + wy, x0:x0 + wx]
+    assert np.array_equal(got64, ref64)
