@@ -27,7 +27,7 @@ hipError_t fwd3d_tiled_launch(const InDesc &in, float *out, float *Zout, int r_p
                               hipStream_t s);
 
 // ---- "rows" variant (nlm3d_rows.hip) ---------------------------------------
-// r_p in {1, 2} with r_s up to 7 / 6; pads the input into a stream-ordered
+// r_p in {1, 2, 3} with r_s up to 7 / 6 / 5; pads the input into a stream-ordered
 // workspace first (caller-padded input needs pad >= r_p + r_s).
 // Large patches (r_p 4-8): 16 x (32 - 2 r_p)^2 tiles on 1024 threads.
 bool fwd3d_rows_big_supported(int r_p, int r_s);
@@ -58,7 +58,7 @@ hipError_t fwd_generic_launch(const InDesc &in, float *out, float *Zout, int rp,
 bool bwd3d_supported(int r_p, int r_s);
 hipError_t bwd3d_launch(const float *u, const float *y, const float *Z, const float *g, int nx, int ny, int nz,
                         int rp, int rs, float h, float *gu, float *gh, hipStream_t s);
-// rows-organised adjoint for r_p = 1, r_s in 1..7 (bwd3d_launch dispatches to it)
+// rows-organised adjoint for r_p 0-2, r_s in 1..7 (bwd3d_launch dispatches to it)
 bool bwd3d_rows_supported(int r_p, int r_s);
 hipError_t bwd3d_rows_launch(const float *u, const float *y, const float *Z, const float *g, int nx, int ny, int nz,
                              int r_p, int r_s, float h, float *gu, float *gh, hipStream_t s);

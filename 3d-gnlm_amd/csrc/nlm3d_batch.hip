@@ -79,8 +79,8 @@ __device__ __noinline__ void tile_rows(const RowsTile &a, float *lds) { rows_til
 
 // kVRows / kVRowsShort tile: r_s is a template parameter of the tile body (its
 // dx loop is unrolled), so dispatch on it; the cases match nlm3d_rows.hip.
-// LIGHT: only the bodies that fit three waves per SIMD (r_p = 1 rows, r_p = 2
-// short rows).
+// LIGHT: only the bodies that fit three waves per SIMD (r_p = 1 and 3 rows,
+// r_p = 2 short rows).
 template <int RP, bool LIGHT>
 __device__ __forceinline__ void run_rows_rp(const BatchVol &v, const BatchTile &t, float *lds)
 {
@@ -126,6 +126,14 @@ __device__ __forceinline__ void run_rows_rp(const BatchVol &v, const BatchTile &
             case 5: tile_rows<2, 5, false, false>(a, lds); break;
             default: tile_rows<2, 6, false, false>(a, lds); break;
             }
+        }
+    } else if constexpr (RP == 3) {
+        switch (v.rs) {
+        case 1: tile_rows<3, 1, false, LIGHT>(a, lds); break;
+        case 2: tile_rows<3, 2, false, LIGHT>(a, lds); break;
+        case 3: tile_rows<3, 3, false, LIGHT>(a, lds); break;
+        case 4: tile_rows<3, 4, false, LIGHT>(a, lds); break;
+        default: tile_rows<3, 5, false, LIGHT>(a, lds); break;
         }
     }
 }
@@ -191,7 +199,7 @@ __device__ __forceinline__ void run_tile_rp(const BatchVol &v, const BatchTile &
 // Volume v can run in the LIGHT kernel (three waves per SIMD).
 __host__ __device__ inline bool batch_light(int rp, int variant)
 {
-    return (rp == 1 && variant == kVRows) || (rp == 2 && variant == kVRowsShort);
+    return ((rp == 1 || rp == 3) && variant == kVRows) || (rp == 2 && variant == kVRowsShort);
 }
 
 // LIGHT: every volume of the plan satisfies batch_light; the kernel then holds
@@ -227,7 +235,8 @@ __global__ __launch_bounds__(kFwdThreads) __attribute__((amdgpu_waves_per_eu(LIG
         const BatchVol v = vols[tl.vol];
         if constexpr (LIGHT) {
             if (v.rp == 1) run_rows_rp<1, true>(v, tl, lds);
-            else run_rows_rp<2, true>(v, tl, lds);
+            else if (v.rp == 2) run_rows_rp<2, true>(v, tl, lds);
+            else run_rows_rp<3, true>(v, tl, lds);
         } else {
             switch (v.rp) {
             case 0: run_tile_rp<0>(v, tl, lds); break;

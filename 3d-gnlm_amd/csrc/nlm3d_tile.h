@@ -571,8 +571,8 @@ __device__ __forceinline__ void fwd3_slab2_tile(const Fwd3Tile &a, float *lds, f
 
 // Tile variants.
 // kVRows / kVRowsShort / kVRowsBig are the padded-copy variants of
-// nlm3d_rows.h (short: 16 x 12 x 12 tiles, r_p = 2 only; big: 16 x
-// (32 - 2 r_p)^2 tiles on 1024 threads, r_p 4-8, per-volume launches only).
+// nlm3d_rows.h (rows: r_p 1-3; short: 16 x 12 x 12 tiles, r_p = 2 only; big:
+// 16 x (32 - 2 r_p)^2 tiles on 1024 threads, r_p 4-8, per-volume launches only).
 enum Fwd3Variant {
     kV16x16x16 = 0,
     kV16x16x8 = 1,
@@ -592,12 +592,12 @@ __host__ __device__ constexpr bool fwd3_is_slab(int v) { return v == kVSlab || v
 __host__ __device__ constexpr int fwd3_tx(int v, int rp) { return (void)v, (void)rp, 16; }
 __host__ __device__ constexpr int fwd3_ty(int v, int rp)
 {
-    return v == kVRowsBig ? 32 - 2 * rp : (fwd3_is_rows(v) ? (rp == 1 ? 14 : 12) : (fwd3_is_slab(v) ? 16 - 2 * rp : 16));
+    return v == kVRowsBig ? 32 - 2 * rp : (fwd3_is_rows(v) ? 16 - 2 * rp : (fwd3_is_slab(v) ? 16 - 2 * rp : 16));
 }
 __host__ __device__ constexpr int fwd3_tz(int v, int rp)
 {
     return v == kVRowsBig ? 32 - 2 * rp : v == kVRowsShort ? 12
-                            : (v == kVRows ? (rp == 1 ? 14 : 28)
+                            : (v == kVRows ? (rp == 1 ? 14 : (rp == 2 ? 28 : 10))
                                            : (fwd3_is_slab(v) ? 32 - 2 * rp : (v == kV16x16x8 ? 8 : 16)));
 }
 

@@ -21,14 +21,14 @@ from nlm_metrics import max_rel, max_rel_fl, report
 pytestmark = pytest.mark.gpu
 
 
-def _volumes(dev, n, seed, dims=(16, 24, 32, 40, 48)):
+def _volumes(dev, n, seed, dims=(16, 24, 32, 40, 48), rps=(1, 2)):
     rng = np.random.default_rng(seed)
     vols = []
     for i in range(n):
         nx, ny, nz = (int(rng.choice(dims)) for _ in range(3))
         sigma = float(rng.uniform(0.02, 0.10))
         u, _ = gnlm.phantom3d(nx, ny, nz, n_ellipsoids=4, axis_lo=3, axis_hi=10, sigma=sigma, seed=seed * 100 + i)
-        rp = int(rng.choice([1, 2]))
+        rp = int(rng.choice(rps))
         rs = int(rng.choice([3, 5]))
         vols.append((torch.from_numpy(u).to(dev), rp, rs, sigma))
     return vols
@@ -126,3 +126,22 @@ def test_batch_slab_and_direct_variants(dev):
             plan.close()
     finally:
         gnlm.reset_tuning()
+
+
+def test_batch_with_patch_radius_3(dev):
+    """r_p = 3 volumes (16 x 10 x 10 rows tiles, the default for NS = 7) in a
+    plan with r_p 1 and 2 volumes: bit-identical to per-volume launches."""
+    vols = _volumes(dev, 9, 1, rps=(1, 2, 3))
+    assert {rp for _, rp, _, _ in vols} == {1, 2, 3}
+    for _, rp, rs, _ in vols:
+        if rp == 3:
+            assert gnlm.tuned_variant(rp, rs) == gnlm.VARIANT_ROWS
+    outs = [torch.empty_like(u) for u, *_ in vols]
+    plan = gnlm.BatchPlan([(u, o, rp, rs, h) for (u, rp, rs, h), o in zip(vols, outs)])
+    refs = [gnlm.nlm3d(u, rp, rs, h) for u, rp, rs, h in vols]
+    for rep in range(2):
+        plan.run()
+        torch.cuda.synchronize()
+        for o, r in zip(outs, refs):
+            assert torch.equal(o, r), rep
+    plan.close()
