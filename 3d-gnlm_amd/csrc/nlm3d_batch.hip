@@ -103,7 +103,8 @@ __device__ __forceinline__ void run_rows_rp(const BatchVol &v, const BatchTile &
         case 4: tile_rows<1, 4, false, LIGHT>(a, lds); break;
         case 5: tile_rows<1, 5, false, LIGHT>(a, lds); break;
         case 6: tile_rows<1, 6, false, LIGHT>(a, lds); break;
-        default: tile_rows<1, 7, false, LIGHT>(a, lds); break;
+        case 7: tile_rows<1, 7, false, LIGHT>(a, lds); break;
+        default: tile_rows<1, 8, false, LIGHT>(a, lds); break;
         }
     } else if constexpr (RP == 2) {
         if (LIGHT || v.variant == kVRowsShort) {
@@ -113,7 +114,8 @@ __device__ __forceinline__ void run_rows_rp(const BatchVol &v, const BatchTile &
             case 3: tile_rows<2, 3, true, LIGHT>(a, lds); break;
             case 4: tile_rows<2, 4, true, LIGHT>(a, lds); break;
             case 5: tile_rows<2, 5, true, LIGHT>(a, lds); break;
-            default: tile_rows<2, 6, true, LIGHT>(a, lds); break;
+            case 6: tile_rows<2, 6, true, LIGHT>(a, lds); break;
+            default: tile_rows<2, 7, true, LIGHT>(a, lds); break;
             }
             return;
         }
@@ -124,7 +126,8 @@ __device__ __forceinline__ void run_rows_rp(const BatchVol &v, const BatchTile &
             case 3: tile_rows<2, 3, false, false>(a, lds); break;
             case 4: tile_rows<2, 4, false, false>(a, lds); break;
             case 5: tile_rows<2, 5, false, false>(a, lds); break;
-            default: tile_rows<2, 6, false, false>(a, lds); break;
+            case 6: tile_rows<2, 6, false, false>(a, lds); break;
+            default: tile_rows<2, 7, false, false>(a, lds); break;
             }
         }
     } else if constexpr (RP == 3) {
@@ -133,7 +136,9 @@ __device__ __forceinline__ void run_rows_rp(const BatchVol &v, const BatchTile &
         case 2: tile_rows<3, 2, false, LIGHT>(a, lds); break;
         case 3: tile_rows<3, 3, false, LIGHT>(a, lds); break;
         case 4: tile_rows<3, 4, false, LIGHT>(a, lds); break;
-        default: tile_rows<3, 5, false, LIGHT>(a, lds); break;
+        case 5: tile_rows<3, 5, false, LIGHT>(a, lds); break;
+        case 6: tile_rows<3, 6, false, LIGHT>(a, lds); break;
+        default: tile_rows<3, 7, false, LIGHT>(a, lds); break;
         }
     }
 }

@@ -24,14 +24,15 @@ hipError_t rows_launch_t(const InDesc &in, float *out, float *Zout, float k, hip
     return rows_launch_cfg<RowsCfg<RP, RS, SHORT>>(in, out, Zout, k, s, nlm3d_fwd_rows<RP, RS, SHORT>);
 }
 
-// Instantiated radii: r_p in {1, 2, 3}; r_s up to 7 (r_p = 1), 6 (r_p = 2)
-// or 5 (r_p = 3), so that the loaded rows stay at <= 32 samples; short tiles
-// for r_p = 2.
+// Instantiated radii: r_p in {1, 2, 3} with r_s up to 8 (r_p = 1) or 7 (r_p 2
+// and 3): the loaded rows hold 16 + 2 LP <= 40 samples (LP = 4 ceil((r_p +
+// r_s) / 4)); short tiles for r_p = 2.
 #define GNLM_ROWS_CASES(X)                                                                                       \
     X(1, 1, false) X(1, 2, false) X(1, 3, false) X(1, 4, false) X(1, 5, false) X(1, 6, false) X(1, 7, false)     \
-    X(2, 1, false) X(2, 2, false) X(2, 3, false) X(2, 4, false) X(2, 5, false) X(2, 6, false) X(2, 1, true)      \
-    X(2, 2, true) X(2, 3, true) X(2, 4, true) X(2, 5, true) X(2, 6, true) X(3, 1, false) X(3, 2, false)          \
-    X(3, 3, false) X(3, 4, false) X(3, 5, false)
+    X(1, 8, false) X(2, 1, false) X(2, 2, false) X(2, 3, false) X(2, 4, false) X(2, 5, false) X(2, 6, false)     \
+    X(2, 7, false) X(2, 1, true) X(2, 2, true) X(2, 3, true) X(2, 4, true) X(2, 5, true) X(2, 6, true)           \
+    X(2, 7, true) X(3, 1, false) X(3, 2, false) X(3, 3, false) X(3, 4, false) X(3, 5, false) X(3, 6, false)      \
+    X(3, 7, false)
 
 }  // namespace
 

@@ -128,15 +128,19 @@ ROWS_CASES = [
     ((28, 24, 32), 2, 5, 0.1),    # whole 16 x 12 x 28 tiles
     ((37, 29, 33), 2, 5, 0.2),    # partial tiles on every axis
     ((17, 19, 23), 2, 3, 0.25),
-    ((12, 12, 12), 2, 6, 0.3),    # largest instantiated r_s for r_p = 2
+    ((12, 12, 12), 2, 6, 0.3),
+    ((13, 14, 15), 2, 7, 0.3),    # largest instantiated r_s for r_p = 2 (40-sample rows)
     ((30, 14, 16), 1, 3, 0.3),
-    ((20, 20, 20), 1, 7, 0.3),    # largest instantiated r_s for r_p = 1
+    ((20, 20, 20), 1, 7, 0.3),
+    ((18, 17, 21), 1, 8, 0.3),    # largest instantiated r_s for r_p = 1
     ((16, 16, 16), 1, 1, 0.3),
     ((9, 8, 7), 1, 4, 0.3),       # smaller than one tile, reflection folds
     ((5, 6, 7), 1, 3, 0.3),       # dims smaller than the halo
     ((1, 20, 20), 1, 2, 0.3),     # single plane
     ((37, 29, 33), 3, 4, 0.2),    # r_p = 3: 16 x 10 x 10 tiles, partial on every axis
-    ((20, 20, 20), 3, 5, 0.3),    # largest instantiated r_s for r_p = 3
+    ((20, 20, 20), 3, 5, 0.3),
+    ((17, 15, 13), 3, 6, 0.3),
+    ((14, 15, 13), 3, 7, 0.3),    # largest instantiated r_s for r_p = 3
     ((9, 8, 7), 3, 1, 0.3),       # smaller than one tile
 ]
 # short 16 x 12 x 12 tiles (r_p = 2 only)
@@ -145,6 +149,7 @@ ROWS_SHORT_CASES = [
     ((37, 29, 33), 2, 5, 0.2),    # partial tiles on every axis
     ((17, 19, 23), 2, 1, 0.25),
     ((12, 12, 12), 2, 6, 0.3),
+    ((19, 14, 17), 2, 7, 0.3),    # largest instantiated r_s (40-sample rows)
     ((9, 8, 7), 2, 3, 0.3),       # smaller than one tile
 ]
 
