@@ -27,7 +27,7 @@ hipError_t fwd3d_tiled_launch(const InDesc &in, float *out, float *Zout, int r_p
                               hipStream_t s);
 
 // ---- "rows" variant (nlm3d_rows.hip) ---------------------------------------
-// r_p in {1, 2, 3} with r_s up to 7 / 6 / 5; pads the input into a stream-ordered
+// r_p in {1, 2, 3, 4} with r_s up to 8 / 7 / 7 / 7; pads the input into a stream-ordered
 // workspace first (caller-padded input needs pad >= r_p + r_s).
 // Large patches (r_p 4-8): 16 x (32 - 2 r_p)^2 tiles on 1024 threads.
 bool fwd3d_rows_big_supported(int r_p, int r_s);

@@ -126,6 +126,7 @@ size_t lds_rp(int rs, int variant)
 size_t fwd3d_lds_bytes(int r_p, int r_s, int variant)
 {
     if (variant == kVRowsBig) return fwd3d_rows_big_lds_bytes(r_p, r_s);
+    if (variant == kVRows && r_p == 4) return fwd3d_rows_lds_bytes(4, r_s, false);
     switch (r_p) {
     case 0: return lds_rp<0>(r_s, variant);
     case 1: return lds_rp<1>(r_s, variant);
@@ -138,6 +139,8 @@ size_t fwd3d_lds_bytes(int r_p, int r_s, int variant)
 bool fwd3d_variant_supported(int r_p, int r_s, int variant)
 {
     if (variant == kVRowsBig) return fwd3d_rows_big_supported(r_p, r_s);
+    // r_p = 4 has rows tiles (16 x 8 x 8) besides the big ones
+    if (variant == kVRows && r_p == 4) return fwd3d_rows_supported(4, r_s, false);
     if (r_s < 0 || r_p < 0 || r_p > 3 || variant < 0 || variant >= kNumFwd3Variants) return false;
     if (fwd3_is_rows(variant)) return fwd3d_rows_supported(r_p, r_s, variant == kVRowsShort);
     return fwd3d_lds_bytes(r_p, r_s, variant) <= kLdsLimitBytes;
@@ -155,6 +158,7 @@ hipError_t fwd3d_tiled_launch_variant(const InDesc &in, float *out, float *Zout,
 {
     if (!fwd3d_variant_supported(r_p, r_s, variant)) return hipErrorInvalidValue;
     if (variant == kVRowsBig) return fwd3d_rows_big_launch(in, out, Zout, r_p, r_s, k, s);
+    if (variant == kVRows && r_p == 4) return fwd3d_rows_launch(in, out, Zout, 4, r_s, k, false, s);
     switch (r_p) {
     case 0: return launch_rp<0>(in, out, Zout, r_s, k, variant, s);
     case 1: return launch_rp<1>(in, out, Zout, r_s, k, variant, s);

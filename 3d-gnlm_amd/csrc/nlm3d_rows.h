@@ -66,7 +66,8 @@ struct PadVol {
 // Tiles: 16 x (16 - 2 r_p) x TZ, so that the grown rows of one z_ext plane
 // fill 16 lanes: r_p = 1: 16 x 14 x 14; r_p = 2: 16 x 12 x 28 (two rows per
 // thread) or, SHORT, 16 x 12 x 12; r_p = 3: 16 x 10 x 10 (phase-3 items of x
-// pairs with 5-plane runs, 160 items).
+// pairs with 5-plane runs, 160 items); r_p = 4: 16 x 8 x 8 (single columns,
+// 4-plane runs).
 // SHORT (r_p = 2 only): one phase-1 row per thread and 3-plane phase-3 runs
 // -- a larger halo share than the 28-plane tiles, but half the live
 // registers: the per-volume kernel runs four waves per SIMD (128 VGPRs; its
@@ -82,16 +83,16 @@ struct RowsCfg {
     static constexpr int LP = 4 * ((R + 3) / 4);         // halo of the copy in x (aligned)
     static constexpr int TX = 16;
     static constexpr int NS = TX + 2 * LP;               // samples per loaded row
-    static_assert(RP >= 1 && RP <= 3, "rows tiles are for r_p 1-3");
+    static_assert(RP >= 1 && RP <= 4, "rows tiles are for r_p 1-4");
     static constexpr int TY = 16 - 2 * RP;
-    static constexpr int TZ = RP == 1 ? 14 : (RP == 3 ? 10 : (SHORT ? 12 : 28));
+    static constexpr int TZ = RP == 1 ? 14 : (RP == 3 ? 10 : (RP == 4 ? 8 : (SHORT ? 12 : 28)));
     static constexpr int EX = TX + 2 * RP, EY = TY + 2 * RP, EZ = TZ + 2 * RP;
     static constexpr int NROWS = EY * EZ;
     static constexpr int ROWS = NROWS / kRowsThreads;    // phase-1 rows per thread
     static_assert(EY == 16 && NROWS % kRowsThreads == 0, "16-lane groups hold one z_ext plane of rows");
     // phase 2 items: (x pair, z_ext, run of RY y), P2R per thread
     static constexpr int VX = 2;
-    static constexpr int RY = RP == 1 ? 7 : (RP == 2 ? 6 : 5);
+    static constexpr int RY = RP == 1 ? 7 : (RP == 2 ? 6 : (RP == 3 ? 5 : 4));
     static constexpr int NXV = TX / VX;
     static constexpr int P2N = NXV * EZ * (TY / RY);
     static constexpr int P2R = P2N / kRowsThreads;
@@ -99,8 +100,8 @@ struct RowsCfg {
     // phase 3 items: (group of V3 x columns, y, run of RZ z); the short tiles
     // read their columns four at a time (128-bit LDS reads: the phase's LDS
     // time halves, and 192 items leave a fourth wave free)
-    static constexpr int RZ = SHORT ? 3 : (RP == 3 ? 5 : 7);
-    static constexpr int V3 = SHORT ? 4 : (RP == 2 ? 1 : 2);
+    static constexpr int RZ = SHORT ? 3 : (RP == 3 ? 5 : (RP == 4 ? 4 : 7));
+    static constexpr int V3 = SHORT ? 4 : ((RP == 2 || RP == 4) ? 1 : 2);
     static constexpr int P3N = (TX / V3) * TY * (TZ / RZ);
     static constexpr int P3R = (P3N + kRowsThreads - 1) / kRowsThreads;
     static_assert(TZ % RZ == 0, "whole z runs");

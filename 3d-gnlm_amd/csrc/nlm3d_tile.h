@@ -571,7 +571,7 @@ __device__ __forceinline__ void fwd3_slab2_tile(const Fwd3Tile &a, float *lds, f
 
 // Tile variants.
 // kVRows / kVRowsShort / kVRowsBig are the padded-copy variants of
-// nlm3d_rows.h (rows: r_p 1-3; short: 16 x 12 x 12 tiles, r_p = 2 only; big:
+// nlm3d_rows.h (rows: r_p 1-4; short: 16 x 12 x 12 tiles, r_p = 2 only; big:
 // 16 x (32 - 2 r_p)^2 tiles on 1024 threads, r_p 4-8, per-volume launches only).
 enum Fwd3Variant {
     kV16x16x16 = 0,
@@ -597,7 +597,7 @@ __host__ __device__ constexpr int fwd3_ty(int v, int rp)
 __host__ __device__ constexpr int fwd3_tz(int v, int rp)
 {
     return v == kVRowsBig ? 32 - 2 * rp : v == kVRowsShort ? 12
-                            : (v == kVRows ? (rp == 1 ? 14 : (rp == 2 ? 28 : 10))
+                            : (v == kVRows ? (rp == 1 ? 14 : (rp == 2 ? 28 : (rp == 3 ? 10 : 8)))
                                            : (fwd3_is_slab(v) ? 32 - 2 * rp : (v == kV16x16x8 ? 8 : 16)));
 }
 

@@ -159,7 +159,7 @@ void gnlm_reset_tuning(void);                        /* forget all choices  */
  * 2 = "slab" (16 x (16-2r_p) x (32-2r_p), DPP y sums, direct x/z sums,
  * pre-scaled input; agrees with the others to FP32 rounding), 3 = "slab2"
  * (variant 2 with 64/128-bit LDS accesses; bit-identical to 2), 4 = "rows"
- * (r_p 1 with r_s <= 8, r_p 2 and 3 with r_s <= 7: a padded aligned copy of the
+ * (r_p 1 with r_s <= 8, r_p 2-4 with r_s <= 7: a padded aligned copy of the
  * input in a stream-ordered workspace, per-row register reuse across dx,
  * pairwise-tree box sums; agrees with the others to FP32 rounding),
  * 5 = "rows, short tiles" (variant 4 on 16 x 12 x 12 tiles, r_p 2 only:

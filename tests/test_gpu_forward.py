@@ -142,6 +142,8 @@ ROWS_CASES = [
     ((17, 15, 13), 3, 6, 0.3),
     ((14, 15, 13), 3, 7, 0.3),    # largest instantiated r_s for r_p = 3
     ((9, 8, 7), 3, 1, 0.3),       # smaller than one tile
+    ((20, 21, 22), 4, 3, 0.3),    # r_p = 4: 16 x 8 x 8 tiles
+    ((9, 8, 7), 4, 2, 0.3),
 ]
 # short 16 x 12 x 12 tiles (r_p = 2 only)
 ROWS_SHORT_CASES = [
@@ -223,6 +225,24 @@ def test_3d_rows_big_padded_equals_reflect(dev):
     finally:
         gnlm.reset_tuning()
     assert np.array_equal(a, b), report(b, a)
+
+
+@pytest.mark.parametrize("rs", [2, 5, 7])
+def test_3d_rp4_rows_equal_big_tiles(dev, rs):
+    """r_p = 4 runs on rows tiles (16 x 8 x 8, the default) or big tiles
+    (16 x 24 x 24).  Their patch-sum trees pair outputs the same way (every
+    run starts at an even index), so the two agree bit for bit."""
+    rng = np.random.default_rng(40 + rs)
+    u = rng.random((27, 30, 35), dtype=np.float32)
+    outs = []
+    try:
+        for v in (gnlm.VARIANT_ROWS, gnlm.VARIANT_ROWS_BIG):
+            gnlm.set_variant(4, rs, v)
+            outs.append(_run3(dev, u, 4, rs, 0.3, gnlm.PATH_TILED))
+    finally:
+        gnlm.reset_tuning()
+    assert gnlm.tuned_variant(4, rs) == gnlm.VARIANT_ROWS
+    assert np.array_equal(outs[0], outs[1]), report(outs[0], outs[1])
 
 
 def test_3d_large_patch_default_path_is_tiled(dev):
