@@ -99,7 +99,7 @@ int gnlm2d_forward_ex(const float *d_in, int in_pad, long long in_sy, float *d_o
 int gnlm3d_backward(const float *d_in, const float *d_out, const float *d_Z, const float *d_gout, int nx,
                     int ny, int nz, int r_p, int r_s, float h, float *d_gin, float *d_gh, void *stream);
 
-/* Form of the r_p = 1 adjoint (no counterpart in the reference), process-
+/* Form of the rows-organised adjoint (r_p 0, 1 and 2; no counterpart in the reference), process-
  * wide: GNLM_BWD_GATHER visits every offset d and gathers its terms;
  * GNLM_BWD_SYMMETRIC visits half of them and, since S_d(x) = S_-d(x + d),
  * scatters each one's mirror term to x + d through an LDS accumulator (half
