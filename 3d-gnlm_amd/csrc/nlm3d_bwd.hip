@@ -374,6 +374,7 @@ size_t bwd_lds_bytes_rp(int rp, int rs)
     case 0: return bwd_lds_floats<0>(rs) * sizeof(float);
     case 1: return bwd_lds_floats<1>(rs) * sizeof(float);
     case 2: return bwd_lds_floats<2>(rs) * sizeof(float);
+    case 3: return bwd_lds_floats<3>(rs) * sizeof(float);
     default: return (size_t)-1;
     }
 }
@@ -406,7 +407,7 @@ int bwd_blocks(int nx, int ny, int nz, int rs)
 bool bwd3d_supported(int r_p, int r_s)
 {
     return bwd3d_rows_supported(r_p, r_s) ||
-           (r_p >= 0 && r_p <= 2 && r_s >= 0 && bwd_lds_bytes_rp(r_p, r_s) <= kLdsLimitBytes);
+           (r_p >= 0 && r_p <= 3 && r_s >= 0 && bwd_lds_bytes_rp(r_p, r_s) <= kLdsLimitBytes);
 }
 
 hipError_t bwd3d_fold_launch(const float *gext, float *gu, int nx, int ny, int nz, int R, hipStream_t s)
@@ -431,8 +432,10 @@ hipError_t bwd3d_launch(const float *u, const float *y, const float *Z, const fl
     const int R = rp + rs;
     const long long nv = (long long)nx * ny * nz;
     const long long ext = (nx + 2LL * R) * (ny + 2LL * R) * (nz + 2LL * R);
-    const int nblocks = rp == 0 ? bwd_blocks<0>(nx, ny, nz, rs)
-                                : (rp == 1 ? bwd_blocks<1>(nx, ny, nz, rs) : bwd_blocks<2>(nx, ny, nz, rs));
+    const int nblocks = rp == 0   ? bwd_blocks<0>(nx, ny, nz, rs)
+                        : rp == 1 ? bwd_blocks<1>(nx, ny, nz, rs)
+                        : rp == 2 ? bwd_blocks<2>(nx, ny, nz, rs)
+                                  : bwd_blocks<3>(nx, ny, nz, rs);
     BwdArgs a;
     a.u = u;
     a.nx = nx;
@@ -460,8 +463,10 @@ hipError_t bwd3d_launch(const float *u, const float *y, const float *Z, const fl
         e = launch_bwd<0>(a, lds, s);
     else if (rp == 1)
         e = launch_bwd<1>(a, lds, s);
-    else
+    else if (rp == 2)
         e = launch_bwd<2>(a, lds, s);
+    else
+        e = launch_bwd<3>(a, lds, s);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(fold_grad, dim3((unsigned)gblk), dim3(256), 0, s, (const float *)a.gext, gu, nx, ny, nz, R);
         if (gh) hipLaunchKernelGGL(reduce_dh, dim3(1), dim3(256), 0, s, (const float *)a.dh_part, nblocks,

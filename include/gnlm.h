@@ -92,10 +92,10 @@ int gnlm2d_forward_ex(const float *d_in, int in_pad, long long in_sy, float *d_o
  * are recomputed; nothing of size voxels x offsets is stored.  Results are
  * deterministic.  Scratch (padded copies, the gradient on the extended
  * domain [-R, n+R)^3 with R = r_p + r_s) comes from a stream-ordered
- * workspace released on `stream`.  Returns GNLM_EUNSUPPORTED for r_p > 2 or
- * radii beyond the instantiated ones (r_s <= 7 for r_p 0, 1, 2; beyond that
- * as the LDS image of the fallback tile allows).  They run the
- * rows-organised kernels in the form gnlm_set_backward_form selects. */
+ * workspace released on `stream`.  Supported: r_p 0, 1, 2 with r_s <= 7 (the
+ * rows-organised kernels, in the form gnlm_set_backward_form selects; larger
+ * r_s as the LDS image of the fallback tile allows) and r_p = 3 with r_s <= 3
+ * (the fallback tile); GNLM_EUNSUPPORTED otherwise. */
 int gnlm3d_backward(const float *d_in, const float *d_out, const float *d_Z, const float *d_gout, int nx,
                     int ny, int nz, int r_p, int r_s, float h, float *d_gin, float *d_gh, void *stream);
 

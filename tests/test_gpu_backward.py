@@ -52,6 +52,8 @@ def form(request):
     ((9, 10, 11), 2, 7, 0.3),   # largest instantiated r_s
     ((14, 13, 6), 2, 1, 0.3),
     ((20, 21, 19), 0, 5, 0.3),  # single-voxel patches, several tiles
+    ((12, 13, 14), 3, 2, 0.3),  # patch radius 3: the LDS-block tile (10 x 10 x 8)
+    ((17, 16, 15), 3, 3, 0.3),  # largest r_s it fits
 ])
 def test_backward_matches_oracle(dev, form, shape, rp, rs, h):
     rng = np.random.default_rng(sum(shape) * 7 + rs)
