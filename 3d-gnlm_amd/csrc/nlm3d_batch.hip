@@ -79,8 +79,8 @@ __device__ __noinline__ void tile_rows(const RowsTile &a, float *lds) { rows_til
 
 // kVRows / kVRowsShort tile: r_s is a template parameter of the tile body (its
 // dx loop is unrolled), so dispatch on it; the cases match nlm3d_rows.hip.
-// LIGHT: only the bodies that fit three waves per SIMD (r_p = 1 and 3 rows,
-// r_p = 2 short rows).
+// LIGHT: only the bodies that fit three waves per SIMD (r_p = 1, 3 and 4
+// rows, r_p = 2 short rows).
 template <int RP, bool LIGHT>
 __device__ __forceinline__ void run_rows_rp(const BatchVol &v, const BatchTile &t, float *lds)
 {
@@ -129,6 +129,16 @@ __device__ __forceinline__ void run_rows_rp(const BatchVol &v, const BatchTile &
             case 6: tile_rows<2, 6, false, false>(a, lds); break;
             default: tile_rows<2, 7, false, false>(a, lds); break;
             }
+        }
+    } else if constexpr (RP == 4) {
+        switch (v.rs) {
+        case 1: tile_rows<4, 1, false, LIGHT>(a, lds); break;
+        case 2: tile_rows<4, 2, false, LIGHT>(a, lds); break;
+        case 3: tile_rows<4, 3, false, LIGHT>(a, lds); break;
+        case 4: tile_rows<4, 4, false, LIGHT>(a, lds); break;
+        case 5: tile_rows<4, 5, false, LIGHT>(a, lds); break;
+        case 6: tile_rows<4, 6, false, LIGHT>(a, lds); break;
+        default: tile_rows<4, 7, false, LIGHT>(a, lds); break;
         }
     } else if constexpr (RP == 3) {
         switch (v.rs) {
@@ -204,7 +214,7 @@ __device__ __forceinline__ void run_tile_rp(const BatchVol &v, const BatchTile &
 // Volume v can run in the LIGHT kernel (three waves per SIMD).
 __host__ __device__ inline bool batch_light(int rp, int variant)
 {
-    return ((rp == 1 || rp == 3) && variant == kVRows) || (rp == 2 && variant == kVRowsShort);
+    return ((rp == 1 || rp >= 3) && variant == kVRows) || (rp == 2 && variant == kVRowsShort);
 }
 
 // LIGHT: every volume of the plan satisfies batch_light; the kernel then holds
@@ -241,13 +251,15 @@ __global__ __launch_bounds__(kFwdThreads) __attribute__((amdgpu_waves_per_eu(LIG
         if constexpr (LIGHT) {
             if (v.rp == 1) run_rows_rp<1, true>(v, tl, lds);
             else if (v.rp == 2) run_rows_rp<2, true>(v, tl, lds);
-            else run_rows_rp<3, true>(v, tl, lds);
+            else if (v.rp == 3) run_rows_rp<3, true>(v, tl, lds);
+            else run_rows_rp<4, true>(v, tl, lds);
         } else {
             switch (v.rp) {
             case 0: run_tile_rp<0>(v, tl, lds); break;
             case 1: run_tile_rp<1>(v, tl, lds); break;
             case 2: run_tile_rp<2>(v, tl, lds); break;
-            default: run_tile_rp<3>(v, tl, lds); break;
+            case 3: run_tile_rp<3>(v, tl, lds); break;
+            default: run_rows_rp<4, false>(v, tl, lds); break;  // r_p = 4: rows tiles only
             }
         }
     }
@@ -337,7 +349,7 @@ int gnlm_batch_plan_create(const gnlm_batch_volume *vols, int n, gnlm_batch_plan
         if (!v.d_in || !v.d_out || v.nx < 1 || v.ny < 1 || v.nz < 1 || v.r_p < 0 || v.r_s < 0 ||
             !(v.h > 0.0f) || !std::isfinite(v.h))
             return GNLM_EINVAL;
-        if (v.r_p > 3 || !fwd3d_tiled_supported(v.r_p, v.r_s)) return GNLM_EUNSUPPORTED;
+        if (v.r_p > 4 || !fwd3d_tiled_supported(v.r_p, v.r_s)) return GNLM_EUNSUPPORTED;
         BatchVol &b = hv[(size_t)i];
         b.in = InDesc{v.d_in, v.nx, v.ny, v.nz, 0, 0, 0};
         b.out = v.d_out;
@@ -348,6 +360,7 @@ int gnlm_batch_plan_create(const gnlm_batch_volume *vols, int n, gnlm_batch_plan
         b.k = (float)weight_scale(P, v.h);
         b.sigma = std::sqrt(-b.k);
         b.variant = fwd3d_pick_variant(v.r_p, v.r_s);
+        if (v.r_p == 4 && b.variant != kVRows) return GNLM_EUNSUPPORTED;  // big tiles are per-volume only
         const int tx = fwd3_tx(b.variant, v.r_p), ty = fwd3_ty(b.variant, v.r_p), tz = fwd3_tz(b.variant, v.r_p);
         b.ntx = (v.nx + tx - 1) / tx;
         b.nty = (v.ny + ty - 1) / ty;

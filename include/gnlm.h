@@ -117,8 +117,9 @@ int gnlm_set_backward_form(int form);
  * HIP graph; workgroups pull tiles from a device work queue (longest first).
  * Outputs are bit-identical to per-volume gnlm3d_forward calls.
  * The plan keeps the descriptors (device pointers included): buffers must
- * stay allocated while the plan is used.  r_p <= 3 and the tiled path's
- * radius limits apply (GNLM_EUNSUPPORTED otherwise). */
+ * stay allocated while the plan is used.  r_p <= 4 (r_p = 4 on the rows
+ * tiles only) and the tiled path's radius limits apply (GNLM_EUNSUPPORTED
+ * otherwise). */
 typedef struct gnlm_batch_volume {
     const float *d_in;  /* unpadded input, x fastest        */
     float *d_out;       /* output, same layout              */
@@ -166,7 +167,7 @@ void gnlm_reset_tuning(void);                        /* forget all choices  */
  * fewer registers per thread, more workgroups per CU), 6 = "rows, big tiles"
  * (r_p 4-8 with r_s <= 7, the large patches of the reference's default
  * NS = 17: 16 x (32 - 2 r_p)^2 tiles on 1024 threads; per-volume launches
- * only, as batch plans take r_p <= 3).
+ * only; batch plans run r_p = 4 on variant 4).
  * Untuned default: 5, else 4, else 6 where they are instantiated, else 3
  * where it fits, else 2, else the first direct variant. */
 

@@ -145,3 +145,18 @@ def test_batch_with_patch_radius_3(dev):
         for o, r in zip(outs, refs):
             assert torch.equal(o, r), rep
     plan.close()
+
+
+def test_batch_with_patch_radius_4(dev):
+    """r_p = 4 volumes run on the 16 x 8 x 8 rows tiles inside batch plans:
+    bit-identical to per-volume launches."""
+    vols = _volumes(dev, 5, 3, dims=(16, 24, 40), rps=(2, 4))
+    assert {rp for _, rp, _, _ in vols} == {2, 4}
+    outs = [torch.empty_like(u) for u, *_ in vols]
+    plan = gnlm.BatchPlan([(u, o, rp, rs, h) for (u, rp, rs, h), o in zip(vols, outs)])
+    refs = [gnlm.nlm3d(u, rp, rs, h) for u, rp, rs, h in vols]
+    plan.run()
+    torch.cuda.synchronize()
+    for o, r in zip(outs, refs):
+        assert torch.equal(o, r)
+    plan.close()
