@@ -161,7 +161,7 @@ BWD_AUTO, BWD_GATHER, BWD_SYMMETRIC = 0, 1, 2
 
 
 def set_backward_form(form):
-    """Process-wide form of the r_p = 1 adjoint (gnlm_set_backward_form)."""
+    """Process-wide form of the rows-organised adjoint, r_p 0-2 (gnlm_set_backward_form)."""
     check(lib().gnlm_set_backward_form(int(form)), "gnlm_set_backward_form")
 
 

@@ -1,6 +1,7 @@
 """GPU adjoint: dL/d(input) and dL/dh.
 
-Both forms of the r_p = 1 adjoint (gather, symmetric) are run.
+Both forms of the rows-organised adjoint (gather, symmetric; r_p 0-2) are
+run; r_p = 3 runs on the LDS-block tile in either setting.
 Tolerances:
   * vs the oracle's FP64 chain-rule adjoint on the same input and dL/dy:
     max |gpu - ref| <= 1e-4 * max |ref| for dL/du, and relative 1e-4 for dL/dh.
