@@ -64,7 +64,7 @@ hipError_t bwd3d_rows_launch(const float *u, const float *y, const float *Z, con
                              int r_p, int r_s, float h, float *gu, float *gh, hipStream_t s);
 // shared tail kernels: fold the extended-domain gradient onto the volume
 // through the reflect-101 preimages; fixed-order sum of dL/dh partials x scale
-// Form of the r_p = 1 adjoint: gather over all offsets, or the symmetric
+// Form of the rows adjoint (r_p 0-2): gather over all offsets, or the symmetric
 // form (half of the offsets, each also scattered to x + d); auto is the
 // symmetric form, the faster one at every instantiated r_s on MI355X.
 enum { kBwdAuto = 0, kBwdGather = 1, kBwdSymmetric = 2 };

@@ -29,7 +29,7 @@ def _gpu_grad(dev, u, g, rp, rs, h):
     return gu.cpu().numpy().astype(np.float64), float(gh.item())
 
 
-# the two forms of the r_p = 1 adjoint (gather over every offset; half of the
+# the two forms of the rows adjoint, r_p 0-2 (gather over every offset; half of the
 # offsets with the mirror terms scattered), forced for the whole test
 @pytest.fixture(params=[gnlm.BWD_GATHER, gnlm.BWD_SYMMETRIC], ids=["gather", "symmetric"])
 def form(request):
