@@ -1,5 +1,5 @@
-// nlm3d_bwd_rows.hip -- adjoint of the 3D NLM for patch radius 1, organised
-// like the rows forward (nlm3d_rows.h).
+// nlm3d_bwd_rows.hip -- adjoint of the 3D NLM for patch radii 0, 1 and 2,
+// organised like the rows forward (nlm3d_rows.h).
 //
 // Same gather form as nlm3d_bwd.hip (3DNLM/main.cu:52-68,107-111,125 is the
 // forward being differentiated):
@@ -11,8 +11,9 @@
 // reflect-101 preimages afterwards.
 //
 // Organisation (one workgroup of 256 threads per gradient tile T of
-// 14 x 14 x 10 positions; Q = T grown by r_p = 16 x 16 x 12 is where S, w
-// and E live; D = Q grown by r_p holds the squared differences):
+// (16 - 2 r_p) x (16 - 2 r_p) x (12 - 2 r_p) positions, 14 x 14 x 10 for
+// r_p = 1; Q = T grown by r_p = 16 x 16 x 12 is where S, w and E live; D = Q
+// grown by r_p holds the squared differences):
 //   * a pre-pass writes a padded copy of u~ and a padded, interleaved
 //     (G, GY, u~, 0) array ((G, GY) zero outside the volume); the tile reads
 //     both with buffer loads whose per-lane offsets are fixed for the tile
