@@ -9,7 +9,10 @@ Tolerances (FP32), stated per comparison:
     path; against FP64 <= 1e-5 for patch 3^3 and <= 1e-4 for patch 5^3 at
     256^3, where the FP32 patch sums themselves (CPU and GPU alike) are
     ~3.5e-5 from FP64.
-  * tiled vs generic path, padded vs reflect input: bit-identical.
+  * 3D direct-sum tiles vs the generic path, padded vs reflect input, r_p = 4
+    rows vs big tiles: bit-identical; the 2D tile vs the generic path:
+    bit-identical on integer-valued input, within 2e-6 otherwise (tree vs
+    left-to-right patch sums).
 """
 import numpy as np
 import pytest
