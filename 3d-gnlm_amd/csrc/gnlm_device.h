@@ -24,6 +24,7 @@ struct InDesc {
 
 __device__ __forceinline__ int reflect101(int i, int n)
 {
+    if ((unsigned)i < (unsigned)n) return i;  // inside: no division
     if (n <= 1) return 0;
     const int period = 2 * (n - 1);
     int m = i % period;

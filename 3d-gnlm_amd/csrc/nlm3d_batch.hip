@@ -154,23 +154,24 @@ __device__ __forceinline__ void run_rows_rp(const BatchVol &v, const BatchTile &
 }
 
 // Padded copies of the rows-variant volumes: blockIdx.y picks the volume.
+// Grid z walks the planes and grid x the samples of a plane, so the index
+// arithmetic is 32-bit (each copy is below 2^30 floats).
 __global__ void batch_pad(const BatchVol *__restrict__ vols, const int *__restrict__ ids)
 {
     const BatchVol &v = vols[ids[blockIdx.y]];
     float *dst = const_cast<float *>(v.pv.p);
-    const long long n = (long long)v.px * v.py * v.pz;
-    const int r = v.halo;
-    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
-         i += (long long)gridDim.x * blockDim.x) {
-        const int ix = (int)(i % v.px);
-        const long long q = i / v.px;
-        const int x = ix - v.lp;
-        const int y = (int)(q % v.py) - r;
-        const int z = (int)(q / v.py) - r;
-        float val = 0.0f;
-        if (x >= -r && x < v.in.nx + r && y >= -r && y < v.in.ny + r && z >= -r && z < v.in.nz + r)
-            val = load_in(v.in, x, y, z);
-        dst[i] = val;
+    const int r = v.halo, nxy = v.px * v.py;
+    for (int zp = blockIdx.z; zp < v.pz; zp += gridDim.z) {
+        const int z = zp - r;
+        float *plane = dst + (long long)zp * nxy;
+        const bool zin = z >= -r && z < v.in.nz + r;
+        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nxy; i += gridDim.x * blockDim.x) {
+            const int x = i % v.px - v.lp;
+            const int y = i / v.px - r;
+            float val = 0.0f;
+            if (zin && x >= -r && x < v.in.nx + r && y >= -r && y < v.in.ny + r) val = load_in(v.in, x, y, z);
+            plane[i] = val;
+        }
     }
 }
 
@@ -278,7 +279,8 @@ struct gnlm_batch_plan {
     int *d_counter = nullptr;  // kNumXcd queue counters, then kNumXcd + 1 queue starts
     int npad = 0;              // kVRows / kVRowsShort volumes
     int *d_padids = nullptr;   // their indices
-    long long padmax = 0;      // largest padded copy (floats)
+    int padmax_xy = 0;         // largest plane of a padded copy (floats)
+    int padmax_z = 0;          // most planes of a padded copy
     float *d_ws = nullptr;     // padded copies + Z accumulators
     bool light = false;        // every volume fits the three-waves-per-SIMD kernel
     hipGraph_t graph = nullptr;
@@ -311,9 +313,8 @@ int st_of(hipError_t e)
 hipError_t enqueue_batch(const gnlm_batch_plan *p, hipStream_t s)
 {
     if (p->npad > 0) {
-        long long g = (p->padmax + 255) / 256;
-        if (g > 2048) g = 2048;
-        hipLaunchKernelGGL(batch_pad, dim3((unsigned)g, (unsigned)p->npad), dim3(256), 0, s,
+        const int gx = std::min((p->padmax_xy + 255) / 256, 256), gz = std::min(p->padmax_z, 32);
+        hipLaunchKernelGGL(batch_pad, dim3((unsigned)gx, (unsigned)p->npad, (unsigned)gz), dim3(256), 0, s,
                            (const BatchVol *)p->d_vols, (const int *)p->d_padids);
     }
     hipLaunchKernelGGL(reset_counters, dim3(1), dim3(64), 0, s, p->d_counter);
@@ -342,7 +343,8 @@ int gnlm_batch_plan_create(const gnlm_batch_volume *vols, int n, gnlm_batch_plan
     std::vector<double> cost;                 // per-tile cost of each volume
     std::vector<int> padids;
     std::vector<long long> wsoff((size_t)n, -1), zoff((size_t)n, -1);
-    long long wsn = 0, padmax = 0;
+    long long wsn = 0;
+    int padmax_xy = 0, padmax_z = 0;
     size_t lds = 0;
     for (int i = 0; i < n; ++i) {
         const gnlm_batch_volume &v = vols[i];
@@ -388,7 +390,8 @@ int gnlm_batch_plan_create(const gnlm_batch_volume *vols, int n, gnlm_batch_plan
                 zoff[(size_t)i] = wsn;
                 wsn += (nv + 63) / 64 * 64;
             }
-            padmax = std::max(padmax, np);
+            padmax_xy = std::max(padmax_xy, b.px * b.py);
+            padmax_z = std::max(padmax_z, b.pz);
             padids.push_back(i);
         }
         cost.push_back((double)tx * ty * tz * std::pow(2.0 * v.r_s + 1.0, 3));
@@ -430,7 +433,8 @@ int gnlm_batch_plan_create(const gnlm_batch_volume *vols, int n, gnlm_batch_plan
     if (e == hipSuccess && occ < 1) e = hipErrorInvalidConfiguration;
     if (e == hipSuccess) p->nblocks = std::min(ncu * occ, p->ntiles);
     p->npad = (int)padids.size();
-    p->padmax = padmax;
+    p->padmax_xy = padmax_xy;
+    p->padmax_z = padmax_z;
     if (e == hipSuccess && wsn > 0) e = hipMalloc((void **)&p->d_ws, sizeof(float) * (size_t)wsn);
     if (e == hipSuccess && p->npad > 0) e = hipMalloc((void **)&p->d_padids, sizeof(int) * padids.size());
     if (e == hipSuccess && p->npad > 0)

@@ -44,19 +44,22 @@ __device__ __forceinline__ void rows_kernel_body(const RowsArgs &a, float *lds)
 
 // Fill the padded copy: reflect-101 samples on [-R, n + R) in every axis
 // (or the caller's own halo when the input arrives padded), 0 beyond.
+// Grid rows walk the planes, so the index arithmetic within a plane is
+// 32-bit (the copy is below 2^30 floats).
 __global__ void rows_pad(InDesc in, float *dst, int px, int py, int pz, int lp, int r)
 {
-    const long long n = (long long)px * py * pz;
-    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
-         i += (long long)gridDim.x * blockDim.x) {
-        const int ix = (int)(i % px);
-        const long long q = i / px;
-        const int x = ix - lp;
-        const int y = (int)(q % py) - r;
-        const int z = (int)(q / py) - r;
-        float v = 0.0f;
-        if (x >= -r && x < in.nx + r && y >= -r && y < in.ny + r && z >= -r && z < in.nz + r) v = load_in(in, x, y, z);
-        dst[i] = v;
+    const int nxy = px * py;
+    for (int zp = blockIdx.y; zp < pz; zp += gridDim.y) {
+        const int z = zp - r;
+        float *plane = dst + (long long)zp * nxy;
+        const bool zin = z >= -r && z < in.nz + r;
+        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nxy; i += gridDim.x * blockDim.x) {
+            const int x = i % px - lp;
+            const int y = i / px - r;
+            float v = 0.0f;
+            if (zin && x >= -r && x < in.nx + r && y >= -r && y < in.ny + r) v = load_in(in, x, y, z);
+            plane[i] = v;
+        }
     }
 }
 
@@ -95,9 +98,9 @@ hipError_t rows_launch_cfg(const InDesc &in, float *out, float *Zout, float k, h
     a.v.pxy = px * py;
     a.v.bytes = (unsigned)(sizeof(float) * npad);
     a.Zacc = Zout ? Zout : ws + npad;
-    long long g = (npad + 255) / 256;
-    if (g > 65536) g = 65536;
-    hipLaunchKernelGGL(rows_pad, dim3((unsigned)g), dim3(256), 0, s, in, ws, px, py, pz, C::LP, C::R);
+    const int gx = (px * py + 255) / 256 < 1024 ? (px * py + 255) / 256 : 1024;
+    const int gy = pz < 65535 ? pz : 65535;
+    hipLaunchKernelGGL(rows_pad, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, s, in, ws, px, py, pz, C::LP, C::R);
     e = hipGetLastError();
     if (e == hipSuccess) {
         const size_t lds = rows_lds_cfg<C>();
