@@ -140,6 +140,7 @@ ROWS_CASES = [
     ((9, 8, 7), 1, 4, 0.3),       # smaller than one tile, reflection folds
     ((5, 6, 7), 1, 3, 0.3),       # dims smaller than the halo
     ((1, 20, 20), 1, 2, 0.3),     # single plane
+    ((70000, 1, 1), 1, 1, 0.3),   # more padded planes than one grid dimension holds
     ((37, 29, 33), 3, 4, 0.2),    # r_p = 3: 16 x 10 x 10 tiles, partial on every axis
     ((20, 20, 20), 3, 5, 0.3),
     ((17, 15, 13), 3, 6, 0.3),
