@@ -353,6 +353,57 @@ __global__ void fold_grad(const float *gext, float *gu, int nx, int ny, int nz, 
     }
 }
 
+// Preimages of i when R < n - 1 (one reflection at each end): i itself, -i for
+// 1 <= i <= R, then 2(n-1) - i for n-1-R <= i <= n-2.  This is the order in
+// which preimages() lists them, so both folds add the same terms in the same
+// order.
+__device__ inline int preimages_near(int i, int n, int R, int *out)
+{
+    const bool neg = i >= 1 && i <= R, mir = i <= n - 2 && i >= n - 1 - R;
+    out[0] = i;
+    out[1] = neg ? -i : 2 * (n - 1) - i;
+    out[2] = 2 * (n - 1) - i;
+    return 1 + (int)neg + (int)mir;
+}
+
+// fold_grad for R < n - 1 on every axis and an extended plane of fewer than
+// 2^31 positions.  Blocks stride over the z planes (blockIdx.y) and over the
+// x-y plane in 32-bit indices; most voxels have one preimage (one load, one
+// store), and the preimage lists (at most 3 per axis) stay in registers.
+__global__ __launch_bounds__(256) void fold_grad_near(const float *__restrict__ gext, float *__restrict__ gu, int nx,
+                                                      int ny, int nz, int R)
+{
+    const unsigned plane = (unsigned)nx * (unsigned)ny;
+    const unsigned ex = (unsigned)(nx + 2 * R), eplane = ex * (unsigned)(ny + 2 * R);
+    for (int z = blockIdx.y; z < nz; z += gridDim.y) {
+        int pz[3];
+        const int cz = preimages_near(z, nz, R, pz);
+        float *gz = gu + (long long)z * plane;
+        for (unsigned p = blockIdx.x * 256u + threadIdx.x; p < plane; p += gridDim.x * 256u) {
+            const int y = (int)(p / (unsigned)nx), x = (int)(p - (unsigned)y * (unsigned)nx);
+            int px[3], py[3];
+            const int cx = preimages_near(x, nx, R, px), cy = preimages_near(y, ny, R, py);
+            float s = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                if (k < cz) {
+                    const float *gk = gext + (long long)(pz[k] + R) * eplane;
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) {
+                        if (j < cy) {
+                            const float *gj = gk + (unsigned)(py[j] + R) * ex;
+#pragma unroll
+                            for (int i = 0; i < 3; ++i)
+                                if (i < cx) s = s + gj[px[i] + R];
+                        }
+                    }
+                }
+            }
+            gz[p] = s;
+        }
+    }
+}
+
 // Fixed-order reduction of the per-block dL/dh partials, times 2c/h.
 __global__ void reduce_dh(const float *part, int n, float scale, float *out)
 {
@@ -412,6 +463,14 @@ bool bwd3d_supported(int r_p, int r_s)
 
 hipError_t bwd3d_fold_launch(const float *gext, float *gu, int nx, int ny, int nz, int R, hipStream_t s)
 {
+    const long long eplane = (nx + 2LL * R) * (ny + 2LL * R);
+    if (R < nx - 1 && R < ny - 1 && R < nz - 1 && eplane < (1LL << 31)) {
+        const long long plane = (long long)nx * ny;
+        const unsigned gx = (unsigned)std::min<long long>((plane + 255) / 256, 1024);
+        const unsigned gy = (unsigned)std::min<long long>(nz, 65535);
+        hipLaunchKernelGGL(fold_grad_near, dim3(gx, gy), dim3(256), 0, s, gext, gu, nx, ny, nz, R);
+        return hipGetLastError();
+    }
     const long long nv = (long long)nx * ny * nz;
     long long gblk = (nv + 255) / 256;
     if (gblk > 65536) gblk = 65536;
@@ -468,10 +527,12 @@ hipError_t bwd3d_launch(const float *u, const float *y, const float *Z, const fl
     else
         e = launch_bwd<3>(a, lds, s);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(fold_grad, dim3((unsigned)gblk), dim3(256), 0, s, (const float *)a.gext, gu, nx, ny, nz, R);
-        if (gh) hipLaunchKernelGGL(reduce_dh, dim3(1), dim3(256), 0, s, (const float *)a.dh_part, nblocks,
-                                   (float)(2.0 * c / (double)h), gh);
-        e = hipGetLastError();
+        e = bwd3d_fold_launch(a.gext, gu, nx, ny, nz, R, s);
+        if (e == hipSuccess && gh) {
+            hipLaunchKernelGGL(reduce_dh, dim3(1), dim3(256), 0, s, (const float *)a.dh_part, nblocks,
+                               (float)(2.0 * c / (double)h), gh);
+            e = hipGetLastError();
+        }
     }
     hipError_t e2 = hipFreeAsync(ws, s);
     return e != hipSuccess ? e : e2;

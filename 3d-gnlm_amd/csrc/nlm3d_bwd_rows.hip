@@ -400,28 +400,34 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SYM ? 2 : 3)
 // Padded u~ (reflect-101 on [-R, n+R), 0 beyond) and the interleaved
 // (G, GY, u~, 0) with (G, GY) = (g/Z, g*y/Z) inside the volume and (0, 0)
 // outside; both with origin offset H.
-__global__ void bwd_pad(const float *u, const float *g, const float *y, const float *Z, int nx, int ny, int nz,
-                        float *pu, float4 *pg, int px, int py, int pz, int H, int R)
+__global__ __launch_bounds__(256) void bwd_pad(const float *u, const float *g, const float *y, const float *Z, int nx,
+                                               int ny, int nz, float *pu, float4 *pg, int px, int py, int pz, int H, int R)
 {
-    const long long n = (long long)px * py * pz;
-    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
-         i += (long long)gridDim.x * blockDim.x) {
-        const int x = (int)(i % px) - H;
-        const long long q = i / px;
-        const int yy = (int)(q % py) - H;
-        const int z = (int)(q / py) - H;
-        float v = 0.0f;
-        float2 gg = make_float2(0.0f, 0.0f);
-        if (x >= -R && x < nx + R && yy >= -R && yy < ny + R && z >= -R && z < nz + R) {
-            const long long o = reflect101(x, nx) + (long long)nx * (reflect101(yy, ny) + (long long)ny * reflect101(z, nz));
-            v = u[o];
-            if (x >= 0 && x < nx && yy >= 0 && yy < ny && z >= 0 && z < nz) {
-                const float G = g[o] / Z[o];
-                gg = make_float2(G, G * y[o]);
+    // The host checks 16 px py pz < 2^32, so every index here fits 32 bits.  Blocks stride over
+    // the padded z planes (blockIdx.y) and over the x-y plane.
+    const unsigned plane = (unsigned)px * (unsigned)py, nxy = (unsigned)nx * (unsigned)ny;
+    for (int zp = blockIdx.y; zp < pz; zp += gridDim.y) {
+        const int z = zp - H;
+        const bool zin = z >= -R && z < nz + R, zc = z >= 0 && z < nz;
+        const unsigned zo = zin ? nxy * (unsigned)reflect101(z, nz) : 0u;
+        float *puz = pu + (unsigned)zp * plane;
+        float4 *pgz = pg + (unsigned)zp * plane;
+        for (unsigned p = blockIdx.x * 256u + threadIdx.x; p < plane; p += gridDim.x * 256u) {
+            const int yp = (int)(p / (unsigned)px);
+            const int x = (int)(p - (unsigned)yp * (unsigned)px) - H, yy = yp - H;
+            float v = 0.0f;
+            float2 gg = make_float2(0.0f, 0.0f);
+            if (zin && x >= -R && x < nx + R && yy >= -R && yy < ny + R) {
+                const unsigned o = zo + (unsigned)reflect101(x, nx) + (unsigned)nx * (unsigned)reflect101(yy, ny);
+                v = u[o];
+                if (zc && x >= 0 && x < nx && yy >= 0 && yy < ny) {
+                    const float G = g[o] / Z[o];
+                    gg = make_float2(G, G * y[o]);
+                }
             }
+            puz[p] = v;
+            pgz[p] = make_float4(gg.x, gg.y, v, 0.0f);
         }
-        pu[i] = v;
-        pg[i] = make_float4(gg.x, gg.y, v, 0.0f);
     }
 }
 
@@ -431,34 +437,35 @@ __global__ void bwd_pad(const float *u, const float *g, const float *y, const fl
 // (origin at -R) with lx = xr - tx GX + RS in [0, SCX), ly = yr - ty GY + RS
 // in [0, SCY), lz = zr - tz GZ in [0, SCZ), at lz SCP + ly SCX + lx of its slot.
 template <int RP, int RS>
-__global__ void bwd_sym_combine(const float *__restrict__ gsc, float *__restrict__ gext, int ex, int ey, int ez,
-                                int ntx, int nty, int ntz)
+__global__ __launch_bounds__(256) void bwd_sym_combine(const float *__restrict__ gsc, float *__restrict__ gext, int ex,
+                                                       int ey, int ez, int ntx, int nty, int ntz)
 {
     using C = BwdRowsCfg<RP, RS, true>;
-    const long long n = (long long)ex * ey * ez;
-    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
-         i += (long long)gridDim.x * blockDim.x) {
-        const int xr = (int)(i % ex);
-        const long long q = i / ex;
-        const int yr = (int)(q % ey), zr = (int)(q / ey);
-        const int tx1 = min((xr + RS) / C::GX, ntx - 1), ty1 = min((yr + RS) / C::GY, nty - 1),
-                  tz1 = min(zr / C::GZ, ntz - 1);
-        float acc = 0.0f;
-        for (int tz = max(tz1 - 2, 0); tz <= tz1; ++tz) {
-            const int lz = zr - tz * C::GZ;
-            if (lz >= C::SCZ) continue;
-            for (int ty = max(ty1 - 2, 0); ty <= ty1; ++ty) {
-                const int ly = yr - ty * C::GY + RS;
-                if (ly >= C::SCY) continue;
-                for (int tx = max(tx1 - 2, 0); tx <= tx1; ++tx) {
-                    const int lx = xr - tx * C::GX + RS;
-                    if (lx >= C::SCX) continue;
-                    const long long tile = tx + (long long)ntx * (ty + (long long)nty * tz);
-                    acc += gsc[tile * C::SCN + lz * C::SCP + ly * C::SCX + lx];
+    // one extended z plane per blockIdx.y step; ex ey < 2^32 (the host's padded-copy check bounds it)
+    const unsigned plane = (unsigned)ex * (unsigned)ey;
+    for (int zr = blockIdx.y; zr < ez; zr += gridDim.y) {
+        const int tz1 = min(zr / C::GZ, ntz - 1);
+        float *gz = gext + (long long)zr * plane;
+        for (unsigned p = blockIdx.x * 256u + threadIdx.x; p < plane; p += gridDim.x * 256u) {
+            const int yr = (int)(p / (unsigned)ex), xr = (int)(p - (unsigned)yr * (unsigned)ex);
+            const int tx1 = min((xr + RS) / C::GX, ntx - 1), ty1 = min((yr + RS) / C::GY, nty - 1);
+            float acc = 0.0f;
+            for (int tz = max(tz1 - 2, 0); tz <= tz1; ++tz) {
+                const int lz = zr - tz * C::GZ;
+                if (lz >= C::SCZ) continue;
+                for (int ty = max(ty1 - 2, 0); ty <= ty1; ++ty) {
+                    const int ly = yr - ty * C::GY + RS;
+                    if (ly >= C::SCY) continue;
+                    for (int tx = max(tx1 - 2, 0); tx <= tx1; ++tx) {
+                        const int lx = xr - tx * C::GX + RS;
+                        if (lx >= C::SCX) continue;
+                        const long long tile = tx + (long long)ntx * (ty + (long long)nty * tz);
+                        acc += gsc[tile * C::SCN + lz * C::SCP + ly * C::SCX + lx];
+                    }
                 }
             }
+            gz[p] = acc;
         }
-        gext[i] = acc;
     }
 }
 
@@ -505,10 +512,9 @@ hipError_t bwd_rows_t(const float *u, const float *y, const float *Z, const floa
     a.gsc = SYM ? a.dh_part + nblocks : nullptr;
     a.pu = PadVol{pu, px, px * py, (unsigned)(4 * npad)};
     a.pg = PadVol{reinterpret_cast<const float *>(pg), px, px * py, (unsigned)(16 * npad)};
-    long long gb = (npad + 255) / 256;
-    if (gb > 65536) gb = 65536;
-    hipLaunchKernelGGL(bwd_pad, dim3((unsigned)gb), dim3(256), 0, s, u, g, y, Z, nx, ny, nz, pu, pg, px, py, pz, C::H,
-                       C::R);
+    const unsigned pgx = (unsigned)std::min<long long>(((long long)px * py + 255) / 256, 1024);
+    hipLaunchKernelGGL(bwd_pad, dim3(pgx, (unsigned)std::min(pz, 65535)), dim3(256), 0, s, u, g, y, Z, nx, ny, nz, pu,
+                       pg, px, py, pz, C::H, C::R);
     e = hipGetLastError();
     if (e == hipSuccess) {
         const size_t lds = bwd_rows_lds<RP, RS, SYM>();
@@ -521,9 +527,8 @@ hipError_t bwd_rows_t(const float *u, const float *y, const float *Z, const floa
     }
     if (SYM && e == hipSuccess) {
         const int ex = nx + 2 * C::R, ey = ny + 2 * C::R, ez = nz + 2 * C::R;
-        long long cb = (ext + 255) / 256;
-        if (cb > 65536) cb = 65536;
-        hipLaunchKernelGGL((bwd_sym_combine<RP, RS>), dim3((unsigned)cb), dim3(256), 0, s, a.gsc, a.gext, ex, ey, ez,
+        const unsigned cgx = (unsigned)std::min<long long>(((long long)ex * ey + 255) / 256, 1024);
+        hipLaunchKernelGGL((bwd_sym_combine<RP, RS>), dim3(cgx, (unsigned)std::min(ez, 65535)), dim3(256), 0, s, a.gsc, a.gext, ex, ey, ez,
                            a.ntx, a.nty, a.ntz);
         e = hipGetLastError();
     }
