@@ -1,12 +1,20 @@
 // nlm3d_rows.hip -- per-volume launch of the "rows" forward variants 4 and 5
 // (r_p 1-4 and 2; tile bodies in nlm3d_rows.h, launch in nlm3d_rows_launch.h).
+#include <atomic>
 #include "nlm3d_rows_launch.h"
 
 namespace gnlm {
 namespace {
 
-template <int RP, int RS, bool SHORT>
-__global__ __launch_bounds__(kRowsThreads) __attribute__((amdgpu_waves_per_eu(SHORT ? 4 : (RP == 2 ? 2 : (RP >= 3 ? 4 : 3))))) void nlm3d_fwd_rows(RowsArgs a)
+// Waves per SIMD each tile shape is compiled for (its register budget).
+template <int RP, bool SHORT>
+constexpr int rows_waves()
+{
+    return SHORT ? 4 : (RP == 2 ? 2 : (RP >= 3 ? 4 : 3));
+}
+
+template <int RP, int RS, bool SHORT, int WAVES>
+__global__ __launch_bounds__(kRowsThreads) __attribute__((amdgpu_waves_per_eu(WAVES))) void nlm3d_fwd_rows(RowsArgs a)
 {
     extern __shared__ __align__(16) float lds[];
     rows_kernel_body<RowsCfg<RP, RS, SHORT>>(a, lds);
@@ -18,10 +26,45 @@ constexpr size_t rows_lds()
     return rows_lds_cfg<RowsCfg<RP, RS, SHORT>>();
 }
 
+// Compute units of the current device (cached per device).
+int device_cus()
+{
+    static std::atomic<int> cache[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    int n = cache[dev].load(std::memory_order_relaxed);
+    if (n <= 0) {
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+        cache[dev].store(n, std::memory_order_relaxed);
+    }
+    return n;
+}
+
+// r_p = 1 tiles come in two register budgets: 168 VGPRs (3 tiles per CU) and
+// 128 VGPRs (4 per CU; the spills fall in the per-dz flush, outside the offset
+// loop).  Per CU the two process tiles at about the same rate, so the choice
+// is the number of rounds of tiles: a launch of t tiles takes about
+// ceil(t / 4c) rounds of 4/3 the length at 4 per CU, against ceil(t / 3c)
+// at 3 per CU (c compute units).  At 128^3 (800 tiles, 256 CUs) that is one
+// round against two: 1.52 -> 1.36 ms for r_s = 4.  Past four rounds the
+// quantisation no longer matters and the 168-VGPR build stays.
+bool rows_rp1_four_per_cu(long long tiles)
+{
+    const long long c = device_cus(), r3 = (tiles + 3 * c - 1) / (3 * c), r4 = (tiles + 4 * c - 1) / (4 * c);
+    return r3 <= 4 && 4 * r4 < 3 * r3;
+}
+
 template <int RP, int RS, bool SHORT>
 hipError_t rows_launch_t(const InDesc &in, float *out, float *Zout, float k, hipStream_t s)
 {
-    return rows_launch_cfg<RowsCfg<RP, RS, SHORT>>(in, out, Zout, k, s, nlm3d_fwd_rows<RP, RS, SHORT>);
+    using C = RowsCfg<RP, RS, SHORT>;
+    if constexpr (RP == 1) {
+        const long long tiles = (long long)((in.nx + C::TX - 1) / C::TX) * ((in.ny + C::TY - 1) / C::TY) *
+                                ((in.nz + C::TZ - 1) / C::TZ);
+        if (rows_rp1_four_per_cu(tiles))
+            return rows_launch_cfg<C>(in, out, Zout, k, s, nlm3d_fwd_rows<RP, RS, SHORT, 4>);
+    }
+    return rows_launch_cfg<C>(in, out, Zout, k, s, nlm3d_fwd_rows<RP, RS, SHORT, rows_waves<RP, SHORT>()>);
 }
 
 // Instantiated radii: r_p in {1, 2, 3, 4} with r_s up to 8 (r_p = 1) or 7

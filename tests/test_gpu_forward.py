@@ -299,6 +299,27 @@ def test_3d_config1(dev):
     assert max_rel_fl(y, ref64) <= TOL
 
 
+def test_3d_rows_rp1_at_128_cubed(dev):
+    """r_p = 1 at 128^3: 800 tiles of 16 x 14 x 14, more than 3 per CU fit at
+    once on a 256-CU device, so the launch takes the 128-VGPR build (4 tiles
+    per CU, one round).  Slabs at both z ends (the last tile layer is partial)
+    and in the middle against the CPU FP32 path and FP64 (<= 1e-5 relative)."""
+    rng = np.random.default_rng(128)
+    u = rng.random((128, 128, 128), dtype=np.float32)
+    rp, rs, h = 1, 4, 0.25
+    try:
+        gnlm.set_variant(rp, rs, gnlm.VARIANT_ROWS)
+        y = _run3(dev, u, rp, rs, h, gnlm.PATH_TILED)
+    finally:
+        gnlm.reset_tuning()
+    for z0 in (0, 61, 125):
+        ref32 = oracle.nlm3d_separable_f32(u, rp, rs, h, z0=z0, z1=z0 + 3)
+        ref64 = oracle.nlm3d_separable_f64(u, rp, rs, h, z0=z0, z1=z0 + 3)
+        print("rp1 128^3 slab", z0, "vs fp32", report(y[z0:z0 + 3], ref32), "vs fp64", report(y[z0:z0 + 3], ref64))
+        assert max_rel(y[z0:z0 + 3], ref32) <= TOL
+        assert max_rel(y[z0:z0 + 3], ref64) <= TOL
+
+
 def test_3d_padded_equals_reflect(dev):
     rng = np.random.default_rng(11)
     u = rng.random((13, 17, 19), dtype=np.float32)
