@@ -339,9 +339,9 @@ def main():
                     help="one launch of the workload plus a 256 MiB calibration copy (for rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(the volume path shards into z-slabs across GPUs by design, but only the one-GPU "
-              "share is built and measured)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(the volume path would shard into z-slabs across GPUs; only one GPU's share "
+              "is implemented and measured)", file=sys.stderr)
         sys.exit(2)
 
     import torch
