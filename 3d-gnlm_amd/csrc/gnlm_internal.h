@@ -70,6 +70,9 @@ hipError_t bwd3d_rows_launch(const float *u, const float *y, const float *Z, con
 enum { kBwdAuto = 0, kBwdGather = 1, kBwdSymmetric = 2 };
 int bwd3d_set_form(int form);  // -1 for an unknown form
 hipError_t bwd3d_fold_launch(const float *gext, float *gu, int nx, int ny, int nz, int R, hipStream_t s);
+
+// Compute units of the current device (queried once per device; 256 if the query fails).
+int device_cus();
 hipError_t bwd3d_reduce_dh_launch(const float *part, int n, float scale, float *out, hipStream_t s);
 
 // ---- misc (gnlm_misc.hip) --------------------------------------------------

@@ -92,6 +92,7 @@ struct BwdRowsArgs {
     float k;         // w = exp2(S * k)
     float negc2;     // -2c: E carries the factor 2 of the B term
     int ntx, nty, ntz;
+    int nparts;      // SYM: workgroups per tile, each over a contiguous run of offset rows (1: gather form)
 };
 
 __device__ __forceinline__ float2 pad_load2(const PadRsrc &r, unsigned voff, int soff)
@@ -122,7 +123,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SYM ? 2 : 3)
     const int px = __builtin_amdgcn_readfirstlane(a.pu.px), pxy = __builtin_amdgcn_readfirstlane(a.pu.pxy);
     const float kw = a.k, negc2 = a.negc2;
 
-    const int t = xcd_tile(blockIdx.x, a.ntx * a.nty * a.ntz);
+    // unit = (tile, part): the parts of a tile are adjacent, so they share an XCD's L2
+    const int np = a.nparts;
+    const int unit = xcd_tile(blockIdx.x, a.ntx * a.nty * a.ntz * np);
+    const int t = unit / np, part = unit - t * np;
     // gradient tile origin on the extended domain
     const int Xt = -C::R + (t % a.ntx) * C::GX;
     const int Yt = -C::R + ((t / a.ntx) % a.nty) * C::GY;
@@ -194,7 +198,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SYM ? 2 : 3)
     // itself contributes G(z) (w_0 = 1, u~(z) - u~(z) = 0)
     float gacc[GZ];
 #pragma unroll
-    for (int z = 0; z < GZ; ++z) gacc[z] = SYM ? Gc[z + RP] : 0.0f;
+    for (int z = 0; z < GZ; ++z) gacc[z] = (SYM && part == 0) ? Gc[z + RP] : 0.0f;
     // sum of S E over the tile's positions: sum_d w S t1 = (1/2) sum_d S E / (-2c)
     // summed over the whole extended domain (t2_d(x) = t1_-d(x+d))
     float dh = 0.0f;
@@ -206,6 +210,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SYM ? 2 : 3)
 
     constexpr int NW = 2 * RS + 1, NOFF = NW * NW * NW;
     constexpr int O0 = SYM ? (NOFF + 1) / 2 : 0;  // first offset visited
+    // part p visits the offset rows (dy, dz) [OR0 + p NRW / np, OR0 + (p + 1) NRW / np);
+    // part 0 also takes the partial row before OR0.  Every part then starts its
+    // shifted-row windows at a row start (or at O0), as one part over all rows does.
+    constexpr int OR0 = (O0 + NW - 1) / NW, NRW = NW * NW - OR0;
+    const int ob = part == 0 ? O0 : (OR0 + part * NRW / np) * NW;
+    const int oe = (OR0 + (part + 1) * NRW / np) * NW;
+    // d of the first offset: the scalar buffer offset and the scatter offset follow
+    const int dxb = ob % NW - RS, dyb = (ob / NW) % NW - RS, dzb = ob / (NW * NW) - RS;
     // scalar offset of d relative to d = (-RS, -RS, -RS) (always >= 0), stepped
     // through the offsets in (dz, dy, dx) order
     auto sd_next = [&](int sd, int o) {  // offset index o -> o + 1
@@ -232,13 +244,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SYM ? 2 : 3)
     float winx[NWINX];
     // the column values at x + d, one offset ahead
     float ud[QZ], Gd[QZ], GYd[QZ];
-    int sd = SYM ? s0 + 1 : 0;  // d = (1, 0, 0) or (-RS, -RS, -RS)
-    int scd = 1;                // SYM: scatter offset of d, starting at d = (1, 0, 0)
+    int sd = s0 + dzb * pxy + dyb * px + dxb;      // 0 at d = (-RS, -RS, -RS)
+    int scd = dzb * C::SCP + dyb * C::SCX + dxb;   // SYM: scatter offset of d (dz >= 0)
     load_col(sd, ud, Gd, GYd);
 
-    for (int o = O0; o < NOFF; ++o) {
-        const int sn = (o + 1 < NOFF) ? sd_next(sd, o) : sd;  // next offset (reloads the last one at the end)
-        if (o == O0 || o % NW == 0) {  // a new (dy, dz) (SYM starts at dx = 1: window[m] = sample at d + m)
+    for (int o = ob; o < oe; ++o) {
+        const int sn = (o + 1 < oe) ? sd_next(sd, o) : sd;  // next offset (reloads the last one at the end)
+        if (o == ob || o % NW == 0) {  // a new (dy, dz) (SYM starts at dx = 1: window[m] = sample at d + m)
 #pragma unroll
             for (int m = 0; m < NWIN; m += 2) {  // row starts are 8-byte aligned (even padded x)
                 const float2 t = pad_load2(ru, rbase, 4 * (sd + m));
@@ -372,7 +384,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SYM ? 2 : 3)
             for (int z = 0; z < GZ; ++z) SC[scb + z * C::SCP] += gacc[z];
         }
         __syncthreads();
-        float *dst = a.gsc + (size_t)t * C::SCN;
+        float *dst = a.gsc + (size_t)unit * C::SCN;
         for (int i = tid; i < C::SCN; i += NT) dst[i] = SC[i];
     } else if (inT) {  // ---- gradient of the tile's extended-domain positions ----
         const long long ex = a.nx + 2LL * C::R, ey = a.ny + 2LL * C::R, ez = a.nz + 2LL * C::R;
@@ -433,12 +445,12 @@ __global__ __launch_bounds__(256) void bwd_pad(const float *u, const float *g, c
 
 // Symmetric form: the gradient on the extended domain is the sum of the
 // scatter regions of the (up to 2 x 2 x 2) tiles that cover a position, added
-// in a fixed tile order.  Tile (tx, ty, tz) covers ext positions (xr, yr, zr)
+// in a fixed tile order (a tile's np part slots in part order).  Tile (tx, ty, tz) covers ext positions (xr, yr, zr)
 // (origin at -R) with lx = xr - tx GX + RS in [0, SCX), ly = yr - ty GY + RS
 // in [0, SCY), lz = zr - tz GZ in [0, SCZ), at lz SCP + ly SCX + lx of its slot.
 template <int RP, int RS>
 __global__ __launch_bounds__(256) void bwd_sym_combine(const float *__restrict__ gsc, float *__restrict__ gext, int ex,
-                                                       int ey, int ez, int ntx, int nty, int ntz)
+                                                       int ey, int ez, int ntx, int nty, int ntz, int np)
 {
     using C = BwdRowsCfg<RP, RS, true>;
     // one extended z plane per blockIdx.y step; ex ey < 2^32 (the host's padded-copy check bounds it)
@@ -460,7 +472,8 @@ __global__ __launch_bounds__(256) void bwd_sym_combine(const float *__restrict__
                         const int lx = xr - tx * C::GX + RS;
                         if (lx >= C::SCX) continue;
                         const long long tile = tx + (long long)ntx * (ty + (long long)nty * tz);
-                        acc += gsc[tile * C::SCN + lz * C::SCP + ly * C::SCX + lx];
+                        const float *slot = gsc + tile * np * C::SCN + lz * C::SCP + ly * C::SCX + lx;
+                        for (int p = 0; p < np; ++p) acc += slot[(long long)p * C::SCN];
                     }
                 }
             }
@@ -473,6 +486,25 @@ template <int RP, int RS, bool SYM>
 constexpr size_t bwd_rows_lds()
 {
     return BwdRowsCfg<RP, RS, SYM>::LDS_FLOATS * sizeof(float);
+}
+
+// Workgroups per tile for the symmetric form: a launch of t tiles on a device
+// that holds `slots` of them at once takes about ceil(np t / slots) / np
+// rounds of one tile's work when each tile's offset rows are split over np
+// workgroups.  More parts are taken only when they cut that by at least 5%
+// (each part repeats the tile's loads and writes its own scatter slot).
+int bwd_sym_parts(long long t, long long slots, int nrows)
+{
+    int best = 1;
+    double cbest = (double)((t + slots - 1) / slots);
+    for (int np = 2; np <= 4 && np <= nrows; ++np) {
+        const double c = (double)((np * t + slots - 1) / slots) / np;
+        if (c < 0.95 * cbest) {
+            best = np;
+            cbest = c;
+        }
+    }
+    return best;
 }
 
 template <int RP, int RS, bool SYM>
@@ -499,17 +531,31 @@ hipError_t bwd_rows_t(const float *u, const float *y, const float *Z, const floa
     const double c = 1.0 / ((double)P * (double)h * (double)h);
     a.k = (float)weight_scale(P, h);
     a.negc2 = (float)(-2.0 * c);
+    const size_t lds = bwd_rows_lds<RP, RS, SYM>();
+    hipError_t e = hipFuncSetAttribute((const void *)nlm3d_bwd_rows<RP, RS, SYM>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    a.nparts = 1;
+    if (SYM) {
+        constexpr int NW = 2 * RS + 1, O0 = (NW * NW * NW + 1) / 2, NRW = NW * NW - (O0 + NW - 1) / NW;
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)nlm3d_bwd_rows<RP, RS, SYM>, NT,
+                                                         lds) == hipSuccess &&
+            per_cu > 0)
+            a.nparts = bwd_sym_parts(nblocks, (long long)per_cu * device_cus(), NRW);
+    }
+    const int nunits = nblocks * a.nparts;
     float *ws = nullptr;
     // layout: (G, GY, u~, 0) copy | u~ copy | gradient on the extended domain | dL/dh partials
-    //         | scatter regions (SYM)
-    const size_t nfl = (size_t)(4 * npad + npad + ext + nblocks) + (size_t)nblocks * C::SCN;
-    hipError_t e = hipMallocAsync((void **)&ws, sizeof(float) * nfl, s);
+    //         | scatter regions (SYM, one per unit)
+    const size_t nfl = (size_t)(4 * npad + npad + ext + nunits) + (size_t)nunits * C::SCN;
+    e = hipMallocAsync((void **)&ws, sizeof(float) * nfl, s);
     if (e != hipSuccess) return e;
     float4 *pg = reinterpret_cast<float4 *>(ws);
     float *pu = ws + 4 * npad;
     a.gext = pu + npad;
     a.dh_part = a.gext + ext;
-    a.gsc = SYM ? a.dh_part + nblocks : nullptr;
+    a.gsc = SYM ? a.dh_part + nunits : nullptr;
     a.pu = PadVol{pu, px, px * py, (unsigned)(4 * npad)};
     a.pg = PadVol{reinterpret_cast<const float *>(pg), px, px * py, (unsigned)(16 * npad)};
     const unsigned pgx = (unsigned)std::min<long long>(((long long)px * py + 255) / 256, 1024);
@@ -517,26 +563,21 @@ hipError_t bwd_rows_t(const float *u, const float *y, const float *Z, const floa
                        pg, px, py, pz, C::H, C::R);
     e = hipGetLastError();
     if (e == hipSuccess) {
-        const size_t lds = bwd_rows_lds<RP, RS, SYM>();
-        e = hipFuncSetAttribute((const void *)nlm3d_bwd_rows<RP, RS, SYM>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e == hipSuccess) {
-            hipLaunchKernelGGL((nlm3d_bwd_rows<RP, RS, SYM>), dim3(nblocks), dim3(NT), lds, s, a);
-            e = hipGetLastError();
-        }
+        hipLaunchKernelGGL((nlm3d_bwd_rows<RP, RS, SYM>), dim3(nunits), dim3(NT), lds, s, a);
+        e = hipGetLastError();
     }
     if (SYM && e == hipSuccess) {
         const int ex = nx + 2 * C::R, ey = ny + 2 * C::R, ez = nz + 2 * C::R;
         const unsigned cgx = (unsigned)std::min<long long>(((long long)ex * ey + 255) / 256, 1024);
-        hipLaunchKernelGGL((bwd_sym_combine<RP, RS>), dim3(cgx, (unsigned)std::min(ez, 65535)), dim3(256), 0, s, a.gsc, a.gext, ex, ey, ez,
-                           a.ntx, a.nty, a.ntz);
+        hipLaunchKernelGGL((bwd_sym_combine<RP, RS>), dim3(cgx, (unsigned)std::min(ez, 65535)), dim3(256), 0, s, a.gsc,
+                           a.gext, ex, ey, ez, a.ntx, a.nty, a.ntz, a.nparts);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = bwd3d_fold_launch(a.gext, gu, nx, ny, nz, C::R, s);
     // the partials sum S E: dL/dh = (2c/h) sum_all-d w S t1 = -(1/(2h)) sum_all-d S E,
     // and the symmetric form visits half of the offsets (the other half has the same sum)
     if (e == hipSuccess && gh)
-        e = bwd3d_reduce_dh_launch(a.dh_part, nblocks, (float)((SYM ? -1.0 : -0.5) / (double)h), gh, s);
+        e = bwd3d_reduce_dh_launch(a.dh_part, nunits, (float)((SYM ? -1.0 : -0.5) / (double)h), gh, s);
     const hipError_t e2 = hipFreeAsync(ws, s);
     return e != hipSuccess ? e : e2;
 }

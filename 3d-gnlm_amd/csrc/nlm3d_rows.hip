@@ -26,20 +26,6 @@ constexpr size_t rows_lds()
     return rows_lds_cfg<RowsCfg<RP, RS, SHORT>>();
 }
 
-// Compute units of the current device (cached per device).
-int device_cus()
-{
-    static std::atomic<int> cache[64];
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
-    int n = cache[dev].load(std::memory_order_relaxed);
-    if (n <= 0) {
-        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
-        cache[dev].store(n, std::memory_order_relaxed);
-    }
-    return n;
-}
-
 // r_p = 1 tiles come in two register budgets: 168 VGPRs (3 tiles per CU) and
 // 128 VGPRs (4 per CU; the spills fall in the per-dz flush, outside the offset
 // loop).  Per CU the two process tiles at about the same rate, so the choice
@@ -79,6 +65,19 @@ hipError_t rows_launch_t(const InDesc &in, float *out, float *Zout, float k, hip
     X(4, 7, false)
 
 }  // namespace
+
+int device_cus()
+{
+    static std::atomic<int> cache[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    int n = cache[dev].load(std::memory_order_relaxed);
+    if (n <= 0) {
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+        cache[dev].store(n, std::memory_order_relaxed);
+    }
+    return n;
+}
 
 bool fwd3d_rows_supported(int r_p, int r_s, bool short_tiles)
 {

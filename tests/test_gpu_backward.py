@@ -87,9 +87,17 @@ def test_backward_forms_agree_rp2(dev):
     _forms_agree(dev, 2, 3)
 
 
-def _forms_agree(dev, rp, rs):
+def test_backward_forms_agree_whole_tiles(dev):
+    """88^3: 490 tiles of the symmetric form at r_p 1, r_s 4, within one round
+    of 512 workgroup slots on a 256-CU device, so each workgroup takes a whole
+    tile.  The 40^3 volumes above split each tile's offset rows over several
+    workgroups; both layouts agree with the gather form."""
+    _forms_agree(dev, 1, 4, n=88)
+
+
+def _forms_agree(dev, rp, rs, n=40):
     rng = np.random.default_rng(5 + rp)
-    u = rng.random((40, 40, 40)).astype(np.float32)
+    u = rng.random((n, n, n)).astype(np.float32)
     g = rng.standard_normal(u.shape).astype(np.float32)
     try:
         gnlm.set_backward_form(gnlm.BWD_GATHER)
