@@ -445,9 +445,10 @@ __global__ __launch_bounds__(256) void bwd_pad(const float *u, const float *g, c
 
 // Symmetric form: the gradient on the extended domain is the sum of the
 // scatter regions of the (up to 2 x 2 x 2) tiles that cover a position, added
-// in a fixed tile order (a tile's np part slots in part order).  Tile (tx, ty, tz) covers ext positions (xr, yr, zr)
-// (origin at -R) with lx = xr - tx GX + RS in [0, SCX), ly = yr - ty GY + RS
-// in [0, SCY), lz = zr - tz GZ in [0, SCZ), at lz SCP + ly SCX + lx of its slot.
+// in a fixed tile order, and within a tile its np part slots in part order.
+// Tile (tx, ty, tz) covers ext positions (xr, yr, zr) (origin at -R) with
+// lx = xr - tx GX + RS in [0, SCX), ly = yr - ty GY + RS in [0, SCY),
+// lz = zr - tz GZ in [0, SCZ), at lz SCP + ly SCX + lx of each of its slots.
 template <int RP, int RS>
 __global__ __launch_bounds__(256) void bwd_sym_combine(const float *__restrict__ gsc, float *__restrict__ gext, int ex,
                                                        int ey, int ez, int ntx, int nty, int ntz, int np)
