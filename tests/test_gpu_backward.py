@@ -95,6 +95,12 @@ def test_backward_forms_agree_whole_tiles(dev):
     _forms_agree(dev, 1, 4, n=88)
 
 
+def test_backward_forms_agree_three_parts(dev):
+    """70^3: 288 tiles at r_p 1, r_s 4, so on a 256-CU device each tile's 40
+    offset rows go to 3 workgroups of 13, 13 and 14 rows."""
+    _forms_agree(dev, 1, 4, n=70)
+
+
 def _forms_agree(dev, rp, rs, n=40):
     rng = np.random.default_rng(5 + rp)
     u = rng.random((n, n, n)).astype(np.float32)
