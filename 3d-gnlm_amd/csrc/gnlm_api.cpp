@@ -24,6 +24,7 @@ inline int hip_status(hipError_t e)
     if (e == hipSuccess) return GNLM_OK;
     if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return GNLM_ENOMEM;
     if (e == hipErrorInvalidValue) return GNLM_EINVAL;
+    if (e == hipErrorNotSupported) return GNLM_EUNSUPPORTED;
     return GNLM_EHIP;
 }
 
@@ -64,20 +65,38 @@ int gnlm3d_forward_ex(const float *d_in, int in_pad, long long in_sy, long long 
                       float *d_Z, int nx, int ny, int nz, int r_p, int r_s, float h, int path, void *stream)
 {
     if (!d_in || !d_out) return GNLM_EINVAL;
-    if (nx < 1 || ny < 1 || nz < 1 || r_p < 0 || r_s < 0 || bad_h(h)) return GNLM_EINVAL;
-    if ((long long)nx * ny * nz > (1LL << 40)) return GNLM_EINVAL;
-    if (path < GNLM_PATH_AUTO || path > GNLM_PATH_GENERIC) return GNLM_EINVAL;
+    if (bad_h(h)) return GNLM_EINVAL;
+    int kernel = 0;
+    int st = gnlm3d_forward_route(nx, ny, nz, r_p, r_s, path, &kernel);
+    if (st != GNLM_OK) return st;
     const int R = r_p + r_s;
-    int st = check_padded(in_pad, in_sy, in_sz, nx, ny, R);
+    st = check_padded(in_pad, in_sy, in_sz, nx, ny, R);
     if (st != GNLM_OK) return st;
     InDesc in{d_in, nx, ny, nz, in_pad, in_sy, in_sz};
     const int P = (2 * r_p + 1) * (2 * r_p + 1) * (2 * r_p + 1);
     const float k = (float)weight_scale(P, h);
     hipStream_t s = (hipStream_t)stream;
-    const bool tiled_ok = fwd3d_tiled_supported(r_p, r_s);
-    if (path == GNLM_PATH_TILED && !tiled_ok) return GNLM_EUNSUPPORTED;
-    if (path != GNLM_PATH_GENERIC && tiled_ok) return hip_status(fwd3d_tiled_launch(in, d_out, d_Z, r_p, r_s, k, s));
+    if (kernel != GNLM_KERNEL_GENERIC)
+        return hip_status(fwd3d_tiled_launch_variant(in, d_out, d_Z, r_p, r_s, k, kernel, s));
     return hip_status(fwd_generic_launch(in, d_out, d_Z, r_p, r_s, r_p, r_s, k, s));
+}
+
+int gnlm3d_forward_route(int nx, int ny, int nz, int r_p, int r_s, int path, int *kernel)
+{
+    if (!kernel) return GNLM_EINVAL;
+    if (nx < 1 || ny < 1 || nz < 1 || r_p < 0 || r_s < 0) return GNLM_EINVAL;
+    if (path < GNLM_PATH_AUTO || path > GNLM_PATH_GENERIC) return GNLM_EINVAL;
+    // 2^40 voxels (4 TiB per FP32 buffer) exceed any device's memory
+    if ((long long)nx * ny * nz > (1LL << 40)) return GNLM_EUNSUPPORTED;
+    return fwd3d_route(nx, ny, nz, r_p, r_s, path, kernel);
+}
+
+int gnlm3d_backward_route(int nx, int ny, int nz, int r_p, int r_s, int *kernel)
+{
+    if (!kernel) return GNLM_EINVAL;
+    if (nx < 1 || ny < 1 || nz < 1 || r_p < 0 || r_s < 0) return GNLM_EINVAL;
+    if ((long long)nx * ny * nz > (1LL << 40)) return GNLM_EUNSUPPORTED;
+    return bwd3d_route(nx, ny, nz, r_p, r_s, kernel);
 }
 
 int gnlm3d_forward(const float *d_in, float *d_out, float *d_Z, int nx, int ny, int nz, int r_p, int r_s,
@@ -118,8 +137,10 @@ int gnlm3d_backward(const float *d_in, const float *d_out, const float *d_Z, con
                     int nz, int r_p, int r_s, float h, float *d_gin, float *d_gh, void *stream)
 {
     if (!d_in || !d_out || !d_Z || !d_gout || !d_gin) return GNLM_EINVAL;
-    if (nx < 1 || ny < 1 || nz < 1 || r_p < 0 || r_s < 0 || bad_h(h)) return GNLM_EINVAL;
-    if (!bwd3d_supported(r_p, r_s)) return GNLM_EUNSUPPORTED;
+    if (bad_h(h)) return GNLM_EINVAL;
+    int kernel = 0;
+    const int st = gnlm3d_backward_route(nx, ny, nz, r_p, r_s, &kernel);
+    if (st != GNLM_OK) return st;
     return hip_status(bwd3d_launch(d_in, d_out, d_Z, d_gout, nx, ny, nz, r_p, r_s, h, d_gin, d_gh,
                                    (hipStream_t)stream));
 }

@@ -22,9 +22,15 @@ bool fwd3d_variant_supported(int r_p, int r_s, int variant);
 bool fwd3d_tiled_supported(int r_p, int r_s);
 hipError_t fwd3d_tiled_launch_variant(const InDesc &in, float *out, float *Zout, int r_p, int r_s, float k,
                                       int variant, hipStream_t s);
-// Launch with the variant the autotuner chose for (r_p, r_s) (or the default).
-hipError_t fwd3d_tiled_launch(const InDesc &in, float *out, float *Zout, int r_p, int r_s, float k,
-                              hipStream_t s);
+// Whether a rows-family variant's padded copy of an nx*ny*nz volume fits its
+// 32-bit offsets.
+bool fwd3d_rows_fits(int nx, int ny, int nz, int r_p, int r_s, int variant);
+// The kernel a forward of this geometry runs (host logic only): the tile
+// variant from the autotuner's table, or, where a rows-family variant's
+// 32-bit offsets cannot address the volume, the first 64-bit tile variant
+// that fits, else GNLM_KERNEL_GENERIC.  GNLM_EUNSUPPORTED when path is
+// GNLM_PATH_TILED and no tile variant can run it.
+int fwd3d_route(int nx, int ny, int nz, int r_p, int r_s, int path, int *kernel);
 
 // ---- "rows" variant (nlm3d_rows.hip) ---------------------------------------
 // r_p in {1, 2, 3, 4} with r_s up to 8 / 7 / 7 / 7; pads the input into a stream-ordered
@@ -56,6 +62,11 @@ hipError_t fwd_generic_launch(const InDesc &in, float *out, float *Zout, int rp,
 
 // ---- adjoint (nlm3d_bwd.hip, nlm3d_bwd_rows.hip) ----------------------------
 bool bwd3d_supported(int r_p, int r_s);
+bool bwd3d_block_supported(int r_p, int r_s);  // the LDS-block adjoint (64-bit indexing)
+// The adjoint kernel for this geometry (host logic only): the rows kernels in
+// the selected form where their 32-bit offsets address the volume, else the
+// LDS-block tile; GNLM_EUNSUPPORTED when neither can run it.
+int bwd3d_route(int nx, int ny, int nz, int r_p, int r_s, int *kernel);
 hipError_t bwd3d_launch(const float *u, const float *y, const float *Z, const float *g, int nx, int ny, int nz,
                         int rp, int rs, float h, float *gu, float *gh, hipStream_t s);
 // rows-organised adjoint for r_p 0-2, r_s in 1..7 (bwd3d_launch dispatches to it)
@@ -69,6 +80,9 @@ hipError_t bwd3d_rows_launch(const float *u, const float *y, const float *Z, con
 // symmetric form, the faster one at every instantiated r_s on MI355X.
 enum { kBwdAuto = 0, kBwdGather = 1, kBwdSymmetric = 2 };
 int bwd3d_set_form(int form);  // -1 for an unknown form
+bool bwd3d_rows_symmetric_form();
+int bwd3d_set_parts(int nparts);  // 0 = automatic, 1-4 forced; -1 if out of range
+bool bwd3d_rows_fits(int nx, int ny, int nz, int r_p, int r_s, bool sym);
 hipError_t bwd3d_fold_launch(const float *gext, float *gu, int nx, int ny, int nz, int R, hipStream_t s);
 
 // Compute units of the current device (queried once per device; 256 if the query fails).

@@ -144,6 +144,8 @@ int gnlm_num_variants(void) { return kNumFwd3Variants; }
 
 int gnlm_set_backward_form(int form) { return bwd3d_set_form(form) == 0 ? GNLM_OK : GNLM_EINVAL; }
 
+int gnlm_set_backward_parts(int nparts) { return bwd3d_set_parts(nparts) == 0 ? GNLM_OK : GNLM_EINVAL; }
+
 void gnlm_reset_tuning(void)
 {
     std::lock_guard<std::mutex> lk(g_mu);

@@ -455,10 +455,29 @@ int bwd_blocks(int nx, int ny, int nz, int rs)
 
 }  // namespace
 
-bool bwd3d_supported(int r_p, int r_s)
+bool bwd3d_block_supported(int r_p, int r_s)
 {
-    return bwd3d_rows_supported(r_p, r_s) ||
-           (r_p >= 0 && r_p <= 3 && r_s >= 0 && bwd_lds_bytes_rp(r_p, r_s) <= kLdsLimitBytes);
+    return r_p >= 0 && r_p <= 3 && r_s >= 0 && bwd_lds_bytes_rp(r_p, r_s) <= kLdsLimitBytes;
+}
+
+bool bwd3d_supported(int r_p, int r_s) { return bwd3d_rows_supported(r_p, r_s) || bwd3d_block_supported(r_p, r_s); }
+
+int bwd3d_route(int nx, int ny, int nz, int r_p, int r_s, int *kernel)
+{
+    // the rows kernels address their padded copies with 32-bit offsets; the
+    // LDS-block tile indexes in 64 bits and takes the volumes they cannot
+    if (bwd3d_rows_supported(r_p, r_s)) {
+        const bool sym = bwd3d_rows_symmetric_form();
+        if (bwd3d_rows_fits(nx, ny, nz, r_p, r_s, sym)) {
+            *kernel = sym ? GNLM_KERNEL_BWD_ROWS_SYMMETRIC : GNLM_KERNEL_BWD_ROWS_GATHER;
+            return GNLM_OK;
+        }
+    }
+    if (bwd3d_block_supported(r_p, r_s)) {
+        *kernel = GNLM_KERNEL_BWD_BLOCK;
+        return GNLM_OK;
+    }
+    return GNLM_EUNSUPPORTED;
 }
 
 hipError_t bwd3d_fold_launch(const float *gext, float *gu, int nx, int ny, int nz, int R, hipStream_t s)
@@ -487,7 +506,9 @@ hipError_t bwd3d_reduce_dh_launch(const float *part, int n, float scale, float *
 hipError_t bwd3d_launch(const float *u, const float *y, const float *Z, const float *g, int nx, int ny, int nz,
                         int rp, int rs, float h, float *gu, float *gh, hipStream_t s)
 {
-    if (bwd3d_rows_supported(rp, rs)) return bwd3d_rows_launch(u, y, Z, g, nx, ny, nz, rp, rs, h, gu, gh, s);
+    int kernel = 0;
+    if (bwd3d_route(nx, ny, nz, rp, rs, &kernel) != GNLM_OK) return hipErrorNotSupported;
+    if (kernel != GNLM_KERNEL_BWD_BLOCK) return bwd3d_rows_launch(u, y, Z, g, nx, ny, nz, rp, rs, h, gu, gh, s);
     const int R = rp + rs;
     const long long nv = (long long)nx * ny * nz;
     const long long ext = (nx + 2LL * R) * (ny + 2LL * R) * (nz + 2LL * R);

@@ -508,11 +508,28 @@ int bwd_sym_parts(long long t, long long slots, int nrows)
     return best;
 }
 
+// The (G, GY, u~, 0) copy spans [-H, nt * G + RP) on each axis at 16 bytes per
+// position, addressed with 32-bit byte offsets: it must stay below 4 GiB.
+template <int RP, int RS, bool SYM>
+bool bwd_rows_fits_t(int nx, int ny, int nz)
+{
+    using C = BwdRowsCfg<RP, RS, SYM>;
+    const long long ntx = (nx + 2LL * C::R + C::GX - 1) / C::GX, nty = (ny + 2LL * C::R + C::GY - 1) / C::GY,
+                    ntz = (nz + 2LL * C::R + C::GZ - 1) / C::GZ;
+    const long long px = ntx * C::GX + C::RP + C::H, py = nty * C::GY + C::RP + C::H,
+                    pz = ntz * C::GZ + C::RP + C::H;
+    return 16 * px * py * pz < (1LL << 32) && ntx * nty * ntz * 4 < (1LL << 31);
+}
+
+// 0: parts per tile chosen by bwd_sym_parts; 1-4: forced (gnlm_set_backward_parts)
+std::atomic<int> g_bwd_parts{0};
+
 template <int RP, int RS, bool SYM>
 hipError_t bwd_rows_t(const float *u, const float *y, const float *Z, const float *g, int nx, int ny, int nz,
                       float h, float *gu, float *gh, hipStream_t s)
 {
     using C = BwdRowsCfg<RP, RS, SYM>;
+    if (!bwd_rows_fits_t<RP, RS, SYM>(nx, ny, nz)) return hipErrorNotSupported;
     BwdRowsArgs a;
     a.nx = nx;
     a.ny = ny;
@@ -526,8 +543,6 @@ hipError_t bwd_rows_t(const float *u, const float *y, const float *Z, const floa
     const long long npad = (long long)px * py * pz;
     const long long ext = (nx + 2LL * C::R) * (ny + 2LL * C::R) * (nz + 2LL * C::R);
     const int nblocks = a.ntx * a.nty * a.ntz;
-    // 32-bit byte offsets into the (G, GY, u~, 0) copy (16 bytes per position)
-    if (16 * npad >= (1LL << 32)) return hipErrorInvalidValue;
     const int P = C::W * C::W * C::W;
     const double c = 1.0 / ((double)P * (double)h * (double)h);
     a.k = (float)weight_scale(P, h);
@@ -544,6 +559,8 @@ hipError_t bwd_rows_t(const float *u, const float *y, const float *Z, const floa
                                                          lds) == hipSuccess &&
             per_cu > 0)
             a.nparts = bwd_sym_parts(nblocks, (long long)per_cu * device_cus(), NRW);
+        const int forced = g_bwd_parts.load(std::memory_order_relaxed);
+        if (forced > 0) a.nparts = std::min(forced, NRW);
     }
     const int nunits = nblocks * a.nparts;
     float *ws = nullptr;
@@ -605,6 +622,24 @@ bool bwd3d_rows_supported(int r_p, int r_s)
         return bwd_rows_lds<RP, RS, false>() <= kLdsLimitBytes && bwd_rows_lds<RP, RS, true>() <= kLdsLimitBytes;
     GNLM_BWD_ROWS_CASES(GNLM_BR_SUP)
 #undef GNLM_BR_SUP
+    return false;
+}
+
+int bwd3d_set_parts(int nparts)
+{
+    if (nparts < 0 || nparts > 4) return -1;
+    g_bwd_parts.store(nparts, std::memory_order_relaxed);
+    return 0;
+}
+
+bool bwd3d_rows_symmetric_form() { return g_bwd_form.load(std::memory_order_relaxed) != kBwdGather; }
+
+bool bwd3d_rows_fits(int nx, int ny, int nz, int r_p, int r_s, bool sym)
+{
+#define GNLM_BR_FITS(RP, RS) \
+    if (r_p == RP && r_s == RS) return sym ? bwd_rows_fits_t<RP, RS, true>(nx, ny, nz) : bwd_rows_fits_t<RP, RS, false>(nx, ny, nz);
+    GNLM_BWD_ROWS_CASES(GNLM_BR_FITS)
+#undef GNLM_BR_FITS
     return false;
 }
 

@@ -168,9 +168,43 @@ hipError_t fwd3d_tiled_launch_variant(const InDesc &in, float *out, float *Zout,
     }
 }
 
-hipError_t fwd3d_tiled_launch(const InDesc &in, float *out, float *Zout, int r_p, int r_s, float k, hipStream_t s)
+bool fwd3d_rows_fits(int nx, int ny, int nz, int r_p, int r_s, int variant)
 {
-    return fwd3d_tiled_launch_variant(in, out, Zout, r_p, r_s, k, fwd3d_pick_variant(r_p, r_s), s);
+    // the padded copy as rows_launch_cfg lays it out: whole tiles plus the
+    // halo R = r_p + r_s (x: LP = 4 ceil(R / 4)), below 2^30 floats, and the
+    // volume below 2^31 voxels (32-bit offsets in the kernels)
+    const long long tx = fwd3_tx(variant, r_p), ty = fwd3_ty(variant, r_p), tz = fwd3_tz(variant, r_p);
+    const long long R = r_p + r_s, LP = 4 * ((R + 3) / 4);
+    const long long px = (nx + tx - 1) / tx * tx + 2 * LP, py = (ny + ty - 1) / ty * ty + 2 * R,
+                    pz = (nz + tz - 1) / tz * tz + 2 * R;
+    return px * py * pz < (1LL << 30) && (long long)nx * ny * nz < (1LL << 31);
+}
+
+int fwd3d_route(int nx, int ny, int nz, int r_p, int r_s, int path, int *kernel)
+{
+    if (path == GNLM_PATH_GENERIC || !fwd3d_tiled_supported(r_p, r_s)) {
+        if (path == GNLM_PATH_TILED) return GNLM_EUNSUPPORTED;
+        *kernel = GNLM_KERNEL_GENERIC;
+        return GNLM_OK;
+    }
+    int v = fwd3d_pick_variant(r_p, r_s);
+    if ((fwd3_is_rows(v) || v == kVRowsBig) && !fwd3d_rows_fits(nx, ny, nz, r_p, r_s, v)) {
+        // beyond the padded copy's 32-bit offsets: the tiles that read the
+        // input in place and index in 64 bits, else the per-offset passes
+        v = -1;
+        for (int c : {(int)kVSlab2, (int)kVSlab, (int)kV16x16x8, (int)kV16x16x16})
+            if (fwd3d_variant_supported(r_p, r_s, c)) {
+                v = c;
+                break;
+            }
+        if (v < 0) {
+            if (path == GNLM_PATH_TILED) return GNLM_EUNSUPPORTED;
+            *kernel = GNLM_KERNEL_GENERIC;
+            return GNLM_OK;
+        }
+    }
+    *kernel = v;
+    return GNLM_OK;
 }
 
 }  // namespace gnlm

@@ -81,6 +81,12 @@ def _extend(L):
                  "gnlm_num_variants", "gnlm_set_backward_form"):
         getattr(L, name).restype = i
     L.gnlm_set_backward_form.argtypes = [i]
+    L.gnlm_set_backward_parts.argtypes = [i]
+    L.gnlm_set_backward_parts.restype = i
+    L.gnlm3d_forward_route.argtypes = [i, i, i, i, i, i, ctypes.POINTER(i)]
+    L.gnlm3d_forward_route.restype = i
+    L.gnlm3d_backward_route.argtypes = [i, i, i, i, i, ctypes.POINTER(i)]
+    L.gnlm3d_backward_route.restype = i
 
 
 class BatchVolume(ctypes.Structure):
@@ -99,6 +105,13 @@ class BatchPlan:
     def __init__(self, vols):
         arr = (BatchVolume * len(vols))()
         self._keep = []
+        if len(vols) == 0:
+            raise ValueError("BatchPlan needs at least one volume")
+        for i, (u, out, rp, rs, h) in enumerate(vols):
+            _check_vol(u, 3, f"vols[{i}].u")
+            _check_like(out, u, f"vols[{i}].out", f"vols[{i}].u")
+        _check_dev([t for u, out, *_ in vols for t in (u, out)],
+                   [f"vols[{i}].{n}" for i in range(len(vols)) for n in ("u", "out")])
         for i, (u, out, rp, rs, h) in enumerate(vols):
             nz, ny, nx = u.shape
             arr[i] = BatchVolume(_dev_f32(u, "u").value, _dev_f32(out, "out").value, None, nx, ny, nz,
@@ -158,11 +171,33 @@ def reset_tuning():
  VARIANT_ROWS_BIG) = 0, 1, 2, 3, 4, 5, 6
 
 BWD_AUTO, BWD_GATHER, BWD_SYMMETRIC = 0, 1, 2
+KERNEL_GENERIC, KERNEL_BWD_ROWS_GATHER, KERNEL_BWD_ROWS_SYMMETRIC, KERNEL_BWD_BLOCK = 100, 201, 202, 203
+
+
+def forward_route(nx, ny, nz, r_p, r_s, path=PATH_AUTO):
+    """(status, kernel) a 3D forward of this geometry takes (gnlm3d_forward_route;
+    host logic only): kernel is the tile variant 0-6 or KERNEL_GENERIC."""
+    k = ctypes.c_int(-1)
+    st = lib().gnlm3d_forward_route(int(nx), int(ny), int(nz), int(r_p), int(r_s), int(path), ctypes.byref(k))
+    return st, k.value
+
+
+def backward_route(nx, ny, nz, r_p, r_s):
+    """(status, kernel) of the 3D adjoint (gnlm3d_backward_route; host logic only)."""
+    k = ctypes.c_int(-1)
+    st = lib().gnlm3d_backward_route(int(nx), int(ny), int(nz), int(r_p), int(r_s), ctypes.byref(k))
+    return st, k.value
 
 
 def set_backward_form(form):
     """Process-wide form of the rows-organised adjoint, r_p 0-2 (gnlm_set_backward_form)."""
     check(lib().gnlm_set_backward_form(int(form)), "gnlm_set_backward_form")
+
+
+def set_backward_parts(nparts):
+    """Workgroups per tile of the symmetric adjoint: 0 = automatic, 1-4 forced
+    (gnlm_set_backward_parts)."""
+    check(lib().gnlm_set_backward_parts(int(nparts)), "gnlm_set_backward_parts")
 
 
 def check(status, where=""):
@@ -182,9 +217,36 @@ def _dev_f32(t, name):
     return ctypes.c_void_p(t.data_ptr())
 
 
+def _check_vol(u, ndim, name="u"):
+    """u must be a non-empty ndim-D tensor (checked first, so that a bad shape
+    is reported whatever the device), then a contiguous float32 CUDA tensor."""
+    shape = tuple(getattr(u, "shape", ()))
+    if len(shape) != ndim or 0 in shape:
+        raise ValueError(f"{name} must be a non-empty {ndim}-D tensor, got shape {shape}")
+
+
+def _check_like(t, u, name, uname="u"):
+    """t must have u's shape: the kernels read or write exactly u.numel()
+    elements through it."""
+    if tuple(getattr(t, "shape", ())) != tuple(u.shape):
+        raise ValueError(f"{name} must have {uname}'s shape {tuple(u.shape)}, got {tuple(getattr(t, 'shape', ()))}")
+
+
+def _check_dev(ts, names):
+    """Every tensor is a contiguous float32 CUDA tensor on the first one's device."""
+    for t, name in zip(ts, names):
+        _dev_f32(t, name)
+        if t.device != ts[0].device:
+            raise ValueError(f"{name} is on {t.device}, {names[0]} on {ts[0].device}")
+
+
 def nlm3d(u, r_p, r_s, h, want_Z=False, path=PATH_AUTO, out=None):
     """3D NLM of u[z, y, x] (float32 CUDA tensor), reflect-101 borders."""
     import torch
+    _check_vol(u, 3)
+    if out is not None:
+        _check_like(out, u, "out")
+    _check_dev([u] + ([out] if out is not None else []), ["u", "out"])
     nz, ny, nx = u.shape
     y = torch.empty_like(u) if out is None else out
     Z = torch.empty_like(u) if want_Z else None
@@ -198,6 +260,8 @@ def nlm3d(u, r_p, r_s, h, want_Z=False, path=PATH_AUTO, out=None):
 def nlm2d(u, r_p, r_s, h, want_Z=False, path=PATH_AUTO):
     """2D NLM of u[y, x] (float32 CUDA tensor), reflect-101 borders."""
     import torch
+    _check_vol(u, 2)
+    _check_dev([u], ["u"])
     ny, nx = u.shape
     y = torch.empty_like(u)
     Z = torch.empty_like(u) if want_Z else None
@@ -211,6 +275,10 @@ def nlm2d(u, r_p, r_s, h, want_Z=False, path=PATH_AUTO):
 def nlm3d_backward(u, y, Z, gy, r_p, r_s, h):
     """Adjoint of nlm3d: returns (dL/du, dL/dh) as (tensor like u, 0-dim tensor)."""
     import torch
+    _check_vol(u, 3)
+    for t, name in ((y, "y"), (Z, "Z"), (gy, "gy")):
+        _check_like(t, u, name)
+    _check_dev([u, y, Z, gy], ["u", "y", "Z", "gy"])
     nz, ny, nx = u.shape
     gu = torch.empty_like(u)
     gh = torch.empty((), dtype=torch.float32, device=u.device)
@@ -224,6 +292,13 @@ def nlm3d_backward(u, y, Z, gy, r_p, r_s, h):
 def nlm3d_padded(up, pad, nx, ny, nz, r_p, r_s, h, path=PATH_AUTO):
     """3D NLM of a pre-padded volume up[(nz+2pad), (ny+2pad), (nx+2pad)]."""
     import torch
+    nx, ny, nz, pad = int(nx), int(ny), int(nz), int(pad)
+    if min(nx, ny, nz) < 1 or pad < 0:
+        raise ValueError(f"bad dims ({nx}, {ny}, {nz}) or pad {pad}")
+    need = (nz + 2 * pad) * (ny + 2 * pad) * (nx + 2 * pad)
+    if up.numel() < need:
+        raise ValueError(f"up holds {up.numel()} elements; the padded extent needs {need}")
+    _check_dev([up], ["up"])
     y = torch.empty((nz, ny, nx), dtype=torch.float32, device=up.device)
     st = lib().gnlm3d_forward_ex(_dev_f32(up, "up"), int(pad), 0, 0, _dev_f32(y, "out"), None,
                                  nx, ny, nz, int(r_p), int(r_s), float(h), int(path), _stream(up))
@@ -274,3 +349,10 @@ def phantom2d(nx, ny, n_shapes=40, size_lo=0.008, size_hi=0.125, sine_amp=0.2, p
 def hash_f32(a):
     a = np.ascontiguousarray(a, dtype=np.float32)
     return int(lib().gnlm_hash_f32(a.ctypes.data, a.size))
+
+
+def nlm3d_autograd(u, h, r_p, r_s):
+    """Differentiable 3D NLM (torch.autograd), gradients to u and h; see
+    gnlm.autograd for where h should live."""
+    from .autograd import nlm3d_autograd as f
+    return f(u, h, r_p, r_s)

@@ -64,13 +64,14 @@ def _traffic_from_profiles(name):
         return None
 
 
-def cpu_sample(nx, ny, nz, rp, rs, planes=0, budget_s=15.0):
+def cpu_sample(nx, ny, nz, rp, rs, planes=0, budget_s=4.0):
     """The bounded CPU-baseline sample of a 3D workload.
 
     Cost of the separable CPU path on 16 threads: about 7e-12 s x (x y voxels)
     x (planes + 2 r_p) x offsets x (2 r_p + 1)^2.4 (256 planes of the headline
     volume, 256^2 s11 p5, take 9-13 s; 2 planes of 512^2 s13 p17 took 81 s).
-    Takes as many centred z-planes as budget_s allows (or `planes` if > 0).
+    Takes as many centred z-planes as budget_s (per timed run; the baseline
+    is the median of three) allows, or `planes` if > 0.
     When even one plane of the full x-y extent would take longer, it takes one
     plane of a centred x-y window instead; the CPU path then runs on the
     window grown by R = r_p + r_s in every axis (clipped to the volume: a
@@ -99,10 +100,67 @@ def cpu_sample(nx, ny, nz, rp, rs, planes=0, budget_s=15.0):
 
 
 def _threads():
+    """Threads the CPU leg uses: OMP_NUM_THREADS if set, else the affinity count."""
+    try:
+        n = int(os.environ.get("OMP_NUM_THREADS", "0"))
+    except ValueError:
+        n = 0
+    if n > 0:
+        return n
     try:
         return len(os.sched_getaffinity(0))
     except AttributeError:
         return os.cpu_count() or 1
+
+
+def _cpu_info():
+    """(model name, physical cores) of this host, from /proc/cpuinfo: distinct
+    (physical id, core id) pairs among the CPUs this process may run on."""
+    model, cores, cur = None, set(), {}
+    try:
+        allowed = os.sched_getaffinity(0)
+    except AttributeError:
+        allowed = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f.read().splitlines() + [""]:
+                if not line.strip():
+                    if cur and (allowed is None or int(cur.get("processor", -1)) in allowed):
+                        cores.add((cur.get("physical id", "0"), cur.get("core id", cur.get("processor"))))
+                    cur = {}
+                    continue
+                k, _, v = line.partition(":")
+                cur[k.strip()] = v.strip()
+                if k.strip() == "model name" and model is None:
+                    model = v.strip()
+    except OSError:
+        pass
+    return model, (len(cores) or None)
+
+
+def _median_time(fn, runs=3):
+    """Median wall time of `runs` calls of fn (the last call's result kept)."""
+    ts, out = [], None
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        out = fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)), ts, out
+
+
+def batch_summary(r):
+    """The batched-workload object carried on the headline line."""
+    return {"workload": "nlm3d_batch64", "ratio": r["value"], "plain_ms": r["plain_ms"],
+            "batched_ms": r["batched_ms"], "bit_identical": r["bit_identical"], "steps": r["steps"],
+            "warmup": r["warmup"], "tiles": r["config"]["tiles"], "workgroups": r["config"]["workgroups"],
+            "target": ">= 1.5x per-volume launches, identical outputs"}
+
+
+def grad_summary(r):
+    """The differentiable-workload object carried on the headline line."""
+    return {"workload": "nlm3d_grad_8x128", "ratio": r["value"], "fwd_ms": r["fwd_ms"],
+            "fwd_bwd_ms": r["fwd_bwd_ms"], "steps": r["steps"], "warmup": r["warmup"], "fit_h": r["fit_h"],
+            "target": "fwd+bwd <= 3x fwd"}
 
 
 def bench_batch(args):
@@ -335,6 +393,9 @@ def main():
                     help="z-planes in the CPU baseline sample (0: as many as about 15 s of CPU work on 16 cores "
                          "allows -- all 256 planes of the headline volume)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="headline only: skip the batched and differentiable workloads the default run "
+                         "appends to the line (for profiler runs)")
     ap.add_argument("--pmc-pass", action="store_true",
                     help="one launch of the workload plus a 256 MiB calibration copy (for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -420,69 +481,82 @@ def main():
     parity = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle
+        from nlm_metrics import phantom_parity
         yc = y.cpu().numpy()
         threads = _threads()
+        model, phys = _cpu_info()
         if nz > 1:
             (z0, nplanes, y0, wy, x0, wx), (cz0, cz1, cy0, cy1, cx0, cx1) = cpu_sample(nx, ny, nz, rp, rs,
                                                                                       args.cpu_planes)
             R = rp + rs
             crop = np.ascontiguousarray(u[cz0:cz1, cy0:cy1, cx0:cx1])
             win = (slice(None), slice(y0 - cy0, y0 - cy0 + wy), slice(x0 - cx0, x0 - cx0 + wx))
-            tc = time.perf_counter()
-            ref = oracle.nlm3d_separable_f32(crop, rp, rs, h, z0=z0 - cz0, z1=z0 - cz0 + nplanes,
-                                             nthreads=threads)[win]
-            tcpu = time.perf_counter() - tc
+            tcpu, tall, full32 = _median_time(lambda: oracle.nlm3d_separable_f32(
+                crop, rp, rs, h, z0=z0 - cz0, z1=z0 - cz0 + nplanes, nthreads=threads))
+            ref = full32[win]
             got = yc[z0:z0 + nplanes, y0:y0 + wy, x0:x0 + wx]
             svox = nplanes * wx * wy
             sample = (f"z-planes [{z0},{z0 + nplanes}) of the same volume ({svox} voxels)" if (wx, wy) == (nx, ny)
                       else f"the {wx} x {wy} window at ({x0}, {y0}) of z-plane {z0} ({svox} voxels; the CPU "
                       f"path runs on the window grown by {R})")
         else:
-            tc = time.perf_counter()
-            ref = oracle.nlm2d_separable_f32(u, rp, rs, h, nthreads=threads)
-            tcpu = time.perf_counter() - tc
+            tcpu, tall, ref = _median_time(lambda: oracle.nlm2d_separable_f32(u, rp, rs, h, nthreads=threads))
             got = yc
             sample = "the whole image"
             svox = nvox
         cpu = {"value": round(svox / tcpu / 1e6, 3), "unit": unit_vox, "cores": threads, "kind": "port",
-               "sample": sample, "seconds": round(tcpu, 3)}
+               "sample": sample, "seconds": round(tcpu, 3), "runs_s": [round(t, 3) for t in tall],
+               "statistic": "median of 3 timed runs",
+               "cpu_model": model, "physical_cores": phys,
+               "path": "the oracle's separable FP32 path (OpenMP), the CPU restatement of the reference's math"}
+        # the brute-force definition on a bounded sample of the 96x96x48
+        # configuration (patch 3^3, search 7^3), reported for that config only
+        if nz > 1:
+            ub, _ = gnlm.phantom3d(96, 96, 48, n_ellipsoids=12, axis_lo=4.0, axis_hi=20.0, background=0.05,
+                                   sigma=0.05, seed=20261204)
+            zb0, zb1 = 18, 30
+            crop_b = np.ascontiguousarray(ub[zb0 - 4:zb1 + 4])
+            tb, _, _ = _median_time(lambda: oracle.nlm3d_brute(crop_b, 1, 3, 0.05), runs=1)
+            cpu["brute_force_config1"] = {
+                "value": round(96 * 96 * (zb1 - zb0 + 8) / tb / 1e6, 4), "unit": "Mvoxels/s",
+                "sample": f"planes [{zb0 - 4},{zb1 + 4}) of the 96x96x48 phantom (r_p 1, r_s 3, h 0.05), "
+                          "the FP32 brute-force definition on OpenMP threads", "seconds": round(tb, 3)}
         d = np.abs(got.astype(np.float64) - ref)
         floor = 1e-3 * float(ref.max() - ref.min())
         parity = {"max_abs_vs_cpu_fp32": float(d.max()),
                   "max_rel_vs_cpu_fp32": float((d / np.maximum(np.abs(ref), 1e-6)).max()),
                   "max_rel_floor_1e-3_range": float((d / np.maximum(np.abs(ref), floor)).max())}
-        # the FP64 re-run of the same definition (separable FP64 path) on a
-        # slab of the sample (at most 16 planes) or the whole image
-        # the FP64 re-run of the same definition on part of the sample: 3D, the
-        # separable FP64 path on (at most) 16 planes; 2D, FP64 brute force on a
-        # 128^2 window (computed on the window grown by R = r_p + r_s, whose
-        # interior sees exactly the samples the full image gives it)
+        # the FP64 re-run of the same definition, with the normaliser Z and the
+        # magnitude sum A: 3D, the separable FP64 path on (at most) 16 planes
+        # of the sample; 2D, the separable FP64 path on the whole image
         if nz > 1:
             n64 = min(16, nplanes)
-            ref64 = oracle.nlm3d_separable_f64(crop, rp, rs, h, z0=z0 - cz0, z1=z0 - cz0 + n64, nthreads=threads)[win]
-            got64 = got[:n64]
+            ref64, Z64, A64 = oracle.nlm3d_separable_f64_za(crop, rp, rs, h, z0=z0 - cz0, z1=z0 - cz0 + n64,
+                                                            nthreads=threads)
+            ref64, Z64, A64 = ref64[win], Z64[win], A64[win]
+            got64, ref32_64 = got[:n64], ref[:n64]
             what = (f"{n64} planes from z = {z0}" if (wx, wy) == (nx, ny)
                     else f"the same {wx} x {wy} window of z-plane {z0}")
         else:
-            R, m = rp + rs, 128
-            y0, x0 = max(0, ny // 2 - m // 2), max(0, nx // 2 - m // 2)
-            ya, xa = max(0, y0 - R), max(0, x0 - R)
-            crop = u[ya:min(ny, y0 + m + R), xa:min(nx, x0 + m + R)].astype(np.float64)
-            full = oracle.nlm2d_brute(crop, rp, rs, h)
-            ref64 = full[y0 - ya:y0 - ya + m, x0 - xa:x0 - xa + m]
-            got64 = yc[y0:y0 + m, x0:x0 + m]
-            what = f"the {m}^2 window at ({x0}, {y0})"
+            ref64, Z64, A64 = oracle.nlm2d_separable_f64(u, rp, rs, h, nthreads=threads)
+            got64, ref32_64 = yc, ref
+            what = "the whole image"
         d64 = np.abs(got64.astype(np.float64) - ref64)
         parity["max_rel_vs_fp64"] = float((d64 / np.maximum(np.abs(ref64), 1e-6)).max())
         parity["max_abs_vs_fp64"] = float(d64.max())
         parity["fp64_sample"] = what
-        if nz > 1:
-            # the CPU FP32 path against the same FP64 slab: the scale of FP32
-            # rounding on this input (near-zero outputs dominate the 1e-6 floor)
-            dc = np.abs(ref[:n64].astype(np.float64) - ref64)
-            parity["cpu_fp32_max_rel_vs_fp64"] = float((dc / np.maximum(np.abs(ref64), 1e-6)).max())
-            parity["cpu_fp32_max_abs_vs_fp64"] = float(dc.max())
+        rep = phantom_parity(got64, ref32_64, ref64, Z64, A64)
+        parity["bar"] = ("unfloored rel <= 1e-5 vs CPU FP32 and FP64 where |y| >= 0.5% of the range; "
+                         "|gpu - fp64| <= 32 * 2^-24 * A/Z everywhere")
+        parity.update({k: rep[k] for k in ("tau", "voxels_below_tau", "max_rel_vs_fp32_above_tau",
+                                           "max_rel_vs_fp64_above_tau", "max_err_in_fp32_units_of_A_over_Z",
+                                           "voxels_unfloored_rel_over_tol", "ok")})
+        dc = np.abs(ref32_64.astype(np.float64) - ref64)
+        parity["cpu_fp32_max_rel_vs_fp64"] = float((dc / np.maximum(np.abs(ref64), 1e-6)).max())
+        parity["cpu_fp32_max_abs_vs_fp64"] = float(dc.max())
+        parity["cpu_fp32_err_in_fp32_units_of_A_over_Z"] = float((dc / (2.0 ** -24 * A64 / Z64)).max())
 
     line = {
         "metric": ((f"Mvoxels/s 3D NLM {nx}^3" if nx == ny == nz else f"Mvoxels/s 3D NLM {nx}x{ny}x{nz}") +
@@ -505,6 +579,13 @@ def main():
         "cpu_baseline": cpu,
         "parity": parity,
     }
+    if args.workload == "nlm3d_256_p5_s11" and not args.no_extras:
+        # the north star's other two targets, on the same line: the batched
+        # persistent launch (64 uneven volumes) and fwd+bwd vs fwd (8 x 128^3)
+        sub = argparse.Namespace(**vars(args))
+        sub.steps, sub.warmup = 3, 1
+        line["batched"] = batch_summary(bench_batch(sub))
+        line["grad"] = grad_summary(bench_grad(sub))
     print(json.dumps(line))
 
 

@@ -74,6 +74,34 @@ int gnlm3d_forward_ex(const float *d_in, int in_pad, long long in_sy, long long 
                       float *d_Z, int nx, int ny, int nz, int r_p, int r_s, float h, int path,
                       void *stream);
 
+/* Which kernel a call would run, and whether it can run at all: pure host
+ * logic, no device work and no allocation (no counterpart in the reference,
+ * whose launcher runs one fixed kernel set per compile-time geometry,
+ * 3DNLM/main.cu:7622-7667).  gnlm3d_forward_ex and gnlm3d_backward take
+ * exactly this route and return the same status before touching the device.
+ *   forward: *kernel = the tile variant (0-6, see gnlm_autotune) or
+ *            GNLM_KERNEL_GENERIC.  The table's variant is used unless it is a
+ *            rows-family variant (4-6) whose padded copy would pass 2^30
+ *            floats (32-bit offsets); then the first of variants 3, 2, 1, 0
+ *            that fits the radii (they read the input in place with 64-bit
+ *            offsets), else the generic per-offset path.  GNLM_EUNSUPPORTED
+ *            when path = GNLM_PATH_TILED and no tile variant can run it, or
+ *            beyond 2^40 voxels.
+ *   backward: *kernel = GNLM_KERNEL_BWD_ROWS_* (the rows adjoint in the
+ *            selected form) while its 16-byte-per-position copy stays below
+ *            4 GiB, else GNLM_KERNEL_BWD_BLOCK (the LDS-block adjoint, 64-bit
+ *            indexing) where the radii fit its LDS image; GNLM_EUNSUPPORTED
+ *            otherwise.
+ * GNLM_EINVAL for bad dims, radii, path or a NULL kernel pointer. */
+enum {
+    GNLM_KERNEL_GENERIC = 100,
+    GNLM_KERNEL_BWD_ROWS_GATHER = 201,
+    GNLM_KERNEL_BWD_ROWS_SYMMETRIC = 202,
+    GNLM_KERNEL_BWD_BLOCK = 203
+};
+int gnlm3d_forward_route(int nx, int ny, int nz, int r_p, int r_s, int path, int *kernel);
+int gnlm3d_backward_route(int nx, int ny, int nz, int r_p, int r_s, int *kernel);
+
 /* 2D forward on an nx * ny image (x fastest).
  * Replaces the kernel launches of "2D NLM/11x11/main.cu:486-525". */
 int gnlm2d_forward(const float *d_in, float *d_out, int nx, int ny, int r_p, int r_s, float h,
@@ -95,7 +123,9 @@ int gnlm2d_forward_ex(const float *d_in, int in_pad, long long in_sy, float *d_o
  * workspace released on `stream`.  Supported: r_p 0, 1, 2 with r_s <= 7 (the
  * rows-organised kernels, in the form gnlm_set_backward_form selects; larger
  * r_s as the LDS image of the fallback tile allows) and r_p = 3 with r_s <= 3
- * (the fallback tile); GNLM_EUNSUPPORTED otherwise. */
+ * (the fallback tile); volumes too large for the rows kernels' 32-bit
+ * offsets run on the fallback tile where the radii fit it
+ * (gnlm3d_backward_route); GNLM_EUNSUPPORTED otherwise. */
 int gnlm3d_backward(const float *d_in, const float *d_out, const float *d_Z, const float *d_gout, int nx,
                     int ny, int nz, int r_p, int r_s, float h, float *d_gin, float *d_gh, void *stream);
 
@@ -109,6 +139,17 @@ int gnlm3d_backward(const float *d_in, const float *d_out, const float *d_Z, con
  * unknown form. */
 enum { GNLM_BWD_AUTO = 0, GNLM_BWD_GATHER = 1, GNLM_BWD_SYMMETRIC = 2 };
 int gnlm_set_backward_form(int form);
+
+/* Workgroups per tile of the symmetric adjoint (no counterpart in the
+ * reference), process-wide: 0 (default) lets the launch choose 1-4 from the
+ * tile count and the device's occupancy, so that the last round of tiles is
+ * not mostly idle; 1-4 forces that count (capped by the tile's offset rows).
+ * Each part sums its share of the offsets and the parts are added in a fixed
+ * order, so the gradient is deterministic for a given device and volume, but
+ * its last bits depend on the part count -- and hence, under the default, on
+ * the device's compute-unit count and the volume size.  GNLM_EINVAL outside
+ * 0-4. */
+int gnlm_set_backward_parts(int nparts);
 
 /* ---- batched persistent forward --------------------------------------
  * No counterpart in the reference (its launcher processes one compile-time

@@ -337,10 +337,26 @@ int oracle_nlm3d_separable_f32(const float *u, int nx, int ny, int nz,
 }
 
 /* FP64 separable version (same structure, plain double sums): used for
- * full-size FP64 re-runs where the brute force would take too long. */
+ * full-size FP64 re-runs where the brute force would take too long.
+ * Zout and Aout (nullable, planes [z0, z1) like y) receive the normaliser
+ * Z(x) = sum_d w_d(x) and A(x) = sum_d w_d(x) |u~(x+d)|; A/Z is the scale of
+ * the rounding error of any FP32 evaluation of y = sum_d w_d u~(x+d) / Z (the
+ * sum of the magnitudes of its terms), which the parity tests use to bound
+ * the error at outputs near zero. */
+int oracle_nlm3d_separable_f64_za(const float *u, int nx, int ny, int nz,
+                                  int rp, int rs, double h, double *y, double *Zout, double *Aout,
+                                  int z0, int z1, int nthreads);
+
 int oracle_nlm3d_separable_f64(const float *u, int nx, int ny, int nz,
                                int rp, int rs, double h, double *y,
                                int z0, int z1, int nthreads)
+{
+    return oracle_nlm3d_separable_f64_za(u, nx, ny, nz, rp, rs, h, y, NULL, NULL, z0, z1, nthreads);
+}
+
+int oracle_nlm3d_separable_f64_za(const float *u, int nx, int ny, int nz,
+                                  int rp, int rs, double h, double *y, double *Zout, double *Aout,
+                                  int z0, int z1, int nthreads)
 {
     const int R = rp + rs;
     const long ex = nx + 2L * R, ey = ny + 2L * R, ez = nz + 2L * R;
@@ -365,7 +381,8 @@ int oracle_nlm3d_separable_f64(const float *u, int nx, int ny, int nz,
         double *sxy = (double *)malloc(sizeof(double) * wx * ny * wz);
         double *Zs = (double *)malloc(sizeof(double) * (long)nx * ny * SLAB);
         double *Ys = (double *)malloc(sizeof(double) * (long)nx * ny * SLAB);
-        if (!sx || !sxy || !Zs || !Ys) {
+        double *As = (double *)malloc(sizeof(double) * (long)nx * ny * SLAB);
+        if (!sx || !sxy || !Zs || !Ys || !As) {
 #pragma omp atomic write
             fail = 1;
         } else {
@@ -378,6 +395,7 @@ int oracle_nlm3d_separable_f64(const float *u, int nx, int ny, int nz,
                 const int tz = zb - za;
                 memset(Zs, 0, sizeof(double) * (long)nx * ny * SLAB);
                 memset(Ys, 0, sizeof(double) * (long)nx * ny * SLAB);
+                memset(As, 0, sizeof(double) * (long)nx * ny * SLAB);
                 for (int dz = -rs; dz <= rs; ++dz)
                     for (int dy = -rs; dy <= rs; ++dy)
                         for (int dx = -rs; dx <= rs; ++dx) {
@@ -413,6 +431,7 @@ int oracle_nlm3d_separable_f64(const float *u, int nx, int ny, int nz,
                                     const double *cand = pu + ex * (jy + R + dy + ey * (za + kz + R + dz)) + R + dx;
                                     double *Zr = Zs + (long)nx * (jy + (long)ny * kz);
                                     double *Yr = Ys + (long)nx * (jy + (long)ny * kz);
+                                    double *Ar = As + (long)nx * (jy + (long)ny * kz);
                                     for (int x = 0; x < nx; ++x) {
                                         double S = 0.0;
                                         for (int pz = 0; pz <= 2 * rp; ++pz)
@@ -420,6 +439,7 @@ int oracle_nlm3d_separable_f64(const float *u, int nx, int ny, int nz,
                                         double w = exp(-(S / P) * inv);
                                         Zr[x] += w;
                                         Yr[x] += w * cand[x];
+                                        Ar[x] += w * fabs(cand[x]);
                                     }
                                 }
                         }
@@ -427,11 +447,100 @@ int oracle_nlm3d_separable_f64(const float *u, int nx, int ny, int nz,
                     for (int jy = 0; jy < ny; ++jy)
                         for (int x = 0; x < nx; ++x) {
                             long i = x + (long)nx * (jy + (long)ny * kz);
-                            y[x + (long)nx * (jy + (long)ny * (za + kz - z0))] = Ys[i] / Zs[i];
+                            const long o = x + (long)nx * (jy + (long)ny * (za + kz - z0));
+                            y[o] = Ys[i] / Zs[i];
+                            if (Zout) Zout[o] = Zs[i];
+                            if (Aout) Aout[o] = As[i];
                         }
             }
         }
-        free(sx); free(sxy); free(Zs); free(Ys);
+        free(sx); free(sxy); free(Zs); free(Ys); free(As);
+    }
+    free(pu);
+    return fail ? -1 : 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* 2D separable FP64 (plain double sums), OpenMP over rows; Zout, Aout  */
+/* (nullable) as in oracle_nlm3d_separable_f64_za.                      */
+/* ------------------------------------------------------------------ */
+int oracle_nlm2d_separable_f64(const float *u, int nx, int ny, int rp, int rs,
+                               double h, double *y, double *Zout, double *Aout, int nthreads)
+{
+    const int R = rp + rs;
+    const long ex = nx + 2L * R, ey = ny + 2L * R;
+    double *pu = (double *)malloc(sizeof(double) * ex * ey);
+    if (!pu) return -1;
+    for (long j = 0; j < ey; ++j)
+        for (long i = 0; i < ex; ++i)
+            pu[i + ex * j] = u[R101(i - R, nx) + (long)nx * R101(j - R, ny)];
+    const double P = (double)((2 * rp + 1) * (2 * rp + 1));
+    const double inv = 1.0 / (h * h);
+    const int BAND = 16;
+    const int nband = (ny + BAND - 1) / BAND;
+    int fail = 0;
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel num_threads(nthreads)
+#endif
+    {
+        double *sx = (double *)malloc(sizeof(double) * nx * (BAND + 2 * rp));
+        double *Zs = (double *)malloc(sizeof(double) * nx * BAND);
+        double *Ys = (double *)malloc(sizeof(double) * nx * BAND);
+        double *As = (double *)malloc(sizeof(double) * nx * BAND);
+        if (!sx || !Zs || !Ys || !As) {
+#pragma omp atomic write
+            fail = 1;
+        } else {
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 1)
+#endif
+            for (int b = 0; b < nband; ++b) {
+                const int ya = b * BAND;
+                const int yb = (ya + BAND < ny) ? ya + BAND : ny;
+                const int ty = yb - ya;
+                memset(Zs, 0, sizeof(double) * nx * BAND);
+                memset(Ys, 0, sizeof(double) * nx * BAND);
+                memset(As, 0, sizeof(double) * nx * BAND);
+                for (int dy = -rs; dy <= rs; ++dy)
+                    for (int dx = -rs; dx <= rs; ++dx) {
+                        for (int jy = 0; jy < ty + 2 * rp; ++jy) {
+                            const long yy = ya - rp + jy + R;
+                            const double *c = pu + ex * yy + R;
+                            const double *q = pu + ex * (yy + dy) + R + dx;
+                            double *o = sx + (long)nx * jy;
+                            for (int x = 0; x < nx; ++x) {
+                                double acc = 0.0;
+                                for (int px = -rp; px <= rp; ++px) {
+                                    double e = c[x + px] - q[x + px];
+                                    acc += e * e;
+                                }
+                                o[x] = acc;
+                            }
+                        }
+                        for (int jy = 0; jy < ty; ++jy) {
+                            const double *cand = pu + ex * (ya + jy + R + dy) + R + dx;
+                            for (int x = 0; x < nx; ++x) {
+                                double S = 0.0;
+                                for (int py = 0; py <= 2 * rp; ++py) S += sx[x + (long)nx * (jy + py)];
+                                const double w = exp(-(S / P) * inv);
+                                const long i = x + (long)nx * jy;
+                                Zs[i] += w;
+                                Ys[i] += w * cand[x];
+                                As[i] += w * fabs(cand[x]);
+                            }
+                        }
+                    }
+                for (int jy = 0; jy < ty; ++jy)
+                    for (int x = 0; x < nx; ++x) {
+                        const long i = x + (long)nx * jy, o = x + (long)nx * (ya + jy);
+                        y[o] = Ys[i] / Zs[i];
+                        if (Zout) Zout[o] = Zs[i];
+                        if (Aout) Aout[o] = As[i];
+                    }
+            }
+        }
+        free(sx); free(Zs); free(Ys); free(As);
     }
     free(pu);
     return fail ? -1 : 0;

@@ -43,6 +43,12 @@ def lib():
         L.oracle_nlm3d_separable_f32.restype = _i
         L.oracle_nlm3d_separable_f64.argtypes = [_f32p, _i, _i, _i, _i, _i, ctypes.c_double, _f64p, _i, _i, _i]
         L.oracle_nlm3d_separable_f64.restype = _i
+        _vp = ctypes.c_void_p
+        L.oracle_nlm3d_separable_f64_za.argtypes = [_f32p, _i, _i, _i, _i, _i, ctypes.c_double, _f64p, _vp, _vp,
+                                                    _i, _i, _i]
+        L.oracle_nlm3d_separable_f64_za.restype = _i
+        L.oracle_nlm2d_separable_f64.argtypes = [_f32p, _i, _i, _i, _i, ctypes.c_double, _f64p, _vp, _vp, _i]
+        L.oracle_nlm2d_separable_f64.restype = _i
         L.oracle_nlm2d_separable_f32.argtypes = [_f32p, _i, _i, _i, _i, ctypes.c_float, _f32p, _i]
         L.oracle_nlm2d_separable_f32.restype = _i
         L.oracle_nlm3d_grad_f64.argtypes = [_f64p, _f64p, _i, _i, _i, _i, _i, ctypes.c_double, _f64p,
@@ -118,6 +124,36 @@ def nlm3d_separable_f64(u, rp, rs, h, z0=0, z1=None, nthreads=0):
     if rc != 0:
         raise MemoryError("oracle_nlm3d_separable_f64 failed")
     return y
+
+
+def nlm3d_separable_f64_za(u, rp, rs, h, z0=0, z1=None, nthreads=0):
+    """FP64 re-run of planes [z0, z1) with the normaliser Z = sum_d w_d and
+    A = sum_d w_d |u~(x+d)|: returns (y, Z, A), float64."""
+    u = np.ascontiguousarray(u, dtype=np.float32)
+    nz, ny, nx = u.shape
+    z1 = nz if z1 is None else z1
+    y = np.empty((z1 - z0, ny, nx), np.float64)
+    Z = np.empty_like(y)
+    A = np.empty_like(y)
+    rc = lib().oracle_nlm3d_separable_f64_za(u, nx, ny, nz, rp, rs, float(h), y, Z.ctypes.data, A.ctypes.data,
+                                             z0, z1, int(nthreads))
+    if rc != 0:
+        raise MemoryError("oracle_nlm3d_separable_f64_za failed")
+    return y, Z, A
+
+
+def nlm2d_separable_f64(u, rp, rs, h, nthreads=0):
+    """2D FP64 separable path: returns (y, Z, A) as nlm3d_separable_f64_za."""
+    u = np.ascontiguousarray(u, dtype=np.float32)
+    ny, nx = u.shape
+    y = np.empty((ny, nx), np.float64)
+    Z = np.empty_like(y)
+    A = np.empty_like(y)
+    rc = lib().oracle_nlm2d_separable_f64(u, nx, ny, rp, rs, float(h), y, Z.ctypes.data, A.ctypes.data,
+                                          int(nthreads))
+    if rc != 0:
+        raise MemoryError("oracle_nlm2d_separable_f64 failed")
+    return y, Z, A
 
 
 def nlm2d_separable_f32(u, rp, rs, h, nthreads=0):

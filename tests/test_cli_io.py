@@ -21,13 +21,13 @@ import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-CHECK = os.path.join(ROOT, "3d-gnlm_amd", "cli", "image_io_check")
+CHECK = os.path.join(ROOT, "tests", "cli_io", "image_io_check")
 NLM = os.path.join(ROOT, "3d-gnlm_amd", "cli", "nl-means")
 FIXTURE = os.path.join(ROOT, "tests", "golden", "ref_data_128.bmp")
 
 
 def _decode(path, tmp_path, rebmp=None):
-    assert os.path.exists(CHECK), "3d-gnlm_amd/cli/image_io_check not built (run __graft_entry__.build())"
+    assert os.path.exists(CHECK), "tests/cli_io/image_io_check not built (run __graft_entry__.build())"
     raw = tmp_path / "img.raw"
     args = [CHECK, str(path), str(raw)] + ([str(rebmp)] if rebmp else [])
     r = subprocess.run(args, capture_output=True, text=True, timeout=60)

@@ -101,6 +101,36 @@ def test_backward_forms_agree_three_parts(dev):
     _forms_agree(dev, 1, 4, n=70)
 
 
+@pytest.mark.parametrize("rp,rs", [(1, 3), (2, 2), (0, 4)])
+def test_backward_forced_part_counts(dev, rp, rs):
+    """Each tile's offset rows split over np = 1, 2, 3 and 4 workgroups
+    (forced, whatever the device's compute-unit count): every split agrees
+    with the gather form and with the FP64 chain-rule adjoint, and each is
+    deterministic; the part count changes only the summation order."""
+    rng = np.random.default_rng(40 + rp)
+    u = rng.random((30, 28, 26)).astype(np.float32)
+    g = rng.standard_normal(u.shape).astype(np.float32)
+    ref_gu, ref_gh = oracle.nlm3d_grad_f64(u.astype(np.float64), g.astype(np.float64), rp, rs, 0.2)
+    scale = np.abs(ref_gu).max()
+    try:
+        gnlm.set_backward_form(gnlm.BWD_GATHER)
+        ga = _gpu_grad(dev, u, g, rp, rs, 0.2)
+        gnlm.set_backward_form(gnlm.BWD_SYMMETRIC)
+        for np_ in (1, 2, 3, 4):
+            gnlm.set_backward_parts(np_)
+            print(f"case bwd parts={np_} rp={rp} rs={rs}", flush=True)
+            a = _gpu_grad(dev, u, g, rp, rs, 0.2)
+            b = _gpu_grad(dev, u, g, rp, rs, 0.2)
+            assert np.array_equal(a[0], b[0]) and a[1] == b[1], np_
+            assert np.abs(a[0] - ga[0]).max() <= 2e-5 * scale, np_
+            assert abs(a[1] - ga[1]) <= 2e-5 * abs(ga[1]), np_
+            assert np.abs(a[0] - ref_gu).max() <= 1e-4 * scale, np_
+            assert abs(a[1] - ref_gh) <= 1e-4 * abs(ref_gh), np_
+    finally:
+        gnlm.set_backward_parts(0)
+        gnlm.set_backward_form(gnlm.BWD_AUTO)
+
+
 def _forms_agree(dev, rp, rs, n=40):
     rng = np.random.default_rng(5 + rp)
     u = rng.random((n, n, n)).astype(np.float32)
@@ -156,11 +186,13 @@ def test_backward_vs_finite_differences_crop24(dev):
 
 
 def test_fit_h_decreases_loss(dev):
-    """A few gradient steps on h (log-parameterised) lower the MSE vs clean."""
+    """A few gradient steps on h (log-parameterised, a CPU leaf tensor) lower
+    the MSE vs clean."""
     u, clean = _crop24()
     ut = torch.from_numpy(u).to(dev)
     ct = torch.from_numpy(clean).to(dev)
-    logh = torch.tensor(np.log(0.1), dtype=torch.float32, device=dev, requires_grad=True)
+    # h lives on the host: its value reaches the C-ABI without a device wait
+    logh = torch.tensor(np.log(0.1), dtype=torch.float32, requires_grad=True)
     opt = torch.optim.Adam([logh], lr=0.05)
     losses = []
     for _ in range(15):

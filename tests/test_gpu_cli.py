@@ -24,7 +24,7 @@ pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 NLM = os.path.join(ROOT, "3d-gnlm_amd", "cli", "nl-means")
-CHECK = os.path.join(ROOT, "3d-gnlm_amd", "cli", "image_io_check")
+CHECK = os.path.join(ROOT, "tests", "cli_io", "image_io_check")
 FIXTURE = os.path.join(ROOT, "tests", "golden", "ref_data_128.bmp")
 
 

@@ -134,7 +134,38 @@ def test_fp32_paths(shape, rp, rs):
     part = oracle.nlm3d_separable_f32(u, rp, rs, 0.2, z0=1, z1=shape[0] - 1)
     assert np.array_equal(part, s32[1:shape[0] - 1])
     img = rnd((17, 19), 10, np.float32)
-    assert np.array_equal(oracle.nlm2d_brute(img, rp, rs, 0.2), oracle.nlm2d_separable_f32(img, rp, rs, 0.2))
+    i32 = oracle.nlm2d_separable_f32(img, rp, rs, 0.2)
+    assert np.array_equal(oracle.nlm2d_brute(img, rp, rs, 0.2), i32)
+    # the 2D FP32 paths against 2D FP64 (brute force and separable)
+    i64 = oracle.nlm2d_brute(img.astype(np.float64), rp, rs, 0.2)
+    assert np.max(np.abs(i32 - i64) / np.abs(i64)) < 1e-5
+    s64, _, _ = oracle.nlm2d_separable_f64(img, rp, rs, 0.2)
+    assert np.max(np.abs(s64 - i64)) < 1e-13
+
+
+@pytest.mark.parametrize("dims", [(6, 7, 8), None])
+def test_normaliser_and_magnitude_sums(dims):
+    """Z and A = sum_d w_d |u~(x+d)| of the FP64 separable paths: Z equals the
+    brute force's normaliser; for non-negative input A = y Z exactly (every
+    term is its own magnitude); A is even in u (w depends on differences
+    only) while y is odd; and A >= |y| Z for mixed signs."""
+    if dims is None:
+        u = rnd((21, 23), 12, np.float32) * 2 - 0.5
+        y, Z, A = oracle.nlm2d_separable_f64(u, 1, 3, 0.3)
+        yb, Zb = oracle.nlm2d_brute(u.astype(np.float64), 1, 3, 0.3, want_Z=True)
+        f = lambda v: oracle.nlm2d_separable_f64(v, 1, 3, 0.3)  # noqa: E731
+    else:
+        u = rnd(dims, 12, np.float32) * 2 - 0.5
+        y, Z, A = oracle.nlm3d_separable_f64_za(u, 1, 2, 0.3)
+        yb, Zb = oracle.nlm3d_brute(u.astype(np.float64), 1, 2, 0.3, want_Z=True)
+        f = lambda v: oracle.nlm3d_separable_f64_za(v, 1, 2, 0.3)  # noqa: E731
+    assert np.max(np.abs(y - yb)) < 1e-13 and np.max(np.abs(Z - Zb) / Zb) < 1e-13
+    assert (A >= np.abs(y) * Z * (1 - 1e-12)).all()
+    yn, Zn, An = f(-u)
+    assert np.max(np.abs(An - A) / A) < 1e-13 and np.max(np.abs(yn + y)) < 1e-13
+    up = np.abs(u)
+    yp, Zp, Ap = f(up)
+    assert np.max(np.abs(Ap - yp * Zp) / Ap) < 1e-13
 
 
 def _fd_loss(u, clean, rp, rs, h):

@@ -1,0 +1,92 @@
+"""Which kernel a forward or adjoint call runs, and the status it returns for
+volumes past the rows kernels' 32-bit offsets -- host logic only, checked
+without a GPU (the calls return before any allocation or launch)."""
+import ctypes
+
+import pytest
+
+import gnlm
+
+
+def _rows_fits(n, rp, rs, ty, tz):
+    """Size of the padded copy the rows forward builds (16-wide tiles in x,
+    halo R = r_p + r_s, x halo rounded up to a multiple of 4) vs 2^30 floats."""
+    R = rp + rs
+    lp = 4 * ((R + 3) // 4)
+    px = -(-n // 16) * 16 + 2 * lp
+    py = -(-n // ty) * ty + 2 * R
+    pz = -(-n // tz) * tz + 2 * R
+    return px * py * pz < 2 ** 30
+
+
+def test_headline_takes_the_rows_tiles():
+    assert gnlm.forward_route(256, 256, 256, 2, 5) == (gnlm.OK, gnlm.VARIANT_ROWS_SHORT)
+    assert gnlm.forward_route(96, 96, 48, 1, 3) == (gnlm.OK, gnlm.VARIANT_ROWS)
+    assert gnlm.forward_route(512, 512, 128, 8, 6) == (gnlm.OK, gnlm.VARIANT_ROWS_BIG)
+
+
+def test_rows_limit_boundary():
+    """The 16 x 12 x 12 tiles (r_p 2, r_s 5) of a 1000^3 volume still fit the
+    32-bit padded copy, those of 1010^3 do not: the router falls back to the
+    slab tiles, which read the input in place with 64-bit offsets."""
+    assert _rows_fits(1000, 2, 5, 12, 12) and not _rows_fits(1010, 2, 5, 12, 12)
+    assert gnlm.forward_route(1000, 1000, 1000, 2, 5) == (gnlm.OK, gnlm.VARIANT_ROWS_SHORT)
+    st, k = gnlm.forward_route(1010, 1010, 1010, 2, 5)
+    assert st == gnlm.OK and k in (gnlm.VARIANT_SLAB2, gnlm.VARIANT_SLAB, gnlm.VARIANT_16x16x8,
+                                   gnlm.VARIANT_16x16x16), k
+
+
+def test_forward_1100_cubed():
+    for rp, rs in ((1, 3), (2, 5), (3, 3)):
+        st, k = gnlm.forward_route(1100, 1100, 1100, rp, rs)
+        assert st == gnlm.OK and k in (gnlm.VARIANT_SLAB2, gnlm.VARIANT_SLAB, gnlm.VARIANT_16x16x8,
+                                       gnlm.VARIANT_16x16x16), (rp, rs, k)
+    # large patches have no 64-bit tiles: the per-offset passes take them,
+    # and a request for the tiled path alone is unsupported (not "invalid")
+    assert gnlm.forward_route(1100, 1100, 1100, 8, 6) == (gnlm.OK, gnlm.KERNEL_GENERIC)
+    st, _ = gnlm.forward_route(1100, 1100, 1100, 8, 6, gnlm.PATH_TILED)
+    assert st == gnlm.EUNSUPPORTED
+    # past 2^40 voxels nothing can hold the buffers
+    st, _ = gnlm.forward_route(1 << 14, 1 << 14, 1 << 13, 1, 3)
+    assert st == gnlm.EUNSUPPORTED
+
+
+def test_forward_call_returns_route_status_before_the_device():
+    L = gnlm.lib()
+    fake = ctypes.c_void_p(0x1000)
+    assert L.gnlm3d_forward_ex(fake, 0, 0, 0, fake, None, 1 << 14, 1 << 14, 1 << 13, 1, 3, 0.1, 0,
+                               None) == gnlm.EUNSUPPORTED
+    assert L.gnlm3d_forward_ex(fake, 0, 0, 0, fake, None, 1100, 1100, 1100, 8, 6, 0.1, gnlm.PATH_TILED,
+                               None) == gnlm.EUNSUPPORTED
+
+
+def test_backward_700_cubed():
+    """The rows adjoint's (G, GY, u~, 0) copy (16 bytes per position) passes
+    4 GiB near 620^3 at r_p 1 / r_s 4; the LDS-block adjoint takes over."""
+    assert gnlm.backward_route(128, 128, 128, 1, 4) == (gnlm.OK, gnlm.KERNEL_BWD_ROWS_SYMMETRIC)
+    assert gnlm.backward_route(700, 700, 700, 1, 4) == (gnlm.OK, gnlm.KERNEL_BWD_BLOCK)
+    # radii the LDS-block tile cannot hold: unsupported, not invalid
+    st, _ = gnlm.backward_route(700, 700, 700, 2, 7)
+    assert st == gnlm.EUNSUPPORTED
+    L = gnlm.lib()
+    fake = ctypes.c_void_p(0x1000)
+    assert L.gnlm3d_backward(fake, fake, fake, fake, 700, 700, 700, 2, 7, 0.1, fake, None,
+                             None) == gnlm.EUNSUPPORTED
+
+
+def test_backward_form_switch_routes():
+    try:
+        gnlm.set_backward_form(gnlm.BWD_GATHER)
+        assert gnlm.backward_route(64, 64, 64, 2, 3) == (gnlm.OK, gnlm.KERNEL_BWD_ROWS_GATHER)
+    finally:
+        gnlm.set_backward_form(gnlm.BWD_AUTO)
+    assert gnlm.backward_route(64, 64, 64, 2, 3) == (gnlm.OK, gnlm.KERNEL_BWD_ROWS_SYMMETRIC)
+    assert gnlm.backward_route(40, 40, 40, 3, 3) == (gnlm.OK, gnlm.KERNEL_BWD_BLOCK)
+    st, _ = gnlm.backward_route(40, 40, 40, 4, 3)
+    assert st == gnlm.EUNSUPPORTED
+
+
+@pytest.mark.parametrize("args", [(0, 8, 8, 1, 2), (8, 8, 8, -1, 2), (8, 8, 8, 1, -2)])
+def test_route_rejects_bad_geometry(args):
+    assert gnlm.forward_route(*args)[0] == gnlm.EINVAL
+    assert gnlm.backward_route(*args)[0] == gnlm.EINVAL
