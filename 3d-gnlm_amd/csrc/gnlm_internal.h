@@ -11,6 +11,15 @@ namespace gnlm {
 // Exponent scale of the weight: w = exp(-(S/P)/h^2) = exp2(S * k).
 double weight_scale(int P, float h);
 
+// Device scratch handed to a launch by a caller that must not allocate
+// (gnlm3d_forward_noalloc): kernels that need a workspace take it from here
+// instead of hipMallocAsync, or fail with hipErrorOutOfMemory when it is too
+// small.  NULL everywhere else: those launches allocate on their stream.
+struct Workspace {
+    void *p;
+    size_t bytes;
+};
+
 constexpr size_t kLdsLimitBytes = 160 * 1024;  // LDS per workgroup on gfx950
 
 // ---- tiled 3D forward (nlm3d_fwd.hip) --------------------------------------
@@ -21,7 +30,7 @@ size_t fwd3d_lds_bytes(int r_p, int r_s, int variant);
 bool fwd3d_variant_supported(int r_p, int r_s, int variant);
 bool fwd3d_tiled_supported(int r_p, int r_s);
 hipError_t fwd3d_tiled_launch_variant(const InDesc &in, float *out, float *Zout, int r_p, int r_s, float k,
-                                      int variant, hipStream_t s);
+                                      int variant, hipStream_t s, const Workspace *ws = nullptr);
 // Whether a rows-family variant's padded copy of an nx*ny*nz volume fits its
 // 32-bit offsets.
 bool fwd3d_rows_fits(int nx, int ny, int nz, int r_p, int r_s, int variant);
@@ -38,12 +47,13 @@ int fwd3d_route(int nx, int ny, int nz, int r_p, int r_s, int path, int *kernel)
 // Large patches (r_p 4-8): 16 x (32 - 2 r_p)^2 tiles on 1024 threads.
 bool fwd3d_rows_big_supported(int r_p, int r_s);
 size_t fwd3d_rows_big_lds_bytes(int r_p, int r_s);
-hipError_t fwd3d_rows_big_launch(const InDesc &in, float *out, float *Zout, int r_p, int r_s, float k, hipStream_t s);
+hipError_t fwd3d_rows_big_launch(const InDesc &in, float *out, float *Zout, int r_p, int r_s, float k, hipStream_t s,
+                                 const Workspace *ws = nullptr);
 // short_tiles selects the 16 x 12 x 12 tiles (r_p = 2 only).
 bool fwd3d_rows_supported(int r_p, int r_s, bool short_tiles);
 size_t fwd3d_rows_lds_bytes(int r_p, int r_s, bool short_tiles);
 hipError_t fwd3d_rows_launch(const InDesc &in, float *out, float *Zout, int r_p, int r_s, float k, bool short_tiles,
-                             hipStream_t s);
+                             hipStream_t s, const Workspace *ws = nullptr);
 
 // ---- autotuner (gnlm_tune.cpp) ---------------------------------------------
 // Variant for (r_p, r_s): the tuned one if gnlm_autotune ran, otherwise the

@@ -102,9 +102,10 @@ hipError_t launch_cfg(const InDesc &in, float *out, float *Zout, int rs, float k
 }
 
 template <int RP>
-hipError_t launch_rp(const InDesc &in, float *out, float *Zout, int rs, float k, int variant, hipStream_t s)
+hipError_t launch_rp(const InDesc &in, float *out, float *Zout, int rs, float k, int variant, hipStream_t s,
+                     const Workspace *ws)
 {
-    if (fwd3_is_rows(variant)) return fwd3d_rows_launch(in, out, Zout, RP, rs, k, variant == kVRowsShort, s);
+    if (fwd3_is_rows(variant)) return fwd3d_rows_launch(in, out, Zout, RP, rs, k, variant == kVRowsShort, s, ws);
     if (variant == kVSlab) return launch_slab<RP, false>(in, out, Zout, rs, k, s);
     if (variant == kVSlab2) return launch_slab<RP, true>(in, out, Zout, rs, k, s);
     if (variant == kV16x16x8) return launch_cfg<Fwd3Cfg<RP, 16, 16, 8>>(in, out, Zout, rs, k, s);
@@ -154,16 +155,16 @@ bool fwd3d_tiled_supported(int r_p, int r_s)
 }
 
 hipError_t fwd3d_tiled_launch_variant(const InDesc &in, float *out, float *Zout, int r_p, int r_s, float k,
-                                      int variant, hipStream_t s)
+                                      int variant, hipStream_t s, const Workspace *ws)
 {
     if (!fwd3d_variant_supported(r_p, r_s, variant)) return hipErrorInvalidValue;
-    if (variant == kVRowsBig) return fwd3d_rows_big_launch(in, out, Zout, r_p, r_s, k, s);
-    if (variant == kVRows && r_p == 4) return fwd3d_rows_launch(in, out, Zout, 4, r_s, k, false, s);
+    if (variant == kVRowsBig) return fwd3d_rows_big_launch(in, out, Zout, r_p, r_s, k, s, ws);
+    if (variant == kVRows && r_p == 4) return fwd3d_rows_launch(in, out, Zout, 4, r_s, k, false, s, ws);
     switch (r_p) {
-    case 0: return launch_rp<0>(in, out, Zout, r_s, k, variant, s);
-    case 1: return launch_rp<1>(in, out, Zout, r_s, k, variant, s);
-    case 2: return launch_rp<2>(in, out, Zout, r_s, k, variant, s);
-    case 3: return launch_rp<3>(in, out, Zout, r_s, k, variant, s);
+    case 0: return launch_rp<0>(in, out, Zout, r_s, k, variant, s, ws);
+    case 1: return launch_rp<1>(in, out, Zout, r_s, k, variant, s, ws);
+    case 2: return launch_rp<2>(in, out, Zout, r_s, k, variant, s, ws);
+    case 3: return launch_rp<3>(in, out, Zout, r_s, k, variant, s, ws);
     default: return hipErrorInvalidValue;
     }
 }

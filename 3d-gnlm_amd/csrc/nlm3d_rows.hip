@@ -41,16 +41,16 @@ bool rows_rp1_four_per_cu(long long tiles)
 }
 
 template <int RP, int RS, bool SHORT>
-hipError_t rows_launch_t(const InDesc &in, float *out, float *Zout, float k, hipStream_t s)
+hipError_t rows_launch_t(const InDesc &in, float *out, float *Zout, float k, hipStream_t s, const Workspace *ws)
 {
     using C = RowsCfg<RP, RS, SHORT>;
     if constexpr (RP == 1) {
         const long long tiles = (long long)((in.nx + C::TX - 1) / C::TX) * ((in.ny + C::TY - 1) / C::TY) *
                                 ((in.nz + C::TZ - 1) / C::TZ);
         if (rows_rp1_four_per_cu(tiles))
-            return rows_launch_cfg<C>(in, out, Zout, k, s, nlm3d_fwd_rows<RP, RS, SHORT, 4>);
+            return rows_launch_cfg<C>(in, out, Zout, k, s, nlm3d_fwd_rows<RP, RS, SHORT, 4>, ws);
     }
-    return rows_launch_cfg<C>(in, out, Zout, k, s, nlm3d_fwd_rows<RP, RS, SHORT, rows_waves<RP, SHORT>()>);
+    return rows_launch_cfg<C>(in, out, Zout, k, s, nlm3d_fwd_rows<RP, RS, SHORT, rows_waves<RP, SHORT>()>, ws);
 }
 
 // Instantiated radii: r_p in {1, 2, 3, 4} with r_s up to 8 (r_p = 1) or 7
@@ -98,12 +98,12 @@ size_t fwd3d_rows_lds_bytes(int r_p, int r_s, bool short_tiles)
 }
 
 hipError_t fwd3d_rows_launch(const InDesc &in, float *out, float *Zout, int r_p, int r_s, float k, bool short_tiles,
-                             hipStream_t s)
+                             hipStream_t s, const Workspace *ws)
 {
     // a caller-padded input must carry the whole halo the copy needs
     if (in.pad > 0 && in.pad < r_p + r_s) return hipErrorInvalidValue;
 #define GNLM_ROWS_RUN(RP, RS, SH) \
-    if (r_p == RP && r_s == RS && short_tiles == SH) return rows_launch_t<RP, RS, SH>(in, out, Zout, k, s);
+    if (r_p == RP && r_s == RS && short_tiles == SH) return rows_launch_t<RP, RS, SH>(in, out, Zout, k, s, ws);
     GNLM_ROWS_CASES(GNLM_ROWS_RUN)
 #undef GNLM_ROWS_RUN
     return hipErrorInvalidValue;

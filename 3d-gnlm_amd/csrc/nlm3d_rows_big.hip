@@ -41,13 +41,14 @@ size_t fwd3d_rows_big_lds_bytes(int r_p, int r_s)
     return (size_t)-1;
 }
 
-hipError_t fwd3d_rows_big_launch(const InDesc &in, float *out, float *Zout, int r_p, int r_s, float k, hipStream_t s)
+hipError_t fwd3d_rows_big_launch(const InDesc &in, float *out, float *Zout, int r_p, int r_s, float k, hipStream_t s,
+                                 const Workspace *ws)
 {
     // a caller-padded input must carry the whole halo the copy needs
     if (in.pad > 0 && in.pad < r_p + r_s) return hipErrorInvalidValue;
 #define GNLM_RB_RUN(RP, RS)                                                                       \
     if (r_p == RP && r_s == RS)                                                                   \
-        return rows_launch_cfg<RowsBigCfg<RP, RS>>(in, out, Zout, k, s, nlm3d_fwd_rows_big<RP, RS>);
+        return rows_launch_cfg<RowsBigCfg<RP, RS>>(in, out, Zout, k, s, nlm3d_fwd_rows_big<RP, RS>, ws);
     GNLM_ROWS_BIG_CASES(GNLM_RB_RUN)
 #undef GNLM_RB_RUN
     return hipErrorInvalidValue;

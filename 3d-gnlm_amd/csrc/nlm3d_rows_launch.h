@@ -70,9 +70,14 @@ constexpr size_t rows_lds_cfg()
 }
 
 // Workspace = padded copy (+ the Z accumulator when the caller wants no Z),
-// allocated and released on the stream around the two launches.
+// allocated and released on the stream around the two launches -- or, when
+// the caller hands one in (gws), taken from it: nothing is allocated then,
+// and a workspace too small is hipErrorOutOfMemory.  Every float of the
+// padded copy is written by rows_pad and the Z accumulator is written before
+// it is read (first dz), so a reused workspace needs no clearing.
 template <class C>
-hipError_t rows_launch_cfg(const InDesc &in, float *out, float *Zout, float k, hipStream_t s, void (*kernel)(RowsArgs))
+hipError_t rows_launch_cfg(const InDesc &in, float *out, float *Zout, float k, hipStream_t s, void (*kernel)(RowsArgs),
+                           const Workspace *gws = nullptr)
 {
     RowsArgs a;
     a.nx = in.nx;
@@ -91,8 +96,14 @@ hipError_t rows_launch_cfg(const InDesc &in, float *out, float *Zout, float k, h
     if (npad >= (1LL << 30) || nv >= (1LL << 31)) return hipErrorInvalidValue;
     const size_t bytes = sizeof(float) * (size_t)(npad + (Zout ? 0 : nv));
     float *ws = nullptr;
-    hipError_t e = hipMallocAsync((void **)&ws, bytes, s);
-    if (e != hipSuccess) return e;
+    hipError_t e = hipSuccess;
+    if (gws) {
+        if (gws->bytes < bytes || !gws->p) return hipErrorOutOfMemory;
+        ws = static_cast<float *>(gws->p);
+    } else {
+        e = hipMallocAsync((void **)&ws, bytes, s);
+        if (e != hipSuccess) return e;
+    }
     a.v.p = ws;
     a.v.px = px;
     a.v.pxy = px * py;
@@ -110,7 +121,7 @@ hipError_t rows_launch_cfg(const InDesc &in, float *out, float *Zout, float k, h
             e = hipGetLastError();
         }
     }
-    const hipError_t e2 = hipFreeAsync(ws, s);
+    const hipError_t e2 = gws ? hipSuccess : hipFreeAsync(ws, s);
     return e != hipSuccess ? e : e2;
 }
 

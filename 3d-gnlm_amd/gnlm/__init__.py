@@ -83,6 +83,8 @@ def _extend(L):
     L.gnlm_set_backward_form.argtypes = [i]
     L.gnlm_set_backward_parts.argtypes = [i]
     L.gnlm_set_backward_parts.restype = i
+    L.gnlm3d_forward_noalloc.argtypes = [vp, vp, vp, i, i, i, i, i, f, vp]
+    L.gnlm3d_forward_noalloc.restype = i
     L.gnlm3d_forward_route.argtypes = [i, i, i, i, i, i, ctypes.POINTER(i)]
     L.gnlm3d_forward_route.restype = i
     L.gnlm3d_backward_route.argtypes = [i, i, i, i, i, ctypes.POINTER(i)]

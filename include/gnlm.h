@@ -74,6 +74,22 @@ int gnlm3d_forward_ex(const float *d_in, int in_pad, long long in_sy, long long 
                       float *d_Z, int nx, int ny, int nz, int r_p, int r_s, float h, int path,
                       void *stream);
 
+/* 3D forward that allocates nothing -- no counterpart in the reference; for
+ * callers that cannot call hipMalloc (code under stream capture, tight
+ * serving loops, harnesses that forbid allocation).  Same arguments, kernels
+ * and bits as gnlm3d_forward.  The workspace the kernels need (the padded
+ * copy of the input, and the running Z total when d_Z is NULL) comes from a
+ * fixed device array of GNLM_NOALLOC_SCRATCH_BYTES that each device the
+ * library runs on holds until the process ends: about 192^3 voxels at patch
+ * 5^3 / search 11^3 with d_Z NULL.  GNLM_ENOMEM (before anything is
+ * launched) for a volume that needs more; GNLM_EUNSUPPORTED where the route
+ * is the generic per-offset path, which allocates.  All calls on one device
+ * share its array, so calls that overlap on one device (other streams, other
+ * host threads) race: serialise them. */
+#define GNLM_NOALLOC_SCRATCH_BYTES (64u * 1024u * 1024u)
+int gnlm3d_forward_noalloc(const float *d_in, float *d_out, float *d_Z, int nx, int ny, int nz, int r_p,
+                           int r_s, float h, void *stream);
+
 /* Which kernel a call would run, and whether it can run at all: pure host
  * logic, no device work and no allocation (no counterpart in the reference,
  * whose launcher runs one fixed kernel set per compile-time geometry,
