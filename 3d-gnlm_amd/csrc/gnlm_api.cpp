@@ -76,8 +76,10 @@ int gnlm3d_forward_ex(const float *d_in, int in_pad, long long in_sy, long long 
     const int P = (2 * r_p + 1) * (2 * r_p + 1) * (2 * r_p + 1);
     const float k = (float)weight_scale(P, h);
     hipStream_t s = (hipStream_t)stream;
-    if (kernel != GNLM_KERNEL_GENERIC)
-        return hip_status(fwd3d_tiled_launch_variant(in, d_out, d_Z, r_p, r_s, k, kernel, s));
+    if (kernel != GNLM_KERNEL_GENERIC) {
+        const int shape = fwd3d_pick_shape(in, d_out, d_Z, r_p, r_s, k, kernel, s, true);
+        return hip_status(fwd3d_tiled_launch_variant(in, d_out, d_Z, r_p, r_s, k, kernel, s, nullptr, shape));
+    }
     return hip_status(fwd_generic_launch(in, d_out, d_Z, r_p, r_s, r_p, r_s, k, s));
 }
 

@@ -30,7 +30,10 @@ size_t fwd3d_lds_bytes(int r_p, int r_s, int variant);
 bool fwd3d_variant_supported(int r_p, int r_s, int variant);
 bool fwd3d_tiled_supported(int r_p, int r_s);
 hipError_t fwd3d_tiled_launch_variant(const InDesc &in, float *out, float *Zout, int r_p, int r_s, float k,
-                                      int variant, hipStream_t s, const Workspace *ws = nullptr);
+                                      int variant, hipStream_t s, const Workspace *ws = nullptr, int shape = -1);
+// Launch shapes of a variant for (r_p, r_s): the bit-identical alternatives the
+// autotuner chooses between (1 where there is no choice).
+int fwd3d_num_shapes(int r_p, int r_s, int variant);
 // Whether a rows-family variant's padded copy of an nx*ny*nz volume fits its
 // 32-bit offsets.
 bool fwd3d_rows_fits(int nx, int ny, int nz, int r_p, int r_s, int variant);
@@ -53,12 +56,22 @@ hipError_t fwd3d_rows_big_launch(const InDesc &in, float *out, float *Zout, int 
 bool fwd3d_rows_supported(int r_p, int r_s, bool short_tiles);
 size_t fwd3d_rows_lds_bytes(int r_p, int r_s, bool short_tiles);
 hipError_t fwd3d_rows_launch(const InDesc &in, float *out, float *Zout, int r_p, int r_s, float k, bool short_tiles,
-                             hipStream_t s, const Workspace *ws = nullptr);
+                             hipStream_t s, const Workspace *ws = nullptr, int shape = -1);
+// Launch shapes of the rows tiles for (r_p, short): bit-identical builds and
+// tile depths (nlm3d_rows.hip lists them); shape -1 = the default rule.
+int fwd3d_rows_num_shapes(int r_p, bool short_tiles);
 
 // ---- autotuner (gnlm_tune.cpp) ---------------------------------------------
 // Variant for (r_p, r_s): the tuned one if gnlm_autotune ran, otherwise the
 // first supported variant.  Thread-safe.
 int fwd3d_pick_variant(int r_p, int r_s);
+// Launch shape for this volume and variant: the one recorded for its (device,
+// radii, variant, size class), else -- when may_tune, tuning is on and the
+// stream is not capturing -- the fastest of the variant's bit-identical shapes
+// timed on these buffers (recorded), else -1 (the default rule).
+int fwd3d_pick_shape(const InDesc &in, float *out, float *Zout, int r_p, int r_s, float k, int variant,
+                     hipStream_t s, bool may_tune);
+bool fwd3d_tuning_lazy();  // gnlm_set_tuning(GNLM_TUNE_LAZY) in force
 
 // ---- tiled 2D forward (nlm2d_fwd.hip) --------------------------------------
 bool fwd2d_tiled_supported(int r_p, int r_s);

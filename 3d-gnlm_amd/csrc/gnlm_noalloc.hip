@@ -75,7 +75,9 @@ extern "C" int gnlm3d_forward_noalloc(const float *d_in, float *d_out, float *d_
     InDesc in{d_in, nx, ny, nz, 0, 0, 0};
     const int P = (2 * r_p + 1) * (2 * r_p + 1) * (2 * r_p + 1);
     const float k = (float)weight_scale(P, h);
-    e = fwd3d_tiled_launch_variant(in, d_out, d_Z, r_p, r_s, k, kernel, (hipStream_t)stream, &ws);
+    // the shape the forward recorded for this volume class, never a tuning run
+    const int shape = fwd3d_pick_shape(in, d_out, d_Z, r_p, r_s, k, kernel, (hipStream_t)stream, false);
+    e = fwd3d_tiled_launch_variant(in, d_out, d_Z, r_p, r_s, k, kernel, (hipStream_t)stream, &ws, shape);
     if (e == hipErrorOutOfMemory) return GNLM_ENOMEM;  // the volume needs more than the fixed scratch
     if (e == hipErrorInvalidValue) return GNLM_EINVAL;
     return e == hipSuccess ? GNLM_OK : GNLM_EHIP;

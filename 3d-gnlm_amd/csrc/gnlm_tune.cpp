@@ -1,17 +1,30 @@
-// gnlm_tune.cpp -- autotuner for the tiled 3D forward.
+// gnlm_tune.cpp -- autotuners for the tiled 3D forward.
 //
-// For a (patch, search) radius pair it times every tile variant that fits in
-// LDS on a synthetic probe volume on the caller's device and stream, keeps the
-// fastest, and records the choice in a process-wide table guarded by a mutex.
-// The per-volume launch and the batched persistent launch both read that
-// table, so a volume is always processed with the same tile shape, and the
-// per-volume and batched launches of a volume give identical bits.  The
-// direct-sum variants are also bit-identical to each other; the slab variant
-// (sliding sums) agrees with them to rounding.
+// Two levels:
+//   * variants (gnlm_autotune, on request): for a (patch, search) radius pair
+//     it times every tile variant that fits in LDS on a synthetic probe volume
+//     on the caller's device and stream, keeps the fastest, and records the
+//     choice in a process-wide table guarded by a mutex.  The per-volume
+//     launch and the batched persistent launch both read that table, so a
+//     volume is always processed with the same arithmetic, and the per-volume
+//     and batched launches of a volume give identical bits.  The direct-sum
+//     variants are bit-identical to each other; the others agree to rounding.
+//   * launch shapes (automatic, at first use): within the chosen variant, the
+//     builds and tile depths that give the SAME bits -- waves per SIMD
+//     (register budget) and the tile's z depth, i.e. the share of halo rows --
+//     are timed on the caller's own volume the first time a (device, r_p,
+//     r_s, variant, size class) key is seen, and the fastest is kept for that
+//     key.  Since every candidate computes identical outputs, what the timing
+//     picks changes the speed and never the result.  Calls under stream
+//     capture, and gnlm3d_forward_noalloc, never tune: they take the table's
+//     shape or the default rule.
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <atomic>
+#include <cmath>
 #include <map>
 #include <mutex>
+#include <tuple>
 #include <utility>
 #include <vector>
 #include "gnlm_internal.h"
@@ -22,6 +35,23 @@ namespace {
 
 std::mutex g_mu;
 std::map<std::pair<int, int>, int> g_table;  // (r_p, r_s) -> variant
+// (device, r_p, r_s, variant, size class) -> launch shape
+typedef std::tuple<int, int, int, int, int> ShapeKey;
+std::map<ShapeKey, int> g_shapes;
+std::atomic<int> g_tune_mode{GNLM_TUNE_LAZY};
+
+// Size class of a volume: half octaves of its voxel count.  Which shape wins
+// depends on how the volume's tiles fill the device's workgroup slots, which
+// moves with the volume size.
+int size_class(long long nvox) { return (int)std::floor(2.0 * std::log2((double)std::max(1LL, nvox))); }
+
+bool shape_key(int nx, int ny, int nz, int r_p, int r_s, int variant, ShapeKey *key)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return false;
+    *key = ShapeKey{dev, r_p, r_s, variant, size_class((long long)nx * ny * nz)};
+    return true;
+}
 
 // Default when (r_p, r_s) has not been tuned: the rows variants where they
 // are instantiated (short tiles first: for r_p = 2 they run 8-14% faster at
@@ -60,6 +90,59 @@ int fwd3d_pick_variant(int r_p, int r_s)
         if (it != g_table.end()) return it->second;
     }
     return first_supported(r_p, r_s);
+}
+
+bool fwd3d_tuning_lazy() { return g_tune_mode.load(std::memory_order_relaxed) == GNLM_TUNE_LAZY; }
+
+int fwd3d_pick_shape(const InDesc &in, float *out, float *Zout, int r_p, int r_s, float k, int variant,
+                     hipStream_t s, bool may_tune)
+{
+    const int n = fwd3d_num_shapes(r_p, r_s, variant);
+    if (n <= 1) return -1;
+    ShapeKey key;
+    if (!shape_key(in.nx, in.ny, in.nz, r_p, r_s, variant, &key)) return -1;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        auto it = g_shapes.find(key);
+        if (it != g_shapes.end()) return it->second;
+    }
+    if (!may_tune || g_tune_mode.load(std::memory_order_relaxed) != GNLM_TUNE_LAZY) return -1;
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone) return -1;
+    // time every shape on the caller's own volume and buffers (each writes the
+    // same bits): one warm-up launch, then two timed back to back
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        if (e0) (void)hipEventDestroy(e0);
+        return -1;
+    }
+    int best = -1;
+    float best_ms = 0.0f;
+    for (int c = 0; c < n; ++c) {
+        hipError_t e = fwd3d_tiled_launch_variant(in, out, Zout, r_p, r_s, k, variant, s, nullptr, c);
+        if (e == hipSuccess) e = hipEventRecord(e0, s);
+        for (int r = 0; r < 2 && e == hipSuccess; ++r)
+            e = fwd3d_tiled_launch_variant(in, out, Zout, r_p, r_s, k, variant, s, nullptr, c);
+        if (e == hipSuccess) e = hipEventRecord(e1, s);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();  // a shape that cannot launch here is simply not chosen
+            continue;
+        }
+        if (best < 0 || ms < best_ms) {
+            best = c;
+            best_ms = ms;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (best >= 0) {
+        std::lock_guard<std::mutex> lk(g_mu);
+        g_shapes[key] = best;
+    }
+    return best;
 }
 
 }  // namespace gnlm
@@ -150,6 +233,50 @@ void gnlm_reset_tuning(void)
 {
     std::lock_guard<std::mutex> lk(g_mu);
     g_table.clear();
+    g_shapes.clear();
+}
+
+int gnlm_set_tuning(int mode)
+{
+    if (mode != GNLM_TUNE_LAZY && mode != GNLM_TUNE_OFF) return GNLM_EINVAL;
+    g_tune_mode.store(mode, std::memory_order_relaxed);
+    return GNLM_OK;
+}
+
+int gnlm_forward_shape(int nx, int ny, int nz, int r_p, int r_s, int *shape, int *nshapes)
+{
+    int kernel = 0;
+    const int st = gnlm3d_forward_route(nx, ny, nz, r_p, r_s, GNLM_PATH_AUTO, &kernel);
+    if (st != GNLM_OK) return st;
+    const int n = kernel == GNLM_KERNEL_GENERIC ? 1 : fwd3d_num_shapes(r_p, r_s, kernel);
+    if (nshapes) *nshapes = n;
+    if (shape) {
+        *shape = -1;
+        ShapeKey key;
+        if (n > 1 && shape_key(nx, ny, nz, r_p, r_s, kernel, &key)) {
+            std::lock_guard<std::mutex> lk(g_mu);
+            auto it = g_shapes.find(key);
+            if (it != g_shapes.end()) *shape = it->second;
+        }
+    }
+    return GNLM_OK;
+}
+
+int gnlm_set_forward_shape(int nx, int ny, int nz, int r_p, int r_s, int shape)
+{
+    int kernel = 0;
+    const int st = gnlm3d_forward_route(nx, ny, nz, r_p, r_s, GNLM_PATH_AUTO, &kernel);
+    if (st != GNLM_OK) return st;
+    const int n = kernel == GNLM_KERNEL_GENERIC ? 1 : fwd3d_num_shapes(r_p, r_s, kernel);
+    if (shape < -1 || shape >= n) return GNLM_EINVAL;
+    ShapeKey key;
+    if (!shape_key(nx, ny, nz, r_p, r_s, kernel, &key)) return GNLM_EHIP;
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (shape < 0)
+        g_shapes.erase(key);
+    else
+        g_shapes[key] = shape;
+    return GNLM_OK;
 }
 
 }  // extern "C"

@@ -432,6 +432,7 @@ int gnlm_batch_plan_create(const gnlm_batch_volume *vols, int n, gnlm_batch_plan
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kfn, kFwdThreads, lds);
     if (e == hipSuccess && occ < 1) e = hipErrorInvalidConfiguration;
     if (e == hipSuccess) p->nblocks = std::min(ncu * occ, p->ntiles);
+    const int occ_max = occ;
     p->npad = (int)padids.size();
     p->padmax_xy = padmax_xy;
     p->padmax_z = padmax_z;
@@ -453,9 +454,39 @@ int gnlm_batch_plan_create(const gnlm_batch_volume *vols, int n, gnlm_batch_plan
     if (e == hipSuccess) e = hipMemcpy(p->d_vols, hv.data(), sizeof(BatchVol) * hv.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(p->d_tiles, sorted.data(), sizeof(BatchTile) * sorted.size(), hipMemcpyHostToDevice);
-    // capture [reset counter -> persistent kernel] on a private stream
     hipStream_t cs = nullptr;
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+    // Launch-shape tuning (GNLM_TUNE_LAZY): the persistent kernel's
+    // workgroups per CU, the most that fit or one fewer (fewer workgroups
+    // contend less for each CU's LDS and L1 and leave queue stealing more
+    // room).  Any count processes every tile with the same tile body, so the
+    // outputs are the same bits; each candidate runs the whole batch eagerly
+    // (one warm-up, two timed) on the plan's own buffers.
+    if (e == hipSuccess && fwd3d_tuning_lazy() && occ_max > 1) {
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        e = hipEventCreate(&e0);
+        if (e == hipSuccess) e = hipEventCreate(&e1);
+        int best = 0;
+        float best_ms = 0.0f;
+        for (int per_cu = occ_max; per_cu >= occ_max - 1 && e == hipSuccess; --per_cu) {
+            p->nblocks = std::min(ncu * per_cu, p->ntiles);
+            e = enqueue_batch(p, cs);
+            if (e == hipSuccess) e = hipEventRecord(e0, cs);
+            for (int r = 0; r < 2 && e == hipSuccess; ++r) e = enqueue_batch(p, cs);
+            if (e == hipSuccess) e = hipEventRecord(e1, cs);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            float ms = 0.0f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+            if (e == hipSuccess && (best == 0 || ms < best_ms)) {
+                best = per_cu;
+                best_ms = ms;
+            }
+        }
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (e == hipSuccess) p->nblocks = std::min(ncu * best, p->ntiles);
+    }
+    // capture [reset counter -> persistent kernel] on a private stream
     if (e == hipSuccess) e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
         e = enqueue_batch(p, cs);

@@ -103,9 +103,10 @@ hipError_t launch_cfg(const InDesc &in, float *out, float *Zout, int rs, float k
 
 template <int RP>
 hipError_t launch_rp(const InDesc &in, float *out, float *Zout, int rs, float k, int variant, hipStream_t s,
-                     const Workspace *ws)
+                     const Workspace *ws, int shape)
 {
-    if (fwd3_is_rows(variant)) return fwd3d_rows_launch(in, out, Zout, RP, rs, k, variant == kVRowsShort, s, ws);
+    if (fwd3_is_rows(variant))
+        return fwd3d_rows_launch(in, out, Zout, RP, rs, k, variant == kVRowsShort, s, ws, shape);
     if (variant == kVSlab) return launch_slab<RP, false>(in, out, Zout, rs, k, s);
     if (variant == kVSlab2) return launch_slab<RP, true>(in, out, Zout, rs, k, s);
     if (variant == kV16x16x8) return launch_cfg<Fwd3Cfg<RP, 16, 16, 8>>(in, out, Zout, rs, k, s);
@@ -154,17 +155,35 @@ bool fwd3d_tiled_supported(int r_p, int r_s)
     return false;
 }
 
+int fwd3d_num_shapes(int r_p, int r_s, int variant)
+{
+    if (!fwd3d_variant_supported(r_p, r_s, variant)) return 0;
+    // r_p = 4: the 16 x 8 x 8 rows tiles and the 16 x 24 x 24 big tiles pair
+    // their tree outputs from even run starts alike, so they give the same bits
+    if (r_p == 4 && (variant == kVRows || variant == kVRowsBig))
+        return fwd3d_variant_supported(4, r_s, kVRows) && fwd3d_variant_supported(4, r_s, kVRowsBig) ? 2 : 1;
+    if (fwd3_is_rows(variant)) return fwd3d_rows_num_shapes(r_p, variant == kVRowsShort);
+    return 1;
+}
+
 hipError_t fwd3d_tiled_launch_variant(const InDesc &in, float *out, float *Zout, int r_p, int r_s, float k,
-                                      int variant, hipStream_t s, const Workspace *ws)
+                                      int variant, hipStream_t s, const Workspace *ws, int shape)
 {
     if (!fwd3d_variant_supported(r_p, r_s, variant)) return hipErrorInvalidValue;
+    if (shape >= fwd3d_num_shapes(r_p, r_s, variant)) return hipErrorInvalidValue;
+    if (r_p == 4 && (variant == kVRows || variant == kVRowsBig) && shape >= 0) {
+        // shape 0 = the variant itself, 1 = the other r_p = 4 tile (same bits)
+        const bool big = (variant == kVRowsBig) != (shape == 1);
+        return big ? fwd3d_rows_big_launch(in, out, Zout, r_p, r_s, k, s, ws)
+                   : fwd3d_rows_launch(in, out, Zout, 4, r_s, k, false, s, ws);
+    }
     if (variant == kVRowsBig) return fwd3d_rows_big_launch(in, out, Zout, r_p, r_s, k, s, ws);
     if (variant == kVRows && r_p == 4) return fwd3d_rows_launch(in, out, Zout, 4, r_s, k, false, s, ws);
     switch (r_p) {
-    case 0: return launch_rp<0>(in, out, Zout, r_s, k, variant, s, ws);
-    case 1: return launch_rp<1>(in, out, Zout, r_s, k, variant, s, ws);
-    case 2: return launch_rp<2>(in, out, Zout, r_s, k, variant, s, ws);
-    case 3: return launch_rp<3>(in, out, Zout, r_s, k, variant, s, ws);
+    case 0: return launch_rp<0>(in, out, Zout, r_s, k, variant, s, ws, shape);
+    case 1: return launch_rp<1>(in, out, Zout, r_s, k, variant, s, ws, shape);
+    case 2: return launch_rp<2>(in, out, Zout, r_s, k, variant, s, ws, shape);
+    case 3: return launch_rp<3>(in, out, Zout, r_s, k, variant, s, ws, shape);
     default: return hipErrorInvalidValue;
     }
 }
@@ -174,7 +193,9 @@ bool fwd3d_rows_fits(int nx, int ny, int nz, int r_p, int r_s, int variant)
     // the padded copy as rows_launch_cfg lays it out: whole tiles plus the
     // halo R = r_p + r_s (x: LP = 4 ceil(R / 4)), below 2^30 floats, and the
     // volume below 2^31 voxels (32-bit offsets in the kernels)
-    const long long tx = fwd3_tx(variant, r_p), ty = fwd3_ty(variant, r_p), tz = fwd3_tz(variant, r_p);
+    // (deep launch shapes, r_p 1 and short r_p 2, double the tile depth)
+    const int deep = (r_p == 1 && variant == kVRows) || (r_p == 2 && variant == kVRowsShort) ? 2 : 1;
+    const long long tx = fwd3_tx(variant, r_p), ty = fwd3_ty(variant, r_p), tz = deep * fwd3_tz(variant, r_p);
     const long long R = r_p + r_s, LP = 4 * ((R + 3) / 4);
     const long long px = (nx + tx - 1) / tx * tx + 2 * LP, py = (ny + ty - 1) / ty * ty + 2 * R,
                     pz = (nz + tz - 1) / tz * tz + 2 * R;

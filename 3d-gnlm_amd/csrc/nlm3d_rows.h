@@ -72,12 +72,21 @@ struct PadVol {
 // -- a larger halo share than the 28-plane tiles, but half the live
 // registers: the per-volume kernel runs four waves per SIMD (128 VGPRs; its
 // few spills sit outside the offset loops) instead of two.
-template <int RP_, int RS_, bool SHORT_ = false>
+// ZM = 2 ("deep" tiles, r_p 1 and 2): the same tile twice as deep in z on 512
+// threads.  The halo share of phases 1 and 2 drops (r_p = 2: phase-1 rows
+// 448 for 288 tile rows instead of 256 for 144), and every voxel gets exactly
+// the arithmetic of the ZM = 1 tile -- x and y runs are unchanged, and the
+// phase-3 z runs still start at multiples of RZ in volume coordinates, since
+// TZ stays a multiple of RZ -- so the outputs are bit-identical.  Two such
+// workgroups share a CU (four waves per SIMD); rows, phase-2 items and
+// phase-3 items beyond the tile's count are idle (whole waves).
+template <int RP_, int RS_, bool SHORT_ = false, int ZM_ = 1>
 struct RowsCfg {
-    static constexpr int RP = RP_, RS = RS_;
+    static constexpr int RP = RP_, RS = RS_, ZM = ZM_;
     static constexpr bool SHORT = SHORT_;
     static_assert(!SHORT || RP == 2, "short tiles are for r_p = 2");
-    static constexpr int NT = kRowsThreads;              // threads per workgroup
+    static_assert(ZM == 1 || (ZM == 2 && (RP == 1 || (RP == 2 && SHORT))), "deep tiles: r_p 1, short r_p 2");
+    static constexpr int NT = kRowsThreads * ZM;         // threads per workgroup
     static constexpr int W = 2 * RP + 1;                 // patch width
     static constexpr int R = RP + RS;                    // halo of the copy in y, z
     static constexpr int LP = 4 * ((R + 3) / 4);         // halo of the copy in x (aligned)
@@ -85,25 +94,26 @@ struct RowsCfg {
     static constexpr int NS = TX + 2 * LP;               // samples per loaded row
     static_assert(RP >= 1 && RP <= 4, "rows tiles are for r_p 1-4");
     static constexpr int TY = 16 - 2 * RP;
-    static constexpr int TZ = RP == 1 ? 14 : (RP == 3 ? 10 : (RP == 4 ? 8 : (SHORT ? 12 : 28)));
+    static constexpr int TZ = ZM * (RP == 1 ? 14 : (RP == 3 ? 10 : (RP == 4 ? 8 : (SHORT ? 12 : 28))));
     static constexpr int EX = TX + 2 * RP, EY = TY + 2 * RP, EZ = TZ + 2 * RP;
     static constexpr int NROWS = EY * EZ;
-    static constexpr int ROWS = NROWS / kRowsThreads;    // phase-1 rows per thread
-    static_assert(EY == 16 && NROWS % kRowsThreads == 0, "16-lane groups hold one z_ext plane of rows");
+    static constexpr int ROWS = (NROWS + NT - 1) / NT;   // phase-1 rows per thread
+    // rows past NROWS (deep tiles) are masked per lane in phase 1
+    static_assert(EY == 16 && (NROWS % NT == 0 || ROWS == 1), "16-lane groups hold one z_ext plane of rows");
     // phase 2 items: (x pair, z_ext, run of RY y), P2R per thread
     static constexpr int VX = 2;
     static constexpr int RY = RP == 1 ? 7 : (RP == 2 ? 6 : (RP == 3 ? 5 : 4));
     static constexpr int NXV = TX / VX;
     static constexpr int P2N = NXV * EZ * (TY / RY);
-    static constexpr int P2R = P2N / kRowsThreads;
-    static_assert(TY % RY == 0 && P2N % kRowsThreads == 0, "whole phase-2 rounds");
+    static constexpr int P2R = (P2N + NT - 1) / NT;
+    static_assert(TY % RY == 0 && (P2N % NT == 0 || P2R == 1), "whole phase-2 rounds");
     // phase 3 items: (group of V3 x columns, y, run of RZ z); the short tiles
     // read their columns four at a time (128-bit LDS reads: the phase's LDS
     // time halves, and 192 items leave a fourth wave free)
     static constexpr int RZ = SHORT ? 3 : (RP == 3 ? 5 : (RP == 4 ? 4 : 7));
     static constexpr int V3 = SHORT ? 4 : ((RP == 2 || RP == 4) ? 1 : 2);
     static constexpr int P3N = (TX / V3) * TY * (TZ / RZ);
-    static constexpr int P3R = (P3N + kRowsThreads - 1) / kRowsThreads;
+    static constexpr int P3R = (P3N + NT - 1) / NT;
     static_assert(TZ % RZ == 0, "whole z runs");
     // LDS: s_x [z_ext][y_ext][x] (row pitch 20, plane pitch 20*16+16),
     //      s_xy [z_ext][y][x] (plane pitch 16*TY+16)
@@ -360,7 +370,7 @@ __device__ __forceinline__ void rows_tile(const RowsTile &ta, float *lds)
         const int yp = r % C::EY, zp = r / C::EY;
         // z_ext grows with the lane index within a wave: the first lane decides
         // for the whole wave, which keeps the branch scalar
-        p1ok[q] = __builtin_amdgcn_readfirstlane(zp < tzv + 2 * RP);
+        p1ok[q] = __builtin_amdgcn_readfirstlane(zp < tzv + 2 * RP) && (C::NROWS % NT == 0 || r < C::NROWS);
         // at offset (dy, dz) = (-RS, -RS): every scalar offset added below is >= 0
         rbase[q] = 4u * (unsigned)((a.z0 - RP + zp + C::R - RS) * pxy + (a.y0 - RP + yp + C::R - RS) * px + a.x0);
         sxw[q] = zp * C::SXP + yp * C::SXR;
@@ -379,7 +389,15 @@ __device__ __forceinline__ void rows_tile(const RowsTile &ta, float *lds)
         const int xv = it % C::NXV;
         const int zp2 = (it / C::NXV) % C::EZ;
         const int yr2 = it / (C::NXV * C::EZ);
-        p2ok[m] = __builtin_amdgcn_readfirstlane(zp2 < tzv + 2 * RP);
+        // A wave holds 64 / NXV consecutive z_ext values of one y run when EZ
+        // is a multiple of that count, and the first lane can decide for the
+        // wave; otherwise (deep tiles: EZ = 28, 30) a wave can wrap into the
+        // next y run, whose low z_ext rows are needed, so each lane decides.
+        // Items past P2N (deep tiles) skip the phase.
+        if constexpr (C::EZ % (64 / C::NXV) == 0)
+            p2ok[m] = __builtin_amdgcn_readfirstlane(zp2 < tzv + 2 * RP) && (C::P2N % NT == 0 || it < C::P2N);
+        else
+            p2ok[m] = zp2 < tzv + 2 * RP && it < C::P2N;
         p2src[m] = zp2 * C::SXP + yr2 * C::RY * C::SXR + xv * C::VX;
         p2dst[m] = zp2 * C::SYP + yr2 * C::RY * 16 + xv * C::VX;
     }

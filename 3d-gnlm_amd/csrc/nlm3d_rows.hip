@@ -20,6 +20,16 @@ __global__ __launch_bounds__(kRowsThreads) __attribute__((amdgpu_waves_per_eu(WA
     rows_kernel_body<RowsCfg<RP, RS, SHORT>>(a, lds);
 }
 
+// Deep tiles (RowsCfg ZM = 2): 512 threads, two workgroups per CU at four
+// waves per SIMD (128 VGPRs).
+template <int RP, int RS, bool SHORT>
+__global__ __launch_bounds__(2 * kRowsThreads) __attribute__((amdgpu_waves_per_eu(4))) void nlm3d_fwd_rows_deep(
+    RowsArgs a)
+{
+    extern __shared__ __align__(16) float lds[];
+    rows_kernel_body<RowsCfg<RP, RS, SHORT, 2>>(a, lds);
+}
+
 template <int RP, int RS, bool SHORT>
 constexpr size_t rows_lds()
 {
@@ -40,15 +50,44 @@ bool rows_rp1_four_per_cu(long long tiles)
     return r3 <= 4 && 4 * r4 < 3 * r3;
 }
 
+// Launch shapes of one tile family: builds and tile depths that give the
+// same bits (the autotuner times them at a volume's own size).
+//   r_p = 1:        0 = 168 VGPRs, 3 tiles per CU; 1 = 128 VGPRs, 4 per CU;
+//                   2 = deep 16 x 14 x 28 tiles, 2 per CU
+//   r_p = 2 short:  0 = 16 x 12 x 12, 4 per CU; 1 = deep 16 x 12 x 24, 2 per CU
+//   r_p = 3:        0 = 128 VGPRs (4 per CU, spills outside the offset loop);
+//                   1 = 168 VGPRs (3 per CU, no spills)
+// (r_p = 4's second shape, the 16 x 24 x 24 big tile, is launched from the
+// variant dispatch.)  shape < 0: the default rule below.
+int rows_num_shapes(int r_p, bool short_tiles)
+{
+    if (r_p == 1) return 3;
+    if (r_p == 2 && short_tiles) return 2;
+    if (r_p == 3) return 2;
+    return 1;
+}
+
 template <int RP, int RS, bool SHORT>
-hipError_t rows_launch_t(const InDesc &in, float *out, float *Zout, float k, hipStream_t s, const Workspace *ws)
+hipError_t rows_launch_t(const InDesc &in, float *out, float *Zout, float k, hipStream_t s, const Workspace *ws,
+                         int shape)
 {
     using C = RowsCfg<RP, RS, SHORT>;
     if constexpr (RP == 1) {
-        const long long tiles = (long long)((in.nx + C::TX - 1) / C::TX) * ((in.ny + C::TY - 1) / C::TY) *
-                                ((in.nz + C::TZ - 1) / C::TZ);
-        if (rows_rp1_four_per_cu(tiles))
-            return rows_launch_cfg<C>(in, out, Zout, k, s, nlm3d_fwd_rows<RP, RS, SHORT, 4>, ws);
+        if (shape < 0) {
+            const long long tiles = (long long)((in.nx + C::TX - 1) / C::TX) * ((in.ny + C::TY - 1) / C::TY) *
+                                    ((in.nz + C::TZ - 1) / C::TZ);
+            shape = rows_rp1_four_per_cu(tiles) ? 1 : 0;
+        }
+        if (shape == 1) return rows_launch_cfg<C>(in, out, Zout, k, s, nlm3d_fwd_rows<RP, RS, SHORT, 4>, ws);
+        if (shape == 2)
+            return rows_launch_cfg<RowsCfg<RP, RS, SHORT, 2>>(in, out, Zout, k, s, nlm3d_fwd_rows_deep<RP, RS, SHORT>,
+                                                               ws);
+    } else if constexpr (RP == 2 && SHORT) {
+        if (shape == 1)
+            return rows_launch_cfg<RowsCfg<RP, RS, SHORT, 2>>(in, out, Zout, k, s, nlm3d_fwd_rows_deep<RP, RS, SHORT>,
+                                                               ws);
+    } else if constexpr (RP == 3) {
+        if (shape == 1) return rows_launch_cfg<C>(in, out, Zout, k, s, nlm3d_fwd_rows<RP, RS, SHORT, 3>, ws);
     }
     return rows_launch_cfg<C>(in, out, Zout, k, s, nlm3d_fwd_rows<RP, RS, SHORT, rows_waves<RP, SHORT>()>, ws);
 }
@@ -97,13 +136,16 @@ size_t fwd3d_rows_lds_bytes(int r_p, int r_s, bool short_tiles)
     return (size_t)-1;
 }
 
+int fwd3d_rows_num_shapes(int r_p, bool short_tiles) { return rows_num_shapes(r_p, short_tiles); }
+
 hipError_t fwd3d_rows_launch(const InDesc &in, float *out, float *Zout, int r_p, int r_s, float k, bool short_tiles,
-                             hipStream_t s, const Workspace *ws)
+                             hipStream_t s, const Workspace *ws, int shape)
 {
     // a caller-padded input must carry the whole halo the copy needs
     if (in.pad > 0 && in.pad < r_p + r_s) return hipErrorInvalidValue;
+    if (shape >= rows_num_shapes(r_p, short_tiles)) return hipErrorInvalidValue;
 #define GNLM_ROWS_RUN(RP, RS, SH) \
-    if (r_p == RP && r_s == RS && short_tiles == SH) return rows_launch_t<RP, RS, SH>(in, out, Zout, k, s, ws);
+    if (r_p == RP && r_s == RS && short_tiles == SH) return rows_launch_t<RP, RS, SH>(in, out, Zout, k, s, ws, shape);
     GNLM_ROWS_CASES(GNLM_ROWS_RUN)
 #undef GNLM_ROWS_RUN
     return hipErrorInvalidValue;

@@ -83,6 +83,12 @@ def _extend(L):
     L.gnlm_set_backward_form.argtypes = [i]
     L.gnlm_set_backward_parts.argtypes = [i]
     L.gnlm_set_backward_parts.restype = i
+    L.gnlm_set_tuning.argtypes = [i]
+    L.gnlm_set_tuning.restype = i
+    L.gnlm_forward_shape.argtypes = [i, i, i, i, i, ctypes.POINTER(i), ctypes.POINTER(i)]
+    L.gnlm_forward_shape.restype = i
+    L.gnlm_set_forward_shape.argtypes = [i, i, i, i, i, i]
+    L.gnlm_set_forward_shape.restype = i
     L.gnlm3d_forward_noalloc.argtypes = [vp, vp, vp, i, i, i, i, i, f, vp]
     L.gnlm3d_forward_noalloc.restype = i
     L.gnlm3d_forward_route.argtypes = [i, i, i, i, i, i, ctypes.POINTER(i)]
@@ -194,6 +200,30 @@ def backward_route(nx, ny, nz, r_p, r_s):
 def set_backward_form(form):
     """Process-wide form of the rows-organised adjoint, r_p 0-2 (gnlm_set_backward_form)."""
     check(lib().gnlm_set_backward_form(int(form)), "gnlm_set_backward_form")
+
+
+TUNE_LAZY, TUNE_OFF = 0, 1
+
+
+def set_tuning(mode):
+    """TUNE_LAZY (default): time the bit-identical launch shapes at first use per
+    volume size class; TUNE_OFF: the default rule (gnlm_set_tuning)."""
+    check(lib().gnlm_set_tuning(int(mode)), "gnlm_set_tuning")
+
+
+def forward_shape(shape3, r_p, r_s):
+    """(shape recorded for this volume's class or -1, number of shapes);
+    shape3 = (nz, ny, nx)."""
+    nz, ny, nx = (int(v) for v in shape3)
+    sh, n = ctypes.c_int(), ctypes.c_int()
+    check(lib().gnlm_forward_shape(nx, ny, nz, int(r_p), int(r_s), ctypes.byref(sh), ctypes.byref(n)),
+          "gnlm_forward_shape")
+    return sh.value, n.value
+
+
+def set_forward_shape(shape3, r_p, r_s, shape):
+    nz, ny, nx = (int(v) for v in shape3)
+    check(lib().gnlm_set_forward_shape(nx, ny, nz, int(r_p), int(r_s), int(shape)), "gnlm_set_forward_shape")
 
 
 def set_backward_parts(nparts):

@@ -476,6 +476,11 @@ def main():
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": _traffic_from_profiles(args.workload),
                 "kernel_ms": round(ms_kernel, 4), "flop_per_launch": flop}
+    if nz > 1:
+        # the tile variant and the launch shape the first call's tuning chose
+        shape, nshapes = gnlm.forward_shape((nz, ny, nx), rp, rs)
+        roofline["variant"] = gnlm.tuned_variant(rp, rs)
+        roofline["launch_shape"] = {"chosen": shape, "candidates": nshapes}
 
     cpu = None
     parity = None

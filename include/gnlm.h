@@ -189,7 +189,11 @@ typedef struct gnlm_batch_volume {
 typedef struct gnlm_batch_plan gnlm_batch_plan;
 
 /* Builds the tile list, uploads it, and captures the graph (on the current
- * device).  Tile shapes come from the autotuner table (gnlm_autotune). */
+ * device).  Tile shapes come from the autotuner table (gnlm_autotune).  With
+ * GNLM_TUNE_LAZY (gnlm_set_tuning) creation also times the persistent
+ * kernel at the most workgroups per CU that fit and at one fewer -- running
+ * the whole batch three times per count, on the plan's own buffers -- and
+ * captures the faster; the count never changes the outputs. */
 int gnlm_batch_plan_create(const gnlm_batch_volume *vols, int n, gnlm_batch_plan **plan);
 /* Replays the captured graph on `stream` (one graph launch). */
 int gnlm_batch_run(gnlm_batch_plan *plan, void *stream);
@@ -227,6 +231,31 @@ void gnlm_reset_tuning(void);                        /* forget all choices  */
  * only; batch plans run r_p = 4 on variant 4).
  * Untuned default: 5, else 4, else 6 where they are instantiated, else 3
  * where it fits, else 2, else the first direct variant. */
+
+/* ---- launch-shape tuning (automatic) -----------------------------------
+ * No counterpart in the reference (block and grid shapes are #defines there,
+ * 3DNLM/main.cu:15-21).  Within the variant the table selects, a forward has
+ * up to three launch shapes that compute the SAME bits: register budgets
+ * (waves per SIMD) and tile depths in z (the share of halo rows per tile):
+ *   r_p = 1: 16x14x14 tiles at 3 or 4 per CU, or 16x14x28 tiles at 2 per CU;
+ *   r_p = 2 (short tiles): 16x12x12 at 4 per CU, or 16x12x24 at 2 per CU;
+ *   r_p = 3: 16x10x10 at 4 per CU (128 VGPRs) or 3 per CU (168 VGPRs);
+ *   r_p = 4: 16x8x8 tiles or 16x24x24 tiles.
+ * With GNLM_TUNE_LAZY (the default) the first gnlm3d_forward of a (device,
+ * r_p, r_s, variant, size class) -- size class = half octaves of the voxel
+ * count -- times each shape on the caller's own volume and buffers (it
+ * waits for them once) and keeps the fastest for that class; later calls and
+ * gnlm3d_forward_noalloc use it.  Calls under stream capture never tune.
+ * GNLM_TUNE_OFF uses the default rule for untuned classes (r_p = 1: 4 per CU
+ * when that saves a round of tiles, else 3; otherwise shape 0).  Outputs never
+ * depend on which shape runs.  gnlm_reset_tuning forgets these choices too. */
+enum { GNLM_TUNE_LAZY = 0, GNLM_TUNE_OFF = 1 };
+int gnlm_set_tuning(int mode);
+/* *shape = the shape recorded for this volume's class (-1: none yet, the
+ * default rule applies), *nshapes = the candidates (1: no choice). */
+int gnlm_forward_shape(int nx, int ny, int nz, int r_p, int r_s, int *shape, int *nshapes);
+/* Record a shape for this volume's class (-1 forgets it). */
+int gnlm_set_forward_shape(int nx, int ny, int nz, int r_p, int r_s, int shape);
 
 /* uint8 -> float32 element conversion (n elements), for 8-bit inputs such as
  * the reference's (3DNLM/main.cu:7620 copies unsigned char volumes). */
