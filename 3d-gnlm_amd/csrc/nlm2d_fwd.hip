@@ -26,6 +26,7 @@
 // those to rounding (~1e-7 relative in S); for integer-valued input (the
 // uint8 images of the reference) every S is an exact integer in FP32 and the
 // results are identical bit for bit.
+#include <type_traits>
 #include "gnlm_internal.h"
 #include "nlm_box.h"
 
@@ -67,12 +68,47 @@ size_t fwd2_lds_bytes(int rs)
     return ((size_t)lds2_x(G::TX, R) * (G::TY + 2 * R) + 2 * (size_t)G::SXN) * sizeof(float);
 }
 
-template <int RP>
-__global__ __launch_bounds__(NT) void nlm2d_fwd_tiled(Fwd2Args a)
+// Workgroup barrier for LDS traffic only (the offset loop shares nothing else).
+// As inline asm it also keeps the compiler from interleaving the arithmetic of
+// consecutive unrolled offsets across it, which would otherwise keep several
+// offsets' values live at once.
+__device__ __forceinline__ void lds_barrier2()
+{
+    __builtin_amdgcn_sched_barrier(0);
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    __builtin_amdgcn_sched_barrier(0);
+}
+
+// compile-time loop: f(std::integral_constant<int, I>) for I = I0 .. I1
+template <int I, int I1, class F>
+__device__ __forceinline__ void static_for2(F &&f)
+{
+    if constexpr (I <= I1) {
+        f(std::integral_constant<int, I>{});
+        static_for2<I + 1, I1>(f);
+    }
+}
+
+// RSC = 0: search radius a.rs at run time; the shifted samples of a segment
+// are read from LDS for every offset.  RSC > 0: the search radius is RSC
+// (a.rs must equal it); a segment's shifted samples for all dx of one dy
+// (ES + 2 RSC of them) are read from LDS once per dy into registers, and dx
+// is unrolled so that each offset's window is a compile-time register range
+// -- a twentieth of the phase-1 LDS reads at r_s = 10.  Same arithmetic, same
+// bits.
+// Register budget of the compile-time-radius builds: the waves per SIMD that
+// their LDS image allows anyway (patch 11: 3 workgroups per CU; smaller: 4),
+// so that the unrolled offsets do not take the whole register file.
+template <int RP, int RSC>
+constexpr int waves2() { return RSC == 0 ? 1 : (RP >= 4 ? 3 : 4); }
+
+template <int RP, int RSC>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(waves2<RP, RSC>()))) void nlm2d_fwd_tiled(
+    Fwd2Args a)
 {
     using G = G2<RP>;
     extern __shared__ float lds[];
-    const int R = RP + a.rs;
+    const int R = RP + (RSC > 0 ? RSC : a.rs);
     const int LXr = G::TX + 2 * R;
     const int LX = lds2_x(G::TX, R);
     const int LY = G::TY + 2 * R;
@@ -92,15 +128,22 @@ __global__ __launch_bounds__(NT) void nlm2d_fwd_tiled(Fwd2Args a)
     }
     __syncthreads();
 
-    // phase-1 items: (row ry, segment sg), ry fastest
+    // phase-1 items: (row ry, segment sg), ry fastest.  Waves with no item
+    // skip phase 1 (a scalar branch); in a partly filled wave the lanes past
+    // the last item repeat it and only their LDS stores are masked, so that
+    // phase 1 has no divergent control flow around its register arrays.
     float cen[G::ROWS][G::ES];
     int base[G::ROWS];
+    bool wav[G::ROWS], own[G::ROWS];
 #pragma unroll
     for (int q = 0; q < G::ROWS; ++q) {
         const int r = tid + q * NT;
-        const int ry = r % G::EY, sg = r / G::EY;
+        own[q] = r < G::NR;
+        wav[q] = __builtin_amdgcn_readfirstlane(r) < G::NR;
+        const int rc = own[q] ? r : G::NR - 1;
+        const int ry = rc % G::EY, sg = rc / G::EY;
         base[q] = (R - RP + ry) * LX + (R - RP) + sg * G::SEG;
-        if (r < G::NR) {
+        if (wav[q]) {
 #pragma unroll
             for (int i = 0; i < G::ES; ++i) cen[q][i] = U[base[q] + i];
         }
@@ -117,34 +160,39 @@ __global__ __launch_bounds__(NT) void nlm2d_fwd_tiled(Fwd2Args a)
     }
     const int candbase = (R + cy0) * LX + (R + cx);
 
+    const int rs = RSC > 0 ? RSC : a.rs;
     int it = 0;
-    for (int dy = -a.rs; dy <= a.rs; ++dy) {
+    for (int dy = -rs; dy <= rs; ++dy) {
         float Zd[G::YR], Yd[G::YR];  // per-dy partial sums (two-level accumulation)
 #pragma unroll
         for (int m = 0; m < G::YR; ++m) {
             Zd[m] = 0.0f;
             Yd[m] = 0.0f;
         }
-        for (int dx = -a.rs; dx <= a.rs; ++dx, ++it) {
+        // one offset: phase 1 with shifted sample i of row item q from shv(q, i),
+        // the barrier, phase 2
+        auto offset = [&](int dx, auto &&shv) {
             float *SX = SXB + (it & 1) * G::SXN;
             const int doff = dy * LX + dx;
 #pragma unroll
             for (int q = 0; q < G::ROWS; ++q) {
-                const int r = tid + q * NT;
-                if (r < G::NR) {
-                    const int ry = r % G::EY, sg = r / G::EY;
-                    const float *sh = U + base[q] + doff;
+                if (wav[q]) {
+                    const int r = tid + q * NT;
+                    const int rc = own[q] ? r : G::NR - 1;
+                    const int ry = rc % G::EY, sg = rc / G::EY;
                     float e[G::ES];
 #pragma unroll
-                    for (int i = 0; i < G::ES; ++i) e[i] = cen[q][i] - sh[i];
+                    for (int i = 0; i < G::ES; ++i) e[i] = cen[q][i] - shv(q, i);
                     float sx[G::SEG];
                     box_tree_sq<2 * RP + 1, G::SEG>(e, sx);
                     float *dst = SX + (sg * G::SEG) * G::EYP + ry;
+                    if (own[q]) {
 #pragma unroll
-                    for (int j = 0; j < G::SEG; ++j) dst[j * G::EYP] = sx[j];
+                        for (int j = 0; j < G::SEG; ++j) dst[j * G::EYP] = sx[j];
+                    }
                 }
             }
-            __syncthreads();
+            lds_barrier2();
 
             float col[G::YR + 2 * RP];
             const float *src = SX + cx * G::EYP + cy0;
@@ -159,6 +207,35 @@ __global__ __launch_bounds__(NT) void nlm2d_fwd_tiled(Fwd2Args a)
                 Zd[m] = Zd[m] + w;
                 Yd[m] = fmaf(w, cand[m * LX], Yd[m]);
             }
+            // The partial sums of this offset are final here: without this the
+            // compiler sinks the unrolled offsets' weights below the dy loop's
+            // last barrier and keeps every offset's LDS reads live (hundreds of
+            // spilled registers).  The empty volatile asm keeps them in order.
+            if constexpr (RSC > 0) {
+#pragma unroll
+                for (int m = 0; m < G::YR; ++m) asm volatile("" : "+v"(Zd[m]), "+v"(Yd[m]));
+            }
+            ++it;
+        };
+        if constexpr (RSC > 0) {
+            // the shifted samples of every dx of this dy: window[j] = sample at x offset j - RSC
+            constexpr int NWIN = G::ES + 2 * RSC;
+            float win[G::ROWS][NWIN];
+#pragma unroll
+            for (int q = 0; q < G::ROWS; ++q) {
+                if (wav[q]) {
+                    const float *sh = U + base[q] + dy * LX - RSC;
+#pragma unroll
+                    for (int j = 0; j < NWIN; ++j) win[q][j] = sh[j];
+                }
+            }
+            static_for2<-RSC, RSC>([&](auto dxc) {
+                constexpr int DX = decltype(dxc)::value;
+                offset(DX, [&](int q, int i) { return win[q][i + DX + RSC]; });
+            });
+        } else {
+            for (int dx = -rs; dx <= rs; ++dx)
+                offset(dx, [&](int q, int i) { return U[base[q] + dy * LX + dx + i]; });
         }
 #pragma unroll
         for (int m = 0; m < G::YR; ++m) {
@@ -181,8 +258,8 @@ __global__ __launch_bounds__(NT) void nlm2d_fwd_tiled(Fwd2Args a)
     }
 }
 
-template <int RP>
-hipError_t launch2_rp(const InDesc &in, float *out, float *Zout, int rs, float k, hipStream_t s)
+template <int RP, int RSC>
+hipError_t launch2_k(const InDesc &in, float *out, float *Zout, int rs, float k, hipStream_t s)
 {
     using G = G2<RP>;
     Fwd2Args a;
@@ -194,11 +271,25 @@ hipError_t launch2_rp(const InDesc &in, float *out, float *Zout, int rs, float k
     a.ntx = (in.nx + G::TX - 1) / G::TX;
     a.nty = (in.ny + G::TY - 1) / G::TY;
     const size_t lds = fwd2_lds_bytes<RP>(rs);
-    hipError_t e = hipFuncSetAttribute((const void *)nlm2d_fwd_tiled<RP>,
+    hipError_t e = hipFuncSetAttribute((const void *)nlm2d_fwd_tiled<RP, RSC>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(nlm2d_fwd_tiled<RP>, dim3(a.ntx * a.nty), dim3(NT), lds, s, a);
+    hipLaunchKernelGGL((nlm2d_fwd_tiled<RP, RSC>), dim3(a.ntx * a.nty), dim3(NT), lds, s, a);
     return hipGetLastError();
+}
+
+// Search radii with a register-window build (the reference's own 2D geometry,
+// patch 11 / search 21, and patch 5 / search 11); the others run the
+// run-time-radius kernel.
+template <int RP>
+hipError_t launch2_rp(const InDesc &in, float *out, float *Zout, int rs, float k, hipStream_t s)
+{
+    if constexpr (RP == 5) {
+        if (rs == 10) return launch2_k<RP, 10>(in, out, Zout, rs, k, s);
+    } else if constexpr (RP == 2) {
+        if (rs == 5) return launch2_k<RP, 5>(in, out, Zout, rs, k, s);
+    }
+    return launch2_k<RP, 0>(in, out, Zout, rs, k, s);
 }
 
 constexpr size_t kLdsLimit = 160 * 1024;
