@@ -90,6 +90,9 @@ bool bwd3d_block_supported(int r_p, int r_s);  // the LDS-block adjoint (64-bit 
 // the selected form where their 32-bit offsets address the volume, else the
 // LDS-block tile; GNLM_EUNSUPPORTED when neither can run it.
 int bwd3d_route(int nx, int ny, int nz, int r_p, int r_s, int *kernel);
+// any-radius adjoint: per-offset passes through HBM scratch (nlm3d_bwd_generic.hip)
+hipError_t bwd3d_generic_launch(const float *u, const float *y, const float *Z, const float *g, int nx, int ny,
+                                int nz, int rp, int rs, float h, float *gu, float *gh, hipStream_t s);
 hipError_t bwd3d_launch(const float *u, const float *y, const float *Z, const float *g, int nx, int ny, int nz,
                         int rp, int rs, float h, float *gu, float *gh, hipStream_t s);
 // rows-organised adjoint for r_p 0-2, r_s in 1..7 (bwd3d_launch dispatches to it)

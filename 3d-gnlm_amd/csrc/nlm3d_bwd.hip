@@ -477,6 +477,12 @@ int bwd3d_route(int nx, int ny, int nz, int r_p, int r_s, int *kernel)
         *kernel = GNLM_KERNEL_BWD_BLOCK;
         return GNLM_OK;
     }
+    // any radii: per-offset passes through HBM (the dL/dh partials are
+    // reduced with an int count of voxels)
+    if ((long long)nx * ny * nz < (1LL << 31)) {
+        *kernel = GNLM_KERNEL_BWD_GENERIC;
+        return GNLM_OK;
+    }
     return GNLM_EUNSUPPORTED;
 }
 
@@ -508,6 +514,7 @@ hipError_t bwd3d_launch(const float *u, const float *y, const float *Z, const fl
 {
     int kernel = 0;
     if (bwd3d_route(nx, ny, nz, rp, rs, &kernel) != GNLM_OK) return hipErrorNotSupported;
+    if (kernel == GNLM_KERNEL_BWD_GENERIC) return bwd3d_generic_launch(u, y, Z, g, nx, ny, nz, rp, rs, h, gu, gh, s);
     if (kernel != GNLM_KERNEL_BWD_BLOCK) return bwd3d_rows_launch(u, y, Z, g, nx, ny, nz, rp, rs, h, gu, gh, s);
     const int R = rp + rs;
     const long long nv = (long long)nx * ny * nz;

@@ -179,7 +179,8 @@ def reset_tuning():
  VARIANT_ROWS_BIG) = 0, 1, 2, 3, 4, 5, 6
 
 BWD_AUTO, BWD_GATHER, BWD_SYMMETRIC = 0, 1, 2
-KERNEL_GENERIC, KERNEL_BWD_ROWS_GATHER, KERNEL_BWD_ROWS_SYMMETRIC, KERNEL_BWD_BLOCK = 100, 201, 202, 203
+KERNEL_GENERIC, KERNEL_BWD_ROWS_GATHER, KERNEL_BWD_ROWS_SYMMETRIC, KERNEL_BWD_BLOCK, KERNEL_BWD_GENERIC = (
+    100, 201, 202, 203, 204)
 
 
 def forward_route(nx, ny, nz, r_p, r_s, path=PATH_AUTO):

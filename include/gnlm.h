@@ -106,14 +106,18 @@ int gnlm3d_forward_noalloc(const float *d_in, float *d_out, float *d_Z, int nx, 
  *   backward: *kernel = GNLM_KERNEL_BWD_ROWS_* (the rows adjoint in the
  *            selected form) while its 16-byte-per-position copy stays below
  *            4 GiB, else GNLM_KERNEL_BWD_BLOCK (the LDS-block adjoint, 64-bit
- *            indexing) where the radii fit its LDS image; GNLM_EUNSUPPORTED
- *            otherwise.
+ *            indexing) where the radii fit its LDS image, else
+ *            GNLM_KERNEL_BWD_GENERIC (any radii: six passes per offset
+ *            through HBM scratch -- slow, but it differentiates every
+ *            geometry, the reference's NS 17 / SS 13 default included);
+ *            GNLM_EUNSUPPORTED at 2^31 voxels or more.
  * GNLM_EINVAL for bad dims, radii, path or a NULL kernel pointer. */
 enum {
     GNLM_KERNEL_GENERIC = 100,
     GNLM_KERNEL_BWD_ROWS_GATHER = 201,
     GNLM_KERNEL_BWD_ROWS_SYMMETRIC = 202,
-    GNLM_KERNEL_BWD_BLOCK = 203
+    GNLM_KERNEL_BWD_BLOCK = 203,
+    GNLM_KERNEL_BWD_GENERIC = 204
 };
 int gnlm3d_forward_route(int nx, int ny, int nz, int r_p, int r_s, int path, int *kernel);
 int gnlm3d_backward_route(int nx, int ny, int nz, int r_p, int r_s, int *kernel);
@@ -139,9 +143,9 @@ int gnlm2d_forward_ex(const float *d_in, int in_pad, long long in_sy, float *d_o
  * workspace released on `stream`.  Supported: r_p 0, 1, 2 with r_s <= 7 (the
  * rows-organised kernels, in the form gnlm_set_backward_form selects; larger
  * r_s as the LDS image of the fallback tile allows) and r_p = 3 with r_s <= 3
- * (the fallback tile); volumes too large for the rows kernels' 32-bit
- * offsets run on the fallback tile where the radii fit it
- * (gnlm3d_backward_route); GNLM_EUNSUPPORTED otherwise. */
+ * (the fallback tile) run on chip; every other radius pair, and volumes too
+ * large for the rows kernels' 32-bit offsets where the fallback tile cannot
+ * take them, run the any-radius per-offset passes (gnlm3d_backward_route). */
 int gnlm3d_backward(const float *d_in, const float *d_out, const float *d_Z, const float *d_gout, int nx,
                     int ny, int nz, int r_p, int r_s, float h, float *d_gin, float *d_gh, void *stream);
 

@@ -647,65 +647,104 @@ int oracle_nlm2d_separable_f32(const float *u, int nx, int ny, int rp, int rs,
 void oracle_nlm3d_grad_f64(const double *u, const double *g, int nx, int ny, int nz,
                            int rp, int rs, double h, double *gu, double *gh)
 {
+    /* The chain rule below runs on the reflect-101 extension ue over
+     * [-R, n+R)^3 (R = rp + rs), which holds every sample the forward reads;
+     * the gradient reaching an extension position is added onto its source
+     * voxel at the end.  Voxels are split over OpenMP threads, each with its
+     * own extended-domain gradient, summed afterwards. */
     const double P = (double)(2 * rp + 1) * (2 * rp + 1) * (2 * rp + 1);
     const double c = 1.0 / (P * h * h);
     const long N = (long)nx * ny * nz;
-    memset(gu, 0, sizeof(double) * N);
+    const int R = rp + rs;
+    const long ex = nx + 2L * R, ey = ny + 2L * R, ez = nz + 2L * R, NE = ex * ey * ez;
+    double *ue = (double *)malloc(sizeof(double) * NE);
+    double *ge = (double *)calloc((size_t)NE, sizeof(double));
+    if (!ue || !ge) {
+        free(ue);
+        free(ge);
+        *gh = NAN;
+        return;
+    }
+    for (long k = 0; k < ez; ++k)
+        for (long j = 0; j < ey; ++j)
+            for (long i = 0; i < ex; ++i)
+                ue[i + ex * (j + ey * k)] = u[R101(i - R, nx) + (long)nx * (R101(j - R, ny) + (long)ny * R101(k - R, nz))];
     double dh = 0.0;
-#define IDX(X, Y, Z) (R101(X, nx) + (long)nx * (R101(Y, ny) + (long)ny * R101(Z, nz)))
-    for (int z = 0; z < nz; ++z)
-        for (int yy = 0; yy < ny; ++yy)
-            for (int x = 0; x < nx; ++x) {
-                const long o = x + (long)nx * (yy + (long)ny * z);
-                /* forward at x */
-                double Z = 0.0, Y = 0.0;
-                for (int dz = -rs; dz <= rs; ++dz)
-                    for (int dy = -rs; dy <= rs; ++dy)
-                        for (int dx = -rs; dx <= rs; ++dx) {
-                            double S = 0.0;
-                            for (int pz = -rp; pz <= rp; ++pz)
-                                for (int py = -rp; py <= rp; ++py)
-                                    for (int px = -rp; px <= rp; ++px) {
-                                        double e = u[IDX(x + px, yy + py, z + pz)] -
-                                                   u[IDX(x + dx + px, yy + dy + py, z + dz + pz)];
-                                        S += e * e;
-                                    }
-                            double w = exp(-c * S);
-                            Z += w;
-                            Y += w * u[IDX(x + dx, yy + dy, z + dz)];
-                        }
-                const double yx = Y / Z;
-                const double gx = g[o];
-                if (gx == 0.0) continue;
-                for (int dz = -rs; dz <= rs; ++dz)
-                    for (int dy = -rs; dy <= rs; ++dy)
-                        for (int dx = -rs; dx <= rs; ++dx) {
-                            double S = 0.0;
-                            for (int pz = -rp; pz <= rp; ++pz)
-                                for (int py = -rp; py <= rp; ++py)
-                                    for (int px = -rp; px <= rp; ++px) {
-                                        double e = u[IDX(x + px, yy + py, z + pz)] -
-                                                   u[IDX(x + dx + px, yy + dy + py, z + dz + pz)];
-                                        S += e * e;
-                                    }
-                            const double w = exp(-c * S);
-                            const double ucand = u[IDX(x + dx, yy + dy, z + dz)];
-                            const double dLdw = gx * (ucand - yx) / Z;
-                            const double A = -c * w * dLdw; /* dL/dS_d(x) */
-                            gu[IDX(x + dx, yy + dy, z + dz)] += gx * w / Z;
-                            dh += dLdw * w * S * 2.0 * c / h;
-                            for (int pz = -rp; pz <= rp; ++pz)
-                                for (int py = -rp; py <= rp; ++py)
-                                    for (int px = -rp; px <= rp; ++px) {
-                                        const long ia = IDX(x + px, yy + py, z + pz);
-                                        const long ib = IDX(x + dx + px, yy + dy + py, z + dz + pz);
-                                        const double e = u[ia] - u[ib];
-                                        gu[ia] += 2.0 * A * e;
-                                        gu[ib] -= 2.0 * A * e;
-                                    }
-                        }
-            }
-#undef IDX
+#define E3(X, Y, Z) ((X) + R + ex * ((Y) + R + ey * ((long)(Z) + R)))
+#pragma omp parallel reduction(+ : dh)
+    {
+        double *gl = (double *)calloc((size_t)NE, sizeof(double));
+#pragma omp for schedule(dynamic, 1) collapse(2)
+        for (int z = 0; z < nz; ++z)
+            for (int yy = 0; yy < ny; ++yy)
+                for (int x = 0; x < nx && gl; ++x) {
+                    const long o = x + (long)nx * (yy + (long)ny * z);
+                    const double gx = g[o];
+                    if (gx == 0.0) continue;
+                    /* forward at x */
+                    double Z = 0.0, Y = 0.0;
+                    for (int dz = -rs; dz <= rs; ++dz)
+                        for (int dy = -rs; dy <= rs; ++dy)
+                            for (int dx = -rs; dx <= rs; ++dx) {
+                                double S = 0.0;
+                                for (int pz = -rp; pz <= rp; ++pz)
+                                    for (int py = -rp; py <= rp; ++py)
+                                        for (int px = -rp; px <= rp; ++px) {
+                                            double e = ue[E3(x + px, yy + py, z + pz)] -
+                                                       ue[E3(x + dx + px, yy + dy + py, z + dz + pz)];
+                                            S += e * e;
+                                        }
+                                double w = exp(-c * S);
+                                Z += w;
+                                Y += w * ue[E3(x + dx, yy + dy, z + dz)];
+                            }
+                    const double yx = Y / Z;
+                    for (int dz = -rs; dz <= rs; ++dz)
+                        for (int dy = -rs; dy <= rs; ++dy)
+                            for (int dx = -rs; dx <= rs; ++dx) {
+                                double S = 0.0;
+                                for (int pz = -rp; pz <= rp; ++pz)
+                                    for (int py = -rp; py <= rp; ++py)
+                                        for (int px = -rp; px <= rp; ++px) {
+                                            double e = ue[E3(x + px, yy + py, z + pz)] -
+                                                       ue[E3(x + dx + px, yy + dy + py, z + dz + pz)];
+                                            S += e * e;
+                                        }
+                                const double w = exp(-c * S);
+                                const double ucand = ue[E3(x + dx, yy + dy, z + dz)];
+                                const double dLdw = gx * (ucand - yx) / Z;
+                                const double A = -c * w * dLdw; /* dL/dS_d(x) */
+                                gl[E3(x + dx, yy + dy, z + dz)] += gx * w / Z;
+                                dh += dLdw * w * S * 2.0 * c / h;
+                                for (int pz = -rp; pz <= rp; ++pz)
+                                    for (int py = -rp; py <= rp; ++py)
+                                        for (int px = -rp; px <= rp; ++px) {
+                                            const long ia = E3(x + px, yy + py, z + pz);
+                                            const long ib = E3(x + dx + px, yy + dy + py, z + dz + pz);
+                                            const double e = ue[ia] - ue[ib];
+                                            gl[ia] += 2.0 * A * e;
+                                            gl[ib] -= 2.0 * A * e;
+                                        }
+                            }
+                }
+        if (gl) {
+#pragma omp critical
+            for (long i = 0; i < NE; ++i) ge[i] += gl[i];
+            free(gl);
+        } else {
+            dh = NAN;
+        }
+    }
+#undef E3
+    /* fold the extension's gradient onto its source voxels */
+    memset(gu, 0, sizeof(double) * N);
+    for (long k = 0; k < ez; ++k)
+        for (long j = 0; j < ey; ++j)
+            for (long i = 0; i < ex; ++i)
+                gu[R101(i - R, nx) + (long)nx * (R101(j - R, ny) + (long)ny * R101(k - R, nz))] +=
+                    ge[i + ex * (j + ey * k)];
+    free(ue);
+    free(ge);
     *gh = dh;
 }
 

@@ -68,6 +68,52 @@ def test_backward_matches_oracle(dev, form, shape, rp, rs, h):
     assert abs(gh - ref_gh) <= 1e-4 * max(abs(ref_gh), 1e-8), (gh, ref_gh)
 
 
+@pytest.mark.parametrize("shape,rp,rs,h", [
+    ((9, 10, 11), 4, 2, 0.3),   # NS 9: past the on-chip adjoints
+    ((8, 9, 10), 3, 4, 0.3),    # NS 7 with SS 9: past the LDS-block tile's image
+    ((10, 9, 8), 8, 1, 0.4),    # NS 17
+    ((7, 8, 6), 2, 8, 0.3),     # SS 17
+    ((6, 7, 8), 8, 6, 10.0),    # the reference's default NS 17 / SS 13 (3DNLM/config.h:4-5), h on 0-255
+])
+def test_backward_any_radius(dev, shape, rp, rs, h):
+    """The any-radius adjoint (per-offset passes through HBM) against the
+    FP64 chain-rule adjoint: same bars as the on-chip forms."""
+    nz, ny, nx = shape
+    assert gnlm.backward_route(nx, ny, nz, rp, rs) == (gnlm.OK, gnlm.KERNEL_BWD_GENERIC)
+    rng = np.random.default_rng(sum(shape) * 3 + rp)
+    u = rng.random(shape).astype(np.float32) * (255.0 if h > 1 else 1.0)
+    g = rng.standard_normal(shape).astype(np.float32)
+    print(f"case bwd generic shape={shape} rp={rp} rs={rs}", flush=True)
+    gu, gh = _gpu_grad(dev, u, g, rp, rs, h)
+    ref_gu, ref_gh = oracle.nlm3d_grad_f64(u.astype(np.float64), g.astype(np.float64), rp, rs, h)
+    scale = np.abs(ref_gu).max()
+    assert np.abs(gu - ref_gu).max() <= 1e-4 * scale, (np.abs(gu - ref_gu).max(), scale)
+    assert abs(gh - ref_gh) <= 1e-4 * max(abs(ref_gh), 1e-8), (gh, ref_gh)
+
+
+def test_backward_any_radius_vs_finite_differences(dev):
+    """NS 9, SS 5 on a 10^3 phantom crop: the any-radius adjoint of the MSE
+    loss against FP64 central finite differences (eps = 1e-4)."""
+    noisy, clean = gnlm.phantom3d(10, 10, 10, n_ellipsoids=3, axis_lo=2, axis_hi=5, sigma=0.05, seed=31)
+    rp, rs, h = 4, 2, 0.15
+    ut = torch.from_numpy(noisy).to(dev).requires_grad_(True)
+    ht = torch.tensor(h, dtype=torch.float32, requires_grad=True)
+    y = nlm3d(ut, ht, rp, rs)
+    loss = ((y - torch.from_numpy(clean).to(dev)) ** 2).mean()
+    loss.backward()
+    gu = ut.grad.cpu().numpy().astype(np.float64)
+    u64, c64 = noisy.astype(np.float64), clean.astype(np.float64)
+    eps = 1e-4
+    scale = np.abs(gu).max()
+    for i in [(0, 0, 0), (5, 4, 3), (9, 9, 9), (2, 7, 1)]:
+        up = u64.copy(); up[i] += eps
+        um = u64.copy(); um[i] -= eps
+        fd = (oracle.nlm3d_mse_f64(up, c64, rp, rs, h) - oracle.nlm3d_mse_f64(um, c64, rp, rs, h)) / (2 * eps)
+        assert abs(fd - gu[i]) <= 1e-4 * max(abs(fd), scale), (i, fd, gu[i])
+    fdh = (oracle.nlm3d_mse_f64(u64, c64, rp, rs, h + eps) - oracle.nlm3d_mse_f64(u64, c64, rp, rs, h - eps)) / (2 * eps)
+    assert abs(float(ht.grad) - fdh) <= 1e-4 * abs(fdh), (float(ht.grad), fdh)
+
+
 def test_backward_deterministic(dev, form):
     rng = np.random.default_rng(1)
     u = rng.random((20, 22, 24)).astype(np.float32)

@@ -65,12 +65,14 @@ def test_backward_700_cubed():
     4 GiB near 620^3 at r_p 1 / r_s 4; the LDS-block adjoint takes over."""
     assert gnlm.backward_route(128, 128, 128, 1, 4) == (gnlm.OK, gnlm.KERNEL_BWD_ROWS_SYMMETRIC)
     assert gnlm.backward_route(700, 700, 700, 1, 4) == (gnlm.OK, gnlm.KERNEL_BWD_BLOCK)
-    # radii the LDS-block tile cannot hold: unsupported, not invalid
-    st, _ = gnlm.backward_route(700, 700, 700, 2, 7)
+    # radii the LDS-block tile cannot hold: the any-radius passes
+    assert gnlm.backward_route(700, 700, 700, 2, 7) == (gnlm.OK, gnlm.KERNEL_BWD_GENERIC)
+    # 2^31 voxels and up: unsupported (not invalid), returned before the device
+    st, _ = gnlm.backward_route(2048, 1024, 1024, 2, 7)
     assert st == gnlm.EUNSUPPORTED
     L = gnlm.lib()
     fake = ctypes.c_void_p(0x1000)
-    assert L.gnlm3d_backward(fake, fake, fake, fake, 700, 700, 700, 2, 7, 0.1, fake, None,
+    assert L.gnlm3d_backward(fake, fake, fake, fake, 2048, 1024, 1024, 2, 7, 0.1, fake, None,
                              None) == gnlm.EUNSUPPORTED
 
 
@@ -82,8 +84,9 @@ def test_backward_form_switch_routes():
         gnlm.set_backward_form(gnlm.BWD_AUTO)
     assert gnlm.backward_route(64, 64, 64, 2, 3) == (gnlm.OK, gnlm.KERNEL_BWD_ROWS_SYMMETRIC)
     assert gnlm.backward_route(40, 40, 40, 3, 3) == (gnlm.OK, gnlm.KERNEL_BWD_BLOCK)
-    st, _ = gnlm.backward_route(40, 40, 40, 4, 3)
-    assert st == gnlm.EUNSUPPORTED
+    # patches past the on-chip adjoints, up to the reference's NS 17 / SS 13
+    for rp, rs in ((4, 3), (3, 5), (8, 6)):
+        assert gnlm.backward_route(40, 40, 40, rp, rs) == (gnlm.OK, gnlm.KERNEL_BWD_GENERIC)
 
 
 @pytest.mark.parametrize("args", [(0, 8, 8, 1, 2), (8, 8, 8, -1, 2), (8, 8, 8, 1, -2)])
