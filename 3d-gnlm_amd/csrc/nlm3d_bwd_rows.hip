@@ -52,7 +52,11 @@ struct BwdRowsCfg {
     static constexpr bool SYM = SYM_;
     static constexpr int H = 2 * R + RP;                  // halo of the padded copies (all axes)
     // gradient tile T, grown tile Q, phase-1 rows D
-    static constexpr int QY = 16, QZ = 12;
+    // The symmetric form keeps a scatter accumulator in LDS.  For r_p <= 1 its
+    // tiles are 8 positions deep in z, so that LDS (about 47 KB at r_s 4) and
+    // registers fit three workgroups per CU; otherwise 12 deep, two per CU.
+    static constexpr int QY = 16, QZ = (SYM && RP <= 1) ? 8 : 12;
+    static constexpr int WAVES = SYM ? (QZ == 8 ? 3 : 2) : 3;  // waves per SIMD the registers are budgeted for
     static constexpr int GX = 16 - 2 * RP, GY = QY - 2 * RP, GZ = QZ - 2 * RP;
     static constexpr int DY = QY + 2 * RP, DZ = QZ + 2 * RP;
     // names the shared phase-1/2 code uses
@@ -68,10 +72,11 @@ struct BwdRowsCfg {
     static constexpr int P2N = NXV * DZ * (QY / RY);
     static_assert(P2N <= NT, "one phase-2 item per thread");
     // LDS: s_x [z_d][y_d][x] (row pitch 20, plane pitch 400 == 16 mod 64),
-    //      s_xy [z_d][y_q][x] (plane pitch 272), boxed E [z_t][y_q][x]
+    //      s_xy [z_d][y_q][x] (plane pitch 272); the boxed E [z_t][y_q][x]
+    //      shares s_xy's layout and memory (below)
     static constexpr int SXR = 20, SXP = (SXR * DY - 16 + 63) / 64 * 64 + 16, SXN = SXP * DZ;
     static constexpr int SYP = 16 * QY + 16, SYN = SYP * DZ;
-    static constexpr int EZP = 16 * QY, EZN = EZP * GZ;
+    static constexpr int EZP = SYP;
     // SYM: scatter accumulator over T grown by RS in x and y and by RS towards
     // +z (the visited half of the offsets has dz >= 0), [z][y][x]; the plane
     // pitch is a multiple of 64 floats so that a thread's z pairs share one
@@ -79,7 +84,7 @@ struct BwdRowsCfg {
     static constexpr int SCX = GX + 2 * RS, SCY = GY + 2 * RS, SCZ = GZ + RS;
     static constexpr int SCP = (SCX * SCY + 63) / 64 * 64;
     static constexpr int SCN = SYM ? SCP * SCZ : 0;
-    static constexpr int LDS_FLOATS = SXN + 16 + SYN + EZN + SCN;  // + a spare s_x row
+    static constexpr int LDS_FLOATS = SXN + 16 + SYN + SCN;  // + a spare s_x row
 };
 
 struct BwdRowsArgs {
@@ -108,15 +113,22 @@ __device__ __forceinline__ float4 pad_load4(const PadRsrc &r, unsigned voff, int
 }
 
 template <int RP_, int RS, bool SYM>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SYM ? 2 : 3))) void nlm3d_bwd_rows(BwdRowsArgs a)
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(BwdRowsCfg<RP_, RS, SYM>::WAVES))) void nlm3d_bwd_rows(
+    BwdRowsArgs a)
 {
     using C = BwdRowsCfg<RP_, RS, SYM>;
     constexpr int RP = C::RP, QZ = C::QZ, GZ = C::GZ;
     extern __shared__ __align__(16) float lds[];
     float *SX = lds;
     float *SXY = SX + C::SXN + 16;
-    float *EZB = SXY + C::SYN;
-    float *SC = EZB + C::EZN;  // SYM only
+    // The boxed E of plane z and column (x_q, y_q) sits in s_xy's slot of the
+    // same column and plane.  Phase 3 reads only its own column of s_xy, all
+    // of it before writing that column's boxed E; phase 4 reads neighbouring
+    // columns after the barrier that follows phase 3; the next phase 2 writes
+    // s_xy only after the next offset's first barrier, which every thread
+    // reaches after its phase 4.
+    float *EZB = SXY;
+    float *SC = SXY + C::SYN;  // SYM only
     const int tid = threadIdx.x;
     const PadRsrc ru = pad_rsrc(a.pu);
     const PadRsrc rg = pad_rsrc(a.pg);
@@ -237,8 +249,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SYM ? 2 : 3)
         }
     };
     // The shifted samples of this thread's phase-1 row for all dx of one
-    // (dy, dz): loaded once (a lane reads its own row, so these loads touch a
-    // cache line per lane) and slid by one sample per dx in registers.
+    // (dy, dz): loaded once at the row start (a lane reads its own row, so
+    // these loads touch a cache line per lane).  dx is unrolled, so offset j
+    // of a row reads win[i + j] at a static register index.
     constexpr int NWIN = C::EX + 2 * RS, NWINX = EXQ + 2 * RS;
     float win[NWIN];
     float winx[NWINX];
@@ -248,130 +261,145 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(SYM ? 2 : 3)
     int scd = dzb * C::SCP + dyb * C::SCX + dxb;   // SYM: scatter offset of d (dz >= 0)
     load_col(sd, ud, Gd, GYd);
 
-    for (int o = ob; o < oe; ++o) {
-        const int sn = (o + 1 < oe) ? sd_next(sd, o) : sd;  // next offset (reloads the last one at the end)
-        if (o == ob || o % NW == 0) {  // a new (dy, dz) (SYM starts at dx = 1: window[m] = sample at d + m)
+    for (int row = ob / NW; row * NW < oe; ++row) {
+        // this part's offsets of the row: j in [jb, je) (wave-uniform)
+        const int jb = ob > row * NW ? ob - row * NW : 0;
+        const int je = oe < row * NW + NW ? oe - row * NW : NW;
+        {
+            const int sr = sd - jb;  // scalar offset of the row start (dx = -RS)
 #pragma unroll
             for (int m = 0; m < NWIN; m += 2) {  // row starts are 8-byte aligned (even padded x)
-                const float2 t = pad_load2(ru, rbase, 4 * (sd + m));
+                const float2 t = pad_load2(ru, rbase, 4 * (sr + m));
                 win[m] = t.x;
                 win[m + 1] = t.y;
             }
             if constexpr (C::NRX > 0) {
 #pragma unroll
                 for (int m = 0; m < NWINX; m += 2) {
-                    const float2 t = pad_load2(ru, rbasex, 4 * (sd + m));
+                    const float2 t = pad_load2(ru, rbasex, 4 * (sr + m));
                     winx[m] = t.x;
                     winx[m + 1] = t.y;
                 }
             }
         }
-        // phase 1: x sums of squared differences on the rows of D
-        {
-            float qv[C::EX];
+        static_for<0, NW - 1>([&](auto jc) {
+            constexpr int j = decltype(jc)::value;
+            if (j < jb || j >= je) return;
+            const int o = row * NW + j;
+            const int sn = (o + 1 < oe) ? sd_next(sd, o) : sd;  // next offset (reloads the last one at the end)
+            // phase 1: x sums of squared differences on the rows of D
+            {
+                float qv[C::EX];
 #pragma unroll
-            for (int i = 0; i < C::EX; ++i) {
-                const float e = cen[i] - win[i];
-                qv[i] = e * e;
-            }
-            float sx[C::TX];
-            box_tree<C::W, C::TX>(qv, sx);
-            float4 *dst = reinterpret_cast<float4 *>(SX + sxw);
-#pragma unroll
-            for (int j = 0; j < C::TX / 4; ++j)
-                dst[j] = make_float4(sx[4 * j], sx[4 * j + 1], sx[4 * j + 2], sx[4 * j + 3]);
-            if constexpr (C::NRX > 0) {
-                float qx[EXQ];
-#pragma unroll
-                for (int i = 0; i < EXQ; ++i) {
-                    const float e = cenx[i] - winx[i];
-                    qx[i] = e * e;
+                for (int i = 0; i < C::EX; ++i) {
+                    const float e = cen[i] - win[i + j];
+                    qv[i] = e * e;
                 }
-                float s4[4];
-                box_tree<C::W, 4>(qx, s4);
-                *reinterpret_cast<float4 *>(SX + sxwx) = make_float4(s4[0], s4[1], s4[2], s4[3]);
+                float sx[C::TX];
+                box_tree<C::W, C::TX>(qv, sx);
+                float4 *dst = reinterpret_cast<float4 *>(SX + sxw);
+#pragma unroll
+                for (int q = 0; q < C::TX / 4; ++q)
+                    dst[q] = make_float4(sx[4 * q], sx[4 * q + 1], sx[4 * q + 2], sx[4 * q + 3]);
+                if constexpr (C::NRX > 0) {
+                    float qx[EXQ];
+#pragma unroll
+                    for (int i = 0; i < EXQ; ++i) {
+                        const float e = cenx[i] - winx[i + j];
+                        qx[i] = e * e;
+                    }
+                    float s4[4];
+                    box_tree<C::W, 4>(qx, s4);
+                    *reinterpret_cast<float4 *>(SX + sxwx) = make_float4(s4[0], s4[1], s4[2], s4[3]);
+                }
             }
-        }
-        // slide the windows to the next dx
+            lds_barrier();
+            // phase 2: y sums
+            if (p2ok) rows_phase2<C>(SX + p2src, SXY + p2dst);
+            lds_barrier();
+            // phase 3: z sums, weights, E, the G(z+d) w term, dL/dh terms
+            float v[QZ + 2 * RP];
 #pragma unroll
-        for (int m = 0; m < NWIN - 1; ++m) win[m] = win[m + 1];
-        if constexpr (C::NRX > 0) {
+            for (int i = 0; i < QZ + 2 * RP; ++i) v[i] = SXY[i * C::SYP + sxyr];
+            float S[QZ];
+            box_tree<C::W, QZ>(v, S);
+            float EX[QZ], du[GZ], Gw[GZ];
 #pragma unroll
-            for (int m = 0; m < NWINX - 1; ++m) winx[m] = winx[m + 1];
-        }
-        lds_barrier();
-        // phase 2: y sums
-        if (p2ok) rows_phase2<C>(SX + p2src, SXY + p2dst);
-        lds_barrier();
-        // phase 3: z sums, weights, E, the G(z+d) w term, dL/dh terms
-        float v[QZ + 2 * RP];
-#pragma unroll
-        for (int j = 0; j < QZ + 2 * RP; ++j) v[j] = SXY[j * C::SYP + sxyr];
-        float S[QZ];
-        box_tree<C::W, QZ>(v, S);
-        float EX[QZ], du[GZ], Gw[GZ];
-#pragma unroll
-        for (int z = 0; z < QZ; ++z) {
-            const float w = exp2_fast(S[z] * kw);
-            const float t1 = fmaf(Gc[z], ud[z], -GYc[z]);
-            const float t2 = fmaf(Gd[z], uc[z], -GYd[z]);
-            const float E = (negc2 * w) * (t1 + t2);
-            if (z >= RP && z < QZ - RP) {
-                gacc[z - RP] = fmaf(Gd[z], w, gacc[z - RP]);
-                dh = fmaf(S[z], E, dh);
-                du[z - RP] = uc[z] - ud[z];
-                if constexpr (SYM) Gw[z - RP] = Gc[z] * w;
+            for (int z = 0; z < QZ; ++z) {
+                const float w = exp2_fast(S[z] * kw);
+                const float t1 = fmaf(Gc[z], ud[z], -GYc[z]);
+                const float t2 = fmaf(Gd[z], uc[z], -GYd[z]);
+                // The empty asm makes E a value of its own: otherwise its
+                // product is fused into the x box's first add (an fma whose
+                // addend is the shifted lane), and that add can no longer take
+                // the DPP operand itself -- one extra move per z.  Every use of
+                // E reads the pinned value, so the pin itself needs no copy.
+                float E = (negc2 * w) * (t1 + t2);
+                asm("" : "+v"(E));
+                if (z >= RP && z < QZ - RP) {
+                    gacc[z - RP] = fmaf(Gd[z], w, gacc[z - RP]);
+                    dh = fmaf(S[z], E, dh);
+                    du[z - RP] = uc[z] - ud[z];
+                    if constexpr (SYM) Gw[z - RP] = Gc[z] * w;
+                }
+                // x box across the 16-lane row
+                EX[z] = dpp_box<RP>(E);
             }
-            // x box across the 16-lane row.  The empty asm makes E a value of
-            // its own: otherwise its product is fused into the box's first add
-            // (an fma whose addend is the shifted lane), and that add can no
-            // longer take the DPP operand itself -- one extra move per z.
-            float Eo = E;
-            asm("" : "+v"(Eo));
-            EX[z] = dpp_box<RP>(Eo);
-        }
-        // the next offset's column values fly across phase 4 and the next phases 1-2
-        load_col(sn, ud, Gd, GYd);
-        float EZ[GZ];
-        box_tree<C::W, GZ>(EX, EZ);
-#pragma unroll
-        for (int z = 0; z < GZ; ++z) EZB[z * C::EZP + ezw] = EZ[z];
-        lds_barrier();
-        // phase 4: y box of the boxed E, the 2 boxE (u~(z) - u~(z+d)) term;
-        // SYM: the -d term of position x + d, G(x) w_d(x) + boxE_d(x) (u~(x+d) -
-        // u~(x)), added into the accumulator (one add per position and offset:
-        // the order of the adds is fixed)
-        if constexpr (SYM) {
-            float A[GZ];
-#pragma unroll
-            for (int z = 0; z < GZ; ++z) {  // all reads first: the adds below may alias them
-                const float bE = ybox(z);
-                const float pz = bE * du[z];
-                gacc[z] = gacc[z] + pz;
-                A[z] = Gw[z] - pz;
-            }
-            // Plain read-add-write: within one offset the targets x + d of
-            // distinct positions are distinct, and the barriers between
-            // offsets order the updates (LDS float atomics cost ~12x as much).
-            if (inT) {
-                float *sc = SC + scb + scd;
-                float cur[GZ];
-#pragma unroll
-                for (int z = 0; z < GZ; ++z) cur[z] = sc[z * C::SCP];
-#pragma unroll
-                for (int z = 0; z < GZ; ++z) sc[z * C::SCP] = cur[z] + A[z];
-            }
-            // scatter offset of the next d: dz SCP + dy SCX + dx
-            const int ix = o % NW;
-            scd += ix < NW - 1 ? 1 : ((o / NW) % NW < NW - 1 ? C::SCX - (NW - 1) : C::SCP - (NW - 1) * C::SCX - (NW - 1));
-        } else {
+            // The next offset's column values fly across phase 4 and the next
+            // phases 1-2.  Every value computed from this offset's ones is
+            // pinned before the loads (volatile asm keeps the loads below it),
+            // so the loads can land in the same registers: no copies.
 #pragma unroll
             for (int z = 0; z < GZ; ++z) {
-                const float bE = ybox(z);
-                gacc[z] = fmaf(bE, du[z], gacc[z]);
+                asm volatile("" : "+v"(gacc[z]), "+v"(du[z]));
+                if constexpr (SYM) asm volatile("" : "+v"(Gw[z]));
             }
-        }
-        sd = sn;
+#pragma unroll
+            for (int z = 0; z < QZ; ++z) asm volatile("" : "+v"(EX[z]));
+            asm volatile("" : "+v"(dh));
+            load_col(sn, ud, Gd, GYd);
+            float EZ[GZ];
+            box_tree<C::W, GZ>(EX, EZ);
+#pragma unroll
+            for (int z = 0; z < GZ; ++z) EZB[z * C::EZP + ezw] = EZ[z];
+            lds_barrier();
+            // phase 4: y box of the boxed E, the 2 boxE (u~(z) - u~(z+d))
+            // term; SYM: the -d term of position x + d, G(x) w_d(x) + boxE_d(x)
+            // (u~(x+d) - u~(x)), added into the accumulator (one add per
+            // position and offset: the order of the adds is fixed)
+            if constexpr (SYM) {
+                float A[GZ];
+#pragma unroll
+                for (int z = 0; z < GZ; ++z) {  // all reads first: the adds below may alias them
+                    const float bE = ybox(z);
+                    const float pz = bE * du[z];
+                    gacc[z] = gacc[z] + pz;
+                    A[z] = Gw[z] - pz;
+                }
+                // Plain read-add-write: within one offset the targets x + d of
+                // distinct positions are distinct, and the barriers between
+                // offsets order the updates (LDS float atomics cost ~12x as much).
+                if (inT) {
+                    float *sc = SC + scb + scd;
+                    float cur[GZ];
+#pragma unroll
+                    for (int z = 0; z < GZ; ++z) cur[z] = sc[z * C::SCP];
+#pragma unroll
+                    for (int z = 0; z < GZ; ++z) sc[z * C::SCP] = cur[z] + A[z];
+                }
+                // scatter offset of the next d: dz SCP + dy SCX + dx
+                scd += j < NW - 1 ? 1
+                                  : ((o / NW) % NW < NW - 1 ? C::SCX - (NW - 1)
+                                                            : C::SCP - (NW - 1) * C::SCX - (NW - 1));
+            } else {
+#pragma unroll
+                for (int z = 0; z < GZ; ++z) {
+                    const float bE = ybox(z);
+                    gacc[z] = fmaf(bE, du[z], gacc[z]);
+                }
+            }
+            sd = sn;
+        });
     }
 
     if constexpr (SYM) {

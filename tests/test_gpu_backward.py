@@ -134,16 +134,17 @@ def test_backward_forms_agree_rp2(dev):
 
 
 def test_backward_forms_agree_whole_tiles(dev):
-    """88^3: 490 tiles of the symmetric form at r_p 1, r_s 4, within one round
-    of 512 workgroup slots on a 256-CU device, so each workgroup takes a whole
-    tile.  The 40^3 volumes above split each tile's offset rows over several
-    workgroups; both layouts agree with the gather form."""
-    _forms_agree(dev, 1, 4, n=88)
+    """78^3: 735 tiles of the symmetric form at r_p 1, r_s 4, within one round
+    of 768 workgroup slots (three per CU on a 256-CU device), so each
+    workgroup takes a whole tile.  The 40^3 volumes above split each tile's
+    offset rows over several workgroups; both layouts agree with the gather
+    form."""
+    _forms_agree(dev, 1, 4, n=78)
 
 
 def test_backward_forms_agree_three_parts(dev):
-    """70^3: 288 tiles at r_p 1, r_s 4, so on a 256-CU device each tile's 40
-    offset rows go to 3 workgroups of 13, 13 and 14 rows."""
+    """70^3: 504 tiles at r_p 1, r_s 4, so on a 256-CU device (768 slots) each
+    tile's 40 offset rows go to 3 workgroups of 13, 13 and 14 rows."""
     _forms_agree(dev, 1, 4, n=70)
 
 
