@@ -84,7 +84,7 @@ struct BwdRowsCfg {
     static constexpr int SCX = GX + 2 * RS, SCY = GY + 2 * RS, SCZ = GZ + RS;
     static constexpr int SCP = (SCX * SCY + 63) / 64 * 64;
     static constexpr int SCN = SYM ? SCP * SCZ : 0;
-    static constexpr int LDS_FLOATS = SXN + 16 + SYN + SCN;  // + a spare s_x row
+    static constexpr int LDS_FLOATS = SXN + SYN + SCN;
 };
 
 struct BwdRowsArgs {
@@ -120,7 +120,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(BwdRowsCfg<R
     constexpr int RP = C::RP, QZ = C::QZ, GZ = C::GZ;
     extern __shared__ __align__(16) float lds[];
     float *SX = lds;
-    float *SXY = SX + C::SXN + 16;
+    float *SXY = SX + C::SXN;
     // The boxed E of plane z and column (x_q, y_q) sits in s_xy's slot of the
     // same column and plane.  Phase 3 reads only its own column of s_xy, all
     // of it before writing that column's boxed E; phase 4 reads neighbouring
@@ -146,17 +146,22 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(BwdRowsCfg<R
     auto pidx = [&](int x, int y, int z) { return (z + C::H) * pxy + (y + C::H) * px + (x + C::H); };
 
     // ---- phase-1 row (y_d, z_d) of D; element m of a loaded row is x = Xt - RP - R + m ----
-    const bool p1ok = tid < C::NROWS;
-    const int r1 = p1ok ? tid : 0;
+    // Waves with no row skip phase 1 (a scalar branch); in a partly filled
+    // wave the lanes past the last row repeat it, and only their LDS stores
+    // are masked, so phase 1 has no divergent control flow.
+    const bool p1own = tid < C::NROWS;
+    const bool p1wav = C::NROWS >= NT || __builtin_amdgcn_readfirstlane(tid >> 6) * 64 < C::NROWS;
+    const int r1 = p1own ? tid : C::NROWS - 1;
     const int yd = r1 % C::DY, zd = r1 / C::DY;
-    // surplus threads (tid >= NROWS) compute row 0 again into a spare LDS row
     // per-lane offsets sit at d = (-RS, -RS, -RS): the scalar offsets added
     // below (unsigned in the buffer address) are then all >= 0
     const unsigned rbase = 4u * (unsigned)pidx(Xt - RP - C::R, Yt - 2 * RP + yd - RS, Zt - 2 * RP + zd - RS);
-    const int sxw = p1ok ? zd * C::SXP + yd * C::SXR : C::SXN;
+    const int sxw = zd * C::SXP + yd * C::SXR;
     float cen[C::EX];
+    if (p1wav) {
 #pragma unroll
-    for (int i = 0; i < C::EX; ++i) cen[i] = pad_load(ru, rbase, 4 * (RS * pxy + RS * px + RS + i));
+        for (int i = 0; i < C::EX; ++i) cen[i] = pad_load(ru, rbase, 4 * (RS * pxy + RS * px + RS + i));
+    }
     // quarter of an extra row (rows NR1 ... NROWS - 1): 4 x outputs from 4 + 2 RP centres
     constexpr int EXQ = 4 + 2 * RP;
     const int xr = C::NR1 + tid / 4, xj = tid % 4;
@@ -265,7 +270,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(BwdRowsCfg<R
         // this part's offsets of the row: j in [jb, je) (wave-uniform)
         const int jb = ob > row * NW ? ob - row * NW : 0;
         const int je = oe < row * NW + NW ? oe - row * NW : NW;
-        {
+        if (p1wav) {
             const int sr = sd - jb;  // scalar offset of the row start (dx = -RS)
 #pragma unroll
             for (int m = 0; m < NWIN; m += 2) {  // row starts are 8-byte aligned (even padded x)
@@ -288,7 +293,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(BwdRowsCfg<R
             const int o = row * NW + j;
             const int sn = (o + 1 < oe) ? sd_next(sd, o) : sd;  // next offset (reloads the last one at the end)
             // phase 1: x sums of squared differences on the rows of D
-            {
+            if (p1wav) {
                 float qv[C::EX];
 #pragma unroll
                 for (int i = 0; i < C::EX; ++i) {
@@ -298,9 +303,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(BwdRowsCfg<R
                 float sx[C::TX];
                 box_tree<C::W, C::TX>(qv, sx);
                 float4 *dst = reinterpret_cast<float4 *>(SX + sxw);
+                if (p1own) {
 #pragma unroll
-                for (int q = 0; q < C::TX / 4; ++q)
-                    dst[q] = make_float4(sx[4 * q], sx[4 * q + 1], sx[4 * q + 2], sx[4 * q + 3]);
+                    for (int q = 0; q < C::TX / 4; ++q)
+                        dst[q] = make_float4(sx[4 * q], sx[4 * q + 1], sx[4 * q + 2], sx[4 * q + 3]);
+                }
                 if constexpr (C::NRX > 0) {
                     float qx[EXQ];
 #pragma unroll
