@@ -251,3 +251,47 @@ def test_fit_h_decreases_loss(dev):
         losses.append(loss.item())
     print("fit-h losses", losses[0], losses[-1], "h", float(torch.exp(logh.detach())))
     assert losses[-1] < losses[0]
+
+
+SWEEP_DIMS = (1, 2, 3, 5, 7, 13, 17, 29, 31)
+
+
+def _sweep_cases(n=24, seed=20261204):
+    rng = np.random.default_rng(seed)
+    cases = []
+    for _ in range(n):
+        shape = tuple(int(v) for v in rng.choice(SWEEP_DIMS, 3))
+        cases.append((shape, int(rng.integers(0, 3)), int(rng.integers(1, 8))))
+    return cases
+
+
+@pytest.mark.parametrize("shape,rp,rs", _sweep_cases())
+def test_backward_sweep(dev, shape, rp, rs):
+    """Seeded shapes from 1 to 31 per axis (thin volumes, partial tiles on
+    every axis, several reflections of one sample) and radii r_p 0-2, r_s
+    1-7: the symmetric and gather forms agree to FP32 rounding, and on the
+    smaller cases both match the FP64 chain-rule adjoint."""
+    rng = np.random.default_rng(sum(shape) * 11 + 7 * rp + rs)
+    u = rng.random(shape).astype(np.float32)
+    g = rng.standard_normal(shape).astype(np.float32)
+    print(f"case bwd sweep shape={shape} rp={rp} rs={rs}", flush=True)
+    try:
+        gnlm.set_backward_form(gnlm.BWD_GATHER)
+        a = _gpu_grad(dev, u, g, rp, rs, 0.2)
+        gnlm.set_backward_form(gnlm.BWD_SYMMETRIC)
+        b = _gpu_grad(dev, u, g, rp, rs, 0.2)
+    finally:
+        gnlm.set_backward_form(gnlm.BWD_AUTO)
+    scale = max(np.abs(a[0]).max(), 1e-30)
+    assert np.abs(a[0] - b[0]).max() <= 2e-5 * scale, np.abs(a[0] - b[0]).max() / scale
+    # dL/dh is a sum of terms of both signs over every position and offset;
+    # on volumes of a few hundred voxels it cancels to a fraction of its
+    # terms' size (1/200 on some of these), so the two forms are compared at
+    # the oracle's bar
+    assert abs(a[1] - b[1]) <= 1e-4 * max(abs(a[1]), 1e-8), (a[1], b[1])
+    if u.size * (2 * rs + 1) ** 3 <= 10_000_000:
+        ref_gu, ref_gh = oracle.nlm3d_grad_f64(u.astype(np.float64), g.astype(np.float64), rp, rs, 0.2)
+        rscale = max(np.abs(ref_gu).max(), 1e-30)
+        for gu, gh in (a, b):
+            assert np.abs(gu - ref_gu).max() <= 1e-4 * rscale, np.abs(gu - ref_gu).max() / rscale
+            assert abs(gh - ref_gh) <= 1e-4 * max(abs(ref_gh), 1e-8), (gh, ref_gh)

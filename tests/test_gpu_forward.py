@@ -56,6 +56,33 @@ CASES3 = [
 ]
 
 
+SWEEP_DIMS = (1, 2, 3, 5, 7, 13, 17, 29, 31)
+
+
+def _sweep3(n=32, seed=20261204):
+    rng = np.random.default_rng(seed)
+    cases = []
+    for _ in range(n):
+        shape = tuple(int(v) for v in rng.choice(SWEEP_DIMS, 3))
+        cases.append((shape, int(rng.integers(0, 5)), int(rng.integers(0, 8)), float(rng.uniform(0.1, 0.4))))
+    return cases
+
+
+@pytest.mark.parametrize("shape,rp,rs,h", _sweep3())
+def test_3d_sweep(dev, shape, rp, rs, h):
+    """Seeded shapes from 1 to 31 per axis (thin volumes, partial tiles on
+    every axis, repeated reflection) and radii r_p 0-4, r_s 0-7 on the default
+    path: within 1e-5 of the CPU FP32 path and of FP64."""
+    rng = np.random.default_rng(sum(shape) * 13 + 5 * rp + rs)
+    u = rng.random(shape, dtype=np.float32)
+    print(f"case 3d sweep shape={shape} rp={rp} rs={rs} h={h:.3f}", flush=True)
+    y = _run3(dev, u, rp, rs, h)
+    ref32 = oracle.nlm3d_separable_f32(u, rp, rs, h)
+    ref64 = oracle.nlm3d_separable_f64(u.astype(np.float64), rp, rs, h)
+    assert max_rel(y, ref32) <= TOL, report(y, ref32)
+    assert max_rel(y, ref64) <= TOL, report(y, ref64)
+
+
 @pytest.mark.parametrize("shape,rp,rs,h", CASES3)
 def test_3d_matches_oracle(dev, shape, rp, rs, h):
     rng = np.random.default_rng(hash((shape, rp, rs)) & 0xFFFF)
