@@ -5,11 +5,12 @@
 // exp(-(S/nb^2)/PARAMETER^2), weighted mean of the candidate centres), with
 // the patch radius generic instead of hard-wired to nb = 11.
 //
-// One workgroup (256 threads) per 64 x 32 output tile:
+// One workgroup (256 threads) per 32 x 64 output tile (x by y; taller than
+// wide, so the y halo of the phase-1 rows is a smaller share):
 //   * the tile's input with its r_p + r_s halo is staged into LDS with
 //     reflect-101 applied on the fly;
 //   * per offset, phase 1 forms x-direction patch-row sums (s_x) for the
-//     tile's 32 + 2 r_p rows in segments of 16 outputs; each thread keeps its
+//     tile's 64 + 2 r_p rows in segments of 16 outputs; each thread keeps its
 //     segment's centre samples in registers for the whole offset loop and
 //     reads only the shifted samples from LDS.  The sums are shared pairwise
 //     trees over the differences (half of the squares folded into fused
@@ -37,7 +38,7 @@ constexpr int NT = 256;
 
 template <int RP>
 struct G2 {
-    static constexpr int TX = 64, TY = 32;
+    static constexpr int TX = 32, TY = 64;
     static constexpr int SEG = 16;                  // outputs per phase-1 segment
     static constexpr int NSEG = TX / SEG;           // segments per row
     static constexpr int EY = TY + 2 * RP;          // phase-1 rows
