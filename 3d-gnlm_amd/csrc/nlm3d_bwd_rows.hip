@@ -11,17 +11,19 @@
 // reflect-101 preimages afterwards.
 //
 // Organisation (one workgroup of 256 threads per gradient tile T of
-// (16 - 2 r_p) x (16 - 2 r_p) x (12 - 2 r_p) positions, 14 x 14 x 10 for
-// r_p = 1; Q = T grown by r_p = 16 x 16 x 12 is where S, w and E live; D = Q
-// grown by r_p holds the squared differences):
+// (16 - 2 r_p) x (16 - 2 r_p) x (QZ - 2 r_p) positions; Q = T grown by r_p =
+// 16 x 16 x QZ is where S, w and E live, QZ = 8 in the symmetric form for
+// r_p <= 1 (T = 14 x 14 x 6 at r_p = 1) and 12 otherwise; D = Q grown by
+// r_p holds the squared differences):
 //   * a pre-pass writes a padded copy of u~ and a padded, interleaved
 //     (G, GY, u~, 0) array ((G, GY) zero outside the volume); the tile reads
 //     both with buffer loads whose per-lane offsets are fixed for the tile
 //     (one 128-bit load per column position and offset);
 //   * per offset d (dz, dy, dx):
-//       phase 1  x box sums of squared differences on the 18 x 14 rows of D
-//                -> LDS; the centre samples of a row stay in registers, the
-//                shifted ones are read per offset (L1 hits across dx);
+//       phase 1  x box sums of squared differences on the rows of D
+//                (18 x 10 at r_p = 1) -> LDS; the centre samples of a row
+//                stay in registers, the shifted ones are loaded once per
+//                (dy, dz) and dx is unrolled;
 //       phase 2  y box sums -> LDS (the forward's phase 2);
 //       phase 3  one thread per Q column (x = lane in a 16-lane row, y = row
 //                of lanes): z box sums -> S, w, E, the G(z+d) w term and the
