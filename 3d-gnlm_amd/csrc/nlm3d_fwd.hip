@@ -158,10 +158,12 @@ bool fwd3d_tiled_supported(int r_p, int r_s)
 int fwd3d_num_shapes(int r_p, int r_s, int variant)
 {
     if (!fwd3d_variant_supported(r_p, r_s, variant)) return 0;
-    // r_p = 4: the 16 x 8 x 8 rows tiles and the 16 x 24 x 24 big tiles pair
-    // their tree outputs from even run starts alike, so they give the same bits
+    // r_p = 4: the 16 x 8 x 8 rows tiles (and their deep 16 x 8 x 16 form) and
+    // the 16 x 24 x 24 big tiles pair their tree outputs from even run starts
+    // alike, so they give the same bits.  Shapes: 0 = the variant itself,
+    // 1 = the other of rows / big, 2 = deep rows.
     if (r_p == 4 && (variant == kVRows || variant == kVRowsBig))
-        return fwd3d_variant_supported(4, r_s, kVRows) && fwd3d_variant_supported(4, r_s, kVRowsBig) ? 2 : 1;
+        return fwd3d_variant_supported(4, r_s, kVRows) && fwd3d_variant_supported(4, r_s, kVRowsBig) ? 3 : 1;
     if (fwd3_is_rows(variant)) return fwd3d_rows_num_shapes(r_p, variant == kVRowsShort);
     return 1;
 }
@@ -172,7 +174,8 @@ hipError_t fwd3d_tiled_launch_variant(const InDesc &in, float *out, float *Zout,
     if (!fwd3d_variant_supported(r_p, r_s, variant)) return hipErrorInvalidValue;
     if (shape >= fwd3d_num_shapes(r_p, r_s, variant)) return hipErrorInvalidValue;
     if (r_p == 4 && (variant == kVRows || variant == kVRowsBig) && shape >= 0) {
-        // shape 0 = the variant itself, 1 = the other r_p = 4 tile (same bits)
+        // shape 0 = the variant itself, 1 = the other r_p = 4 tile, 2 = deep rows (same bits)
+        if (shape == 2) return fwd3d_rows_launch(in, out, Zout, 4, r_s, k, false, s, ws, 1);
         const bool big = (variant == kVRowsBig) != (shape == 1);
         return big ? fwd3d_rows_big_launch(in, out, Zout, r_p, r_s, k, s, ws)
                    : fwd3d_rows_launch(in, out, Zout, 4, r_s, k, false, s, ws);
@@ -193,13 +196,21 @@ bool fwd3d_rows_fits(int nx, int ny, int nz, int r_p, int r_s, int variant)
     // the padded copy as rows_launch_cfg lays it out: whole tiles plus the
     // halo R = r_p + r_s (x: LP = 4 ceil(R / 4)), below 2^30 floats, and the
     // volume below 2^31 voxels (32-bit offsets in the kernels)
-    // (deep launch shapes, r_p 1 and short r_p 2, double the tile depth)
-    const int deep = (r_p == 1 && variant == kVRows) || (r_p == 2 && variant == kVRowsShort) ? 2 : 1;
-    const long long tx = fwd3_tx(variant, r_p), ty = fwd3_ty(variant, r_p), tz = deep * fwd3_tz(variant, r_p);
+    // of every launch shape the variant may run: the deep shapes (rows tiles
+    // of r_p 1 and 4, short r_p 2) double the tile depth, and r_p = 4 also
+    // runs the other of its rows / big tiles
+    if ((long long)nx * ny * nz >= (1LL << 31)) return false;
     const long long R = r_p + r_s, LP = 4 * ((R + 3) / 4);
-    const long long px = (nx + tx - 1) / tx * tx + 2 * LP, py = (ny + ty - 1) / ty * ty + 2 * R,
-                    pz = (nz + tz - 1) / tz * tz + 2 * R;
-    return px * py * pz < (1LL << 30) && (long long)nx * ny * nz < (1LL << 31);
+    auto fits = [&](long long tx, long long ty, long long tz) {
+        const long long px = (nx + tx - 1) / tx * tx + 2 * LP, py = (ny + ty - 1) / ty * ty + 2 * R,
+                        pz = (nz + tz - 1) / tz * tz + 2 * R;
+        return px * py * pz < (1LL << 30);
+    };
+    if (r_p == 4 && (variant == kVRows || variant == kVRowsBig))
+        return fits(fwd3_tx(kVRows, 4), fwd3_ty(kVRows, 4), 2 * fwd3_tz(kVRows, 4)) &&
+               fits(fwd3_tx(kVRowsBig, 4), fwd3_ty(kVRowsBig, 4), fwd3_tz(kVRowsBig, 4));
+    const int deep = (variant == kVRows && r_p == 1) || (r_p == 2 && variant == kVRowsShort) ? 2 : 1;
+    return fits(fwd3_tx(variant, r_p), fwd3_ty(variant, r_p), deep * fwd3_tz(variant, r_p));
 }
 
 int fwd3d_route(int nx, int ny, int nz, int r_p, int r_s, int path, int *kernel)

@@ -72,9 +72,10 @@ struct PadVol {
 // -- a larger halo share than the 28-plane tiles, but half the live
 // registers: the per-volume kernel runs four waves per SIMD (128 VGPRs; its
 // few spills sit outside the offset loops) instead of two.
-// ZM = 2 ("deep" tiles, r_p 1 and 2): the same tile twice as deep in z on 512
-// threads.  The halo share of phases 1 and 2 drops (r_p = 2: phase-1 rows
-// 448 for 288 tile rows instead of 256 for 144), and every voxel gets exactly
+// ZM = 2 ("deep" tiles, r_p 1, short r_p 2 and r_p 4): the same tile twice
+// as deep in z on 512 threads.  The halo share of phases 1 and 2 drops (r_p =
+// 2: phase-1 rows 448 for 288 tile rows instead of 256 for 144; r_p = 4: 384
+// for 128 instead of 256 for 64), and every voxel gets exactly
 // the arithmetic of the ZM = 1 tile -- x and y runs are unchanged, and the
 // phase-3 z runs still start at multiples of RZ in volume coordinates, since
 // TZ stays a multiple of RZ -- so the outputs are bit-identical.  Two such
@@ -85,7 +86,7 @@ struct RowsCfg {
     static constexpr int RP = RP_, RS = RS_, ZM = ZM_;
     static constexpr bool SHORT = SHORT_;
     static_assert(!SHORT || RP == 2, "short tiles are for r_p = 2");
-    static_assert(ZM == 1 || (ZM == 2 && (RP == 1 || (RP == 2 && SHORT))), "deep tiles: r_p 1, short r_p 2");
+    static_assert(ZM == 1 || (ZM == 2 && (RP != 2 || SHORT)), "deep tiles: r_p 1, 3, 4 or short r_p 2");
     static constexpr int NT = kRowsThreads * ZM;         // threads per workgroup
     static constexpr int W = 2 * RP + 1;                 // patch width
     static constexpr int R = RP + RS;                    // halo of the copy in y, z

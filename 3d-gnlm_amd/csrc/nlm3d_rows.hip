@@ -57,13 +57,17 @@ bool rows_rp1_four_per_cu(long long tiles)
 //   r_p = 2 short:  0 = 16 x 12 x 12, 4 per CU; 1 = deep 16 x 12 x 24, 2 per CU
 //   r_p = 3:        0 = 128 VGPRs (4 per CU, spills outside the offset loop);
 //                   1 = 168 VGPRs (3 per CU, no spills)
-// (r_p = 4's second shape, the 16 x 24 x 24 big tile, is launched from the
-// variant dispatch.)  shape < 0: the default rule below.
+//   r_p = 4:        0 = 16 x 8 x 8, 4 per CU; 1 = deep 16 x 8 x 16, 2 per CU
+// (deep 16 x 10 x 20 tiles for r_p = 3 give the same bits too but ran 2-8%
+// slower than both builds at 96^3-256^3, so they are not a candidate)
+// (r_p = 4's big 16 x 24 x 24 tile is launched from the variant dispatch.)
+// shape < 0: the default rule below.
 int rows_num_shapes(int r_p, bool short_tiles)
 {
     if (r_p == 1) return 3;
     if (r_p == 2 && short_tiles) return 2;
     if (r_p == 3) return 2;
+    if (r_p == 4) return 2;
     return 1;
 }
 
@@ -88,6 +92,10 @@ hipError_t rows_launch_t(const InDesc &in, float *out, float *Zout, float k, hip
                                                                ws);
     } else if constexpr (RP == 3) {
         if (shape == 1) return rows_launch_cfg<C>(in, out, Zout, k, s, nlm3d_fwd_rows<RP, RS, SHORT, 3>, ws);
+    } else if constexpr (RP == 4) {
+        if (shape == 1)
+            return rows_launch_cfg<RowsCfg<RP, RS, SHORT, 2>>(in, out, Zout, k, s, nlm3d_fwd_rows_deep<RP, RS, SHORT>,
+                                                               ws);
     }
     return rows_launch_cfg<C>(in, out, Zout, k, s, nlm3d_fwd_rows<RP, RS, SHORT, rows_waves<RP, SHORT>()>, ws);
 }
