@@ -154,6 +154,45 @@ int gnlm_u8_to_f32(const unsigned char *d_in, float *d_out, long long n, void *s
     return hip_status(u8_to_f32_launch(d_in, d_out, n, (hipStream_t)stream));
 }
 
+int gnlm3d_forward_u8_route(int nx, int ny, int nz, int r_p, int r_s, int *kernel)
+{
+    const int st = gnlm3d_forward_route(nx, ny, nz, r_p, r_s, GNLM_PATH_AUTO, kernel);
+    if (st == GNLM_OK && fwd3d_u8_fits(nx, ny, nz, r_p, r_s)) *kernel = GNLM_KERNEL_U8_INTEGER;
+    return st;
+}
+
+int gnlm3d_forward_u8(const unsigned char *d_in, int in_pad, long long in_sy, long long in_sz, float *d_out,
+                      float *d_Z, int nx, int ny, int nz, int r_p, int r_s, float h, void *stream)
+{
+    if (!d_in || !d_out) return GNLM_EINVAL;
+    if (bad_h(h)) return GNLM_EINVAL;
+    int kernel = 0;
+    int st = gnlm3d_forward_u8_route(nx, ny, nz, r_p, r_s, &kernel);
+    if (st != GNLM_OK) return st;
+    const int R = r_p + r_s;
+    st = check_padded(in_pad, in_sy, in_sz, nx, ny, R);
+    if (st != GNLM_OK) return st;
+    const int P = (2 * r_p + 1) * (2 * r_p + 1) * (2 * r_p + 1);
+    const float k = (float)weight_scale(P, h);
+    hipStream_t s = (hipStream_t)stream;
+    if (kernel == GNLM_KERNEL_U8_INTEGER)
+        return hip_status(fwd3d_u8_launch(d_in, in_pad, in_sy, in_sz, d_out, d_Z, nx, ny, nz, r_p, r_s, k, s));
+    // other radii and sizes: the float kernels on a float copy of the input
+    // (same layout; a padded input is converted up to its last sample)
+    const long long n = in_pad == 0 ? (long long)nx * ny * nz
+                                    : (long long)nx + 2LL * in_pad + in_sy * ((long long)ny + 2LL * in_pad - 1) +
+                                          in_sz * ((long long)nz + 2LL * in_pad - 1);
+    float *d_f = nullptr;
+    hipError_t e = hipMallocAsync((void **)&d_f, sizeof(float) * (size_t)n, s);
+    if (e != hipSuccess) return hip_status(e);
+    e = u8_to_f32_any_launch(d_in, d_f, n, s);
+    st = e == hipSuccess ? gnlm3d_forward_ex(d_f, in_pad, in_sy, in_sz, d_out, d_Z, nx, ny, nz, r_p, r_s, h,
+                                             GNLM_PATH_AUTO, stream)
+                         : hip_status(e);
+    const hipError_t e2 = hipFreeAsync(d_f, s);
+    return st != GNLM_OK ? st : hip_status(e2);
+}
+
 // ---------------------------------------------------------------------------
 // host-pointer entry points (drop-in backing)
 // ---------------------------------------------------------------------------
@@ -167,26 +206,19 @@ int gnlm_nl_means_3d_u8(const unsigned char *h_in, float *h_out, int nx, int ny,
     const int pad = (ss + ns - 2) / 2;  // = r_s + r_p, 3DNLM/nlm.cpp:33-36
     const long long W = nx + 2LL * pad, H = ny + 2LL * pad, D = nz + 2LL * pad;
     const long long nin = W * H * D, nout = (long long)nx * ny * nz;
-    // 16-byte alignment for the float image: round the byte buffer up
-    const long long nin_al = (nin + 15) / 16 * 16;
     unsigned char *d_u8 = nullptr;
-    float *d_f = nullptr, *d_o = nullptr;
+    float *d_o = nullptr;
     int st = GNLM_OK;
-    hipError_t e = hipMalloc((void **)&d_u8, (size_t)nin_al);
-    if (e == hipSuccess) e = hipMalloc((void **)&d_f, sizeof(float) * (size_t)nin);
+    hipError_t e = hipMalloc((void **)&d_u8, (size_t)nin);
     if (e == hipSuccess) e = hipMalloc((void **)&d_o, sizeof(float) * (size_t)nout);
     if (e == hipSuccess) e = hipMemcpy(d_u8, h_in, (size_t)nin, hipMemcpyHostToDevice);
     if (e != hipSuccess) st = hip_status(e);
-    if (st == GNLM_OK) st = gnlm_u8_to_f32(d_u8, d_f, nin, nullptr);
-    if (st == GNLM_OK)
-        st = gnlm3d_forward_ex(d_f, pad, W, W * H, d_o, nullptr, nx, ny, nz, r_p, r_s, h, GNLM_PATH_AUTO,
-                               nullptr);
+    if (st == GNLM_OK) st = gnlm3d_forward_u8(d_u8, pad, W, W * H, d_o, nullptr, nx, ny, nz, r_p, r_s, h, nullptr);
     if (st == GNLM_OK) {
         e = hipMemcpy(h_out, d_o, sizeof(float) * (size_t)nout, hipMemcpyDeviceToHost);
         if (e != hipSuccess) st = hip_status(e);
     }
     if (d_u8) (void)hipFree(d_u8);
-    if (d_f) (void)hipFree(d_f);
     if (d_o) (void)hipFree(d_o);
     return st;
 }

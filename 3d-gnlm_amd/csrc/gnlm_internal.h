@@ -115,7 +115,19 @@ hipError_t bwd3d_fold_launch(const float *gext, float *gu, int nx, int ny, int n
 int device_cus();
 hipError_t bwd3d_reduce_dh_launch(const float *part, int n, float scale, float *out, hipStream_t s);
 
+// ---- 8-bit forward with integer patch distances (nlm3d_u8.hip) -------------
+// r_p 7-8 with r_s 1-7; the input is the caller's 8-bit volume (in_pad = 0:
+// unpadded, reflect-101 on device; in_pad >= r_p + r_s: padded with strides
+// sy, sz in elements).  fwd3d_u8_fits: the radii are instantiated and the
+// volume's copy and records fit 32-bit byte offsets.
+bool fwd3d_u8_supported(int r_p, int r_s);
+bool fwd3d_u8_fits(int nx, int ny, int nz, int r_p, int r_s);
+hipError_t fwd3d_u8_launch(const unsigned char *d_in, int in_pad, long long sy, long long sz, float *out, float *Zout,
+                           int nx, int ny, int nz, int r_p, int r_s, float k, hipStream_t s);
+
 // ---- misc (gnlm_misc.hip) --------------------------------------------------
 hipError_t u8_to_f32_launch(const uint8_t *in, float *out, long long n, hipStream_t s);
+// any alignment (one element per thread)
+hipError_t u8_to_f32_any_launch(const uint8_t *in, float *out, long long n, hipStream_t s);
 
 }  // namespace gnlm

@@ -23,7 +23,22 @@ __global__ void k_u8_to_f32(const uint8_t *__restrict__ in, float *__restrict__ 
     if (t < n - 4 * n4) out[4 * n4 + t] = (float)in[4 * n4 + t];
 }
 
+__global__ void k_u8_to_f32_any(const uint8_t *__restrict__ in, float *__restrict__ out, long long n)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        out[i] = (float)in[i];
+}
+
 }  // namespace
+
+hipError_t u8_to_f32_any_launch(const uint8_t *in, float *out, long long n, hipStream_t s)
+{
+    long long g = (n + 255) / 256;
+    if (g > 65536) g = 65536;
+    if (g < 1) g = 1;
+    hipLaunchKernelGGL(k_u8_to_f32_any, dim3((unsigned)g), dim3(256), 0, s, in, out, n);
+    return hipGetLastError();
+}
 
 hipError_t u8_to_f32_launch(const uint8_t *in, float *out, long long n, hipStream_t s)
 {

@@ -1,0 +1,608 @@
+// nlm3d_u8.hip -- 3D NLM forward on 8-bit volumes with exact integer patch
+// distances: the reference's own input type (3DNLM/main.cu:7620 copies an
+// unsigned char volume; its kernels difference the samples as ints before
+// squaring, 3DNLM/main.cu:59) at the large patches of its default
+// configuration and sweeps (NS 15-17, 3DNLM/config.h:4-5, 3DNLM/run.sh:25-34).
+//
+// Same definition as every other forward (3DNLM/main.cu:52-68 patch SSD over
+// the full offset cube, :107-111 weights, :125 normalisation).  For 8-bit
+// samples a patch SSD is an integer below 2^31 (NS <= 17: 17^3 * 255^2 =
+// 3.2e8), so it is formed exactly in int32 -- in any order, with no rounding
+// at all -- as
+//     S_d(x) = A(x) + A(x + d) - 2 C_d(x),
+//     A(y)   = sum_p u(y + p)^2           (one box per volume: pre-passes),
+//     C_d(x) = sum_p u(x + p) u(x + d + p).
+// The float kernels need a difference and a square per sample and exact
+// pairwise trees for every box; here a sample costs one 16-bit multiply-add
+// (v_mad_i32_i16, both factors read from packed pairs through op_sel) and
+// each box is a prefix or sliding sum, which integers allow without error.
+// Only the weight is floating point: w = exp2(float(S) * k).
+//
+// Tiles: 16 x (32 - 2 r_p) x (32 - 2 r_p) voxels on 1024 threads, the grown
+// rows forming a 32 x 32 grid with one row per thread (the geometry of the
+// float large-patch tiles):
+//   * a pre-pass copies the volume with its reflect-101 border into an 8-bit
+//     buffer whose rows start on 16-byte boundaries (x origin 16 >= r_p + r_s);
+//     a row's 48 bytes around the tile come in three 128-bit loads;
+//   * the row's centre samples stay in registers as pairs of -2u (int16
+//     halves); per (dy, dz) the shifted row is loaded once and repacked into
+//     pairs of u (one v_perm_b32 per pair), and serves all 2 r_s + 1 dx with
+//     compile-time register indices;
+//   * phase 1: prefix sums of the products along the row give the x boxes of
+//     -2 u(x) u(x + d) -> LDS; phase 2: sliding y boxes of (x, z_ext)
+//     columns -> LDS (for r_p = 8 each wave boxes the two planes of rows it
+//     wrote itself, into one of two buffers by the offset's parity, so one
+//     barrier per offset suffices); phase 3: (x, y, run of RZ z) items,
+//     sliding z boxes read with 128/64-bit accesses (z-fastest layout), then
+//     S = A(x) + A(x + d) + box, w = exp2(S k), per-dz partial sums in
+//     registers; (A, u) of the candidate x + d arrive as one 64-bit record;
+//   * weights accumulate in two levels (per dz, then over dz) through the
+//     output buffers, as in the float kernels.
+#include <hip/hip_runtime.h>
+#include "gnlm_internal.h"
+#include "nlm3d_rows.h"
+
+namespace gnlm {
+namespace {
+
+// z-run length of the phase-3 items per r_p (the items must fit the 1024
+// threads: 16 * TY * TZ / RZ <= 1024)
+template <int RP>
+constexpr int u8_rz()
+{
+    return RP == 8 ? 4 : (RP == 7 ? 6 : (RP == 6 ? 10 : (RP == 5 ? 11 : 12)));
+}
+
+template <int RP_, int RS_>
+struct U8Cfg {
+    static constexpr int RP = RP_, RS = RS_;
+    static constexpr int W = 2 * RP + 1, R = RP + RS;
+    static constexpr int NT = 1024, TX = 16, EY = 32, EZ = 32;
+    static constexpr int TY = EY - 2 * RP, TZ = EZ - 2 * RP;
+    static constexpr int EX = TX + 2 * RP;   // centre samples per row
+    static constexpr int LPB = 16;           // x origin of the 8-bit copy
+    static_assert(RP >= 4 && RP <= 8 && RS >= 1 && R <= LPB, "32 - 2 r_p rows, window inside the 48 loaded bytes");
+    static constexpr int NW = TX + 2 * R;    // shifted-row samples per (dy, dz)
+    static constexpr int CB0 = LPB - RP;     // byte of centre sample 0 in the 48 loaded bytes
+    static constexpr int WB0 = LPB - R;      // byte of window sample 0
+    static constexpr int NCP = EX / 2, NWP = NW / 2;
+    // LDS (ints): s_x [z_ext][y_ext][x] (row pitch 20, plane pitch 656 = 16
+    // mod 64) as phase 1 writes it; s_xy [y][x][z_ext], z fastest, so that a
+    // phase-3 item reads its z run with 64/128-bit accesses (x pitch 36: the
+    // 16 x of one y fall on distinct bank quads; y pitch 580: the two halves
+    // of a wave-local phase 2 write 32 banks apart)
+    static constexpr int SXR = 20, SXP = SXR * EY + 16, SXN = SXP * EZ;
+    static constexpr int SYX = 36, SYY = 16 * SYX + 4, SYN = SYY * TY;
+    // WL ("wave-local"): where s_x and two s_xy buffers fit the LDS, each wave
+    // computes the y boxes of the two z_ext planes of rows it wrote itself in
+    // phase 1 -- no barrier between phases 1 and 2 -- into the s_xy buffer of
+    // the offset's parity, so that one barrier per offset (before phase 3)
+    // orders everything; otherwise waves 0-7 run phase 2 on whole columns
+    // between two barriers.
+    static constexpr bool WL = (SXN + 2 * SYN) * 4 <= 160 * 1024;
+    // phase 2 items: (x, z_ext) columns, whole (NP2 = 1) or, wave-local, in
+    // two runs of RY outputs (64 items per wave: 16 x, 2 planes, 2 runs)
+    static constexpr int NP2 = WL ? 2 : 1;
+    static_assert(TY % NP2 == 0, "whole y runs");
+    static constexpr int RY = TY / NP2;
+    static constexpr int P2N = TX * EZ * NP2;
+    static constexpr int RZ = u8_rz<RP>();
+    static_assert(TZ % RZ == 0, "whole z runs");
+    static constexpr int P3N = TX * TY * (TZ / RZ);
+    static_assert(P3N <= NT && P2N <= NT, "one item per thread");
+    static constexpr int LDS_INTS = SXN + (WL ? 2 : 1) * SYN;
+    // phase-3 read width (ints): the z runs start at multiples of RZ
+    static constexpr int V3R = RZ % 4 == 0 ? 4 : (RZ % 2 == 0 ? 2 : 1);
+    static constexpr int NV3 = (RZ + W - 1 + V3R - 1) / V3R * V3R;  // values read per item
+    static_assert(NV3 <= SYX, "a z run stays inside its (y, x) column");
+};
+
+// 8-bit padded copy: sample (x, y, z) at p[(z + R) * pxy + (y + R) * px + x + 16]
+// for x in [-16, nxr + 16), y, z in [-R, n?r + R); 0 outside [-R, n + R).
+struct U8Vol {
+    const unsigned char *p;
+    int px, pxy;
+    unsigned bytes;
+};
+// (A, u) records: 8 bytes each, record (x, y, z) at index
+// (z + RS) * rxy + (y + RS) * rx + (x + RS) for x in [-RS, nxr + RS) etc.:
+// int A = sum_p u(. + p)^2, then the float value of u.
+struct RecVol {
+    const int *p;
+    int rx, rxy;
+    unsigned bytes;
+};
+
+struct U8Args {
+    U8Vol v;
+    RecVol rc;
+    float *out;   // y, and the running Y total between dz steps
+    float *Zacc;  // running Z total between dz steps; Z at the end
+    int nx, ny, nz;
+    float k;
+    int ntx, nty, ntz;
+};
+
+__device__ __forceinline__ unsigned byte_of(const unsigned (&b)[12], int i) { return (b[i >> 2] >> (8 * (i & 3))) & 0xffu; }
+
+// the 48 bytes x0 - 16 .. x0 + 32 of one row
+__device__ __forceinline__ void load_row48(const PadRsrc &r, unsigned voff, int soff, unsigned (&b)[12])
+{
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const u32x4 t = __builtin_amdgcn_raw_buffer_load_b128(r.r, (int)(voff + 16u * j), soff, 0);
+        b[4 * j + 0] = t[0];
+        b[4 * j + 1] = t[1];
+        b[4 * j + 2] = t[2];
+        b[4 * j + 3] = t[3];
+    }
+}
+
+// pairs (u[B0 + 2q], u[B0 + 2q + 1]) as zero-extended int16 halves
+template <int B0, int NP>
+__device__ __forceinline__ void pack_pairs(const unsigned (&b)[12], int (&w)[NP])
+{
+#pragma unroll
+    for (int q = 0; q < NP; ++q) {
+        const int i = B0 + 2 * q, d = i >> 2, o = i & 3;
+        // v_perm_b32 selects bytes of {src0, src1} (0-3: src1, 4-7: src0); 12 gives 0
+        if (o <= 2)
+            w[q] = (int)__builtin_amdgcn_perm(0u, b[d], (unsigned)(o | (12 << 8) | ((o + 1) << 16) | (12 << 24)));
+        else
+            w[q] = (int)__builtin_amdgcn_perm(b[d + 1], b[d], 0x0c040c03u);
+    }
+}
+
+// v_mad_i32_i16: D = sext(A.half) * sext(B.half) + C, on the 16-bit halves
+// op_sel picks -- here two or four of them chained in one asm statement, each
+// result the next one's addend (full-dword writes: no wait states between
+// them).  The compiler does not form this instruction from C (its own choice
+// for 16-bit products is a multiply and an add), and it pads every separate
+// asm statement that reads the previous one's output with an s_nop, since it
+// cannot see inside asm; grouping cuts those to one per group.
+template <int A0, int B0, int A1, int B1>
+__device__ __forceinline__ void mad16x2(int c0, int w0, int c1, int w1, int acc, int &p0, int &p1)
+{
+    asm("v_mad_i32_i16 %0, %2, %3, %4 op_sel:[%7,%8,0,0]\n\t"
+        "v_mad_i32_i16 %1, %5, %6, %0 op_sel:[%9,%10,0,0]"
+        : "=&v"(p0), "=v"(p1)
+        : "v"(c0), "v"(w0), "v"(acc), "v"(c1), "v"(w1), "n"(A0), "n"(B0), "n"(A1), "n"(B1));
+}
+template <int A0, int B0, int A1, int B1, int A2, int B2, int A3, int B3>
+__device__ __forceinline__ void mad16x4(int c0, int w0, int c1, int w1, int c2, int w2, int c3, int w3, int acc,
+                                        int &p0, int &p1, int &p2, int &p3)
+{
+    asm("v_mad_i32_i16 %0, %4, %5, %12 op_sel:[%13,%14,0,0]\n\t"
+        "v_mad_i32_i16 %1, %6, %7, %0 op_sel:[%15,%16,0,0]\n\t"
+        "v_mad_i32_i16 %2, %8, %9, %1 op_sel:[%17,%18,0,0]\n\t"
+        "v_mad_i32_i16 %3, %10, %11, %2 op_sel:[%19,%20,0,0]"
+        : "=&v"(p0), "=&v"(p1), "=&v"(p2), "=v"(p3)
+        : "v"(c0), "v"(w0), "v"(c1), "v"(w1), "v"(c2), "v"(w2), "v"(c3), "v"(w3), "v"(acc), "n"(A0), "n"(B0),
+          "n"(A1), "n"(B1), "n"(A2), "n"(B2), "n"(A3), "n"(B3));
+}
+
+// phase 1 for the offset whose window index of centre sample 0 is SH (= r_s + dx)
+template <class C, int SH>
+__device__ __forceinline__ void u8_phase1(const int (&cen)[C::NCP], const int (&win)[C::NWP], int *dst)
+{
+    int P[C::EX + 1];  // prefix sums of the products: P[i] = sum_{k < i} (-2 u_k) u_{k + dx}
+    P[0] = 0;
+    constexpr int NB4 = C::EX / 4;
+    static_for<0, NB4 - 1>([&](auto bc) {
+        constexpr int I = 4 * decltype(bc)::value, J = I + SH;
+        mad16x4<0, J & 1, 1, (J + 1) & 1, 0, J & 1, 1, (J + 1) & 1>(
+            cen[I >> 1], win[J >> 1], cen[I >> 1], win[(J + 1) >> 1], cen[(I + 2) >> 1], win[(J + 2) >> 1],
+            cen[(I + 2) >> 1], win[(J + 3) >> 1], P[I], P[I + 1], P[I + 2], P[I + 3], P[I + 4]);
+    });
+    if constexpr (C::EX % 4 == 2) {
+        constexpr int I = 4 * NB4, J = I + SH;
+        mad16x2<0, J & 1, 1, (J + 1) & 1>(cen[I >> 1], win[J >> 1], cen[I >> 1], win[(J + 1) >> 1], P[I], P[I + 1],
+                                         P[I + 2]);
+    }
+    int o[16];  // x boxes: P[x + W] - P[x]
+#pragma unroll
+    for (int x = 0; x < 16; ++x) o[x] = x == 0 ? P[C::W] : P[x + C::W] - P[x];
+    int4 *d4 = reinterpret_cast<int4 *>(dst);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) d4[j] = make_int4(o[4 * j], o[4 * j + 1], o[4 * j + 2], o[4 * j + 3]);
+}
+
+// phase 2: y box of a run of RY outputs of one (x, z_ext) column, sliding
+// (exact in integers)
+template <class C>
+__device__ __forceinline__ void u8_phase2(const int *src, int *dst)
+{
+    int v[C::RY + C::W - 1];
+#pragma unroll
+    for (int j = 0; j < C::RY + C::W - 1; ++j) v[j] = src[j * C::SXR];
+    int s = v[0];
+#pragma unroll
+    for (int j = 1; j < C::W; ++j) s += v[j];
+    dst[0] = s;
+#pragma unroll
+    for (int y = 1; y < C::RY; ++y) {
+        s += v[y + C::W - 1] - v[y - 1];
+        dst[y * C::SYY] = s;
+    }
+}
+
+template <class C>
+__device__ __forceinline__ void u8_tile(const U8Args &a, int x0, int y0, int z0, int *lds)
+{
+    constexpr int RP = C::RP, RS = C::RS, RZ = C::RZ, W = C::W;
+    int *SX = lds;
+    int *SXY = lds + C::SXN;
+    const int tid = threadIdx.x;
+    const PadRsrc ru = buf_rsrc(a.v.p, a.v.bytes);
+    const PadRsrc rr = buf_rsrc(a.rc.p, a.rc.bytes);
+    gfloat *out = (gfloat *)a.out;
+    gfloat *Zacc = (gfloat *)a.Zacc;
+    const int px = __builtin_amdgcn_readfirstlane(a.v.px), pxy = __builtin_amdgcn_readfirstlane(a.v.pxy);
+    const int rx = __builtin_amdgcn_readfirstlane(a.rc.rx), rxy = __builtin_amdgcn_readfirstlane(a.rc.rxy);
+    const float kw = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(a.k)));
+    const int tzv = min(C::TZ, a.nz - z0);
+
+    // ---- phase-1 row (y_ext = tid % 32, z_ext = tid / 32) ----
+    const int yp = tid % C::EY, zp = tid / C::EY;
+    // z_ext grows with the lane index: the first lane decides for the wave
+    const bool p1ok = __builtin_amdgcn_readfirstlane(zp < tzv + 2 * RP);
+    // bytes x0 - 16 .. x0 + 32 of the row at (dy, dz) = (-RS, -RS)
+    const unsigned rbase = (unsigned)((z0 - RP + zp + C::R - RS) * pxy + (y0 - RP + yp + C::R - RS) * px + x0);
+    const int sxw = zp * C::SXP + yp * C::SXR;
+    int cen[C::NCP];
+    {
+        unsigned b[12];
+        load_row48(ru, rbase, RS * pxy + RS * px, b);
+#pragma unroll
+        for (int q = 0; q < C::NCP; ++q) {
+            const int lo = -2 * (int)byte_of(b, C::CB0 + 2 * q), hi = -2 * (int)byte_of(b, C::CB0 + 2 * q + 1);
+            cen[q] = p1ok ? (int)(((unsigned)lo & 0xffffu) | ((unsigned)hi << 16)) : 0;
+        }
+    }
+
+    // ---- phase-2 item: (x, z_ext, y run); lanes run x fastest; wave-local:
+    // wave w's lanes cover the planes 2w, 2w + 1 its phase-1 rows filled ----
+    const int p2x = tid % 16;
+    const int p2z = C::WL ? 2 * (tid / 64) + (tid / 16) % 2 : (tid / 16) % C::EZ;
+    const int p2h = C::WL ? (tid % 64) / 32 : tid / (16 * C::EZ);
+    const bool p2ok = tid < C::P2N && __builtin_amdgcn_readfirstlane(p2z < tzv + 2 * RP);
+    const int p2src = p2z * C::SXP + p2h * C::RY * C::SXR + p2x, p2dst = p2h * C::RY * C::SYY + p2x * C::SYX + p2z;
+
+    // ---- phase-3 item: (x, y, z run) ----
+    const int itc = tid < C::P3N ? tid : C::P3N - 1;
+    const int p3x = itc % 16, p3y = (itc / 16) % C::TY, p3zr = itc / (16 * C::TY);
+    const bool p3ok = __builtin_amdgcn_readfirstlane(tid < C::P3N && p3zr * RZ < tzv);
+    const int p3r = p3y * C::SYY + p3x * C::SYX + p3zr * RZ;
+    // record of x at d = (-RS, -RS, -RS), in bytes
+    const unsigned cb = 8u * (unsigned)((z0 + p3zr * RZ) * rxy + (y0 + p3y) * rx + x0 + p3x);
+    int sbuf = 0;  // wave-local: the s_xy buffer of the current offset (0 or SYN)
+    int Ax[RZ];
+#pragma unroll
+    for (int o = 0; o < RZ; ++o)
+        Ax[o] = (int)__builtin_amdgcn_raw_buffer_load_b32(rr.r, (int)cb, 8 * ((RS + o) * rxy + RS * rx + RS), 0);
+
+    for (int dz = -RS; dz <= RS; ++dz) {
+        float Zd[RZ], Yd[RZ];
+#pragma unroll
+        for (int o = 0; o < RZ; ++o) {
+            Zd[o] = 0.0f;
+            Yd[o] = 0.0f;
+        }
+        for (int dy = -RS; dy <= RS; ++dy) {
+            const int soffp = (dz + RS) * pxy + (dy + RS) * px;
+            const int soffr = (dz + RS) * rxy + (dy + RS) * rx;
+            int win[C::NWP];
+            {
+                unsigned b[12];
+                load_row48(ru, rbase, soffp, b);
+                pack_pairs<C::WB0, C::NWP>(b, win);
+            }
+            static_for<-RS, RS>([&](auto dxc) {
+                constexpr int DX = decltype(dxc)::value;
+                if (p1ok) u8_phase1<C, RS + DX>(cen, win, SX + sxw);
+                // candidate records (A, u) at x + d, in flight across the barriers
+                unsigned cA[RZ], cU[RZ];
+#pragma unroll
+                for (int o = 0; o < RZ; ++o) {
+                    const u32x2 t = __builtin_amdgcn_raw_buffer_load_b64(rr.r, (int)cb,
+                                                                         8 * (soffr + DX + RS + o * rxy), 0);
+                    cA[o] = t[0];
+                    cU[o] = t[1];
+                }
+                if constexpr (C::WL) {
+                    // the wave's own phase-1 stores are complete before its reads
+                    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+                    if (p2ok) u8_phase2<C>(SX + p2src, SXY + sbuf + p2dst);
+                    lds_barrier();
+                } else {
+                    lds_barrier();
+                    if (p2ok) u8_phase2<C>(SX + p2src, SXY + p2dst);
+                    lds_barrier();
+                }
+                if (p3ok) {
+                    int v[C::NV3];
+                    const int *s3 = SXY + sbuf + p3r;
+#pragma unroll
+                    for (int j = 0; j < C::NV3; j += C::V3R) {
+                        if constexpr (C::V3R == 4) {
+                            const int4 t = *reinterpret_cast<const int4 *>(s3 + j);
+                            v[j] = t.x;
+                            v[j + 1] = t.y;
+                            v[j + 2] = t.z;
+                            v[j + 3] = t.w;
+                        } else if constexpr (C::V3R == 2) {
+                            const int2 t = *reinterpret_cast<const int2 *>(s3 + j);
+                            v[j] = t.x;
+                            v[j + 1] = t.y;
+                        } else {
+                            v[j] = s3[j];
+                        }
+                    }
+                    int s = v[0];
+#pragma unroll
+                    for (int j = 1; j < W; ++j) s += v[j];
+#pragma unroll
+                    for (int o = 0; o < RZ; ++o) {
+                        if (o > 0) s += v[o + W - 1] - v[o - 1];
+                        const int S = Ax[o] + (int)cA[o] + s;  // exact: 0 <= S < 2^31
+                        const float w = exp2_fast((float)S * kw);
+                        Zd[o] = Zd[o] + w;
+                        Yd[o] = fmaf(w, __uint_as_float(cU[o]), Yd[o]);
+                    }
+                }
+                if constexpr (C::WL) sbuf = C::SYN - sbuf;
+            });
+        }
+        // close the dz block: totals += partials (kept in the output buffers)
+        const int gx = x0 + p3x, gy = y0 + p3y, gz = z0 + p3zr * RZ;
+        if (p3ok && tid < C::P3N && gx < a.nx && gy < a.ny) {
+#pragma unroll
+            for (int o = 0; o < RZ; ++o) {
+                if (gz + o >= a.nz) continue;
+                const int oi = gx + a.nx * (gy + a.ny * (gz + o));
+                float Z = Zd[o], Y = Yd[o];
+                if (dz != -RS) {
+                    Z = Zacc[oi] + Z;
+                    Y = out[oi] + Y;
+                }
+                Zacc[oi] = Z;
+                out[oi] = (dz == RS) ? Y / Z : Y;
+            }
+        }
+    }
+}
+
+template <int RP, int RS>
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4))) void nlm3d_fwd_u8(U8Args a)
+{
+    using C = U8Cfg<RP, RS>;
+    extern __shared__ __align__(16) int lds_i[];
+    const int nfull = a.ntx * a.nty * (a.ntz - (a.nz % C::TZ != 0 ? 1 : 0));
+    const int t = xcd_tile(blockIdx.x, nfull);
+    const int x0 = (t % a.ntx) * C::TX;
+    const int y0 = ((t / a.ntx) % a.nty) * C::TY;
+    const int z0 = (t / (a.ntx * a.nty)) * C::TZ;
+    u8_tile<C>(a, x0, y0, z0, lds_i);
+}
+
+// ---- pre-passes --------------------------------------------------------------
+
+struct InU8 {
+    const unsigned char *data;
+    int nx, ny, nz, pad;
+    long long sy, sz;
+};
+
+__device__ __forceinline__ unsigned char load_u8(const InU8 &d, int x, int y, int z)
+{
+    if (d.pad == 0) {
+        x = reflect101(x, d.nx);
+        y = reflect101(y, d.ny);
+        z = reflect101(z, d.nz);
+        return d.data[(long long)x + (long long)d.nx * ((long long)y + (long long)d.ny * z)];
+    }
+    return d.data[(long long)(x + d.pad) + d.sy * (long long)(y + d.pad) + d.sz * (long long)(z + d.pad)];
+}
+
+// the 8-bit copy: reflect-101 samples on [-r, n + r), 0 beyond
+__global__ void u8_pad(InU8 in, unsigned char *dst, int px, int py, int pz, int r)
+{
+    const int nxy = px * py;
+    for (int zp = blockIdx.y; zp < pz; zp += gridDim.y) {
+        const int z = zp - r;
+        unsigned char *plane = dst + (long long)zp * nxy;
+        const bool zin = z < in.nz + r;
+        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nxy; i += gridDim.x * blockDim.x) {
+            const int x = i % px - 16;
+            const int y = i / px - r;
+            unsigned char v = 0;
+            if (zin && x >= -r && x < in.nx + r && y < in.ny + r) v = load_u8(in, x, y, z);
+            plane[i] = v;
+        }
+    }
+}
+
+// A, separably (integers, exact): X1 = x box of u^2 on x in [-rs, nxr + rs),
+// y, z in [-R, n?r + R); X2 = y box of X1 on y in [-rs, nyr + rs); records =
+// (z box of X2, float(u)) on z in [-rs, nzr + rs).
+__global__ void u8_sq_x(const unsigned char *cp, int px, int pxy, int *X1, int rx, int ry1, int rz1, int rp, int rs,
+                        int R)
+{
+    const int n = rx * ry1;
+    for (int zp = blockIdx.y; zp < rz1; zp += gridDim.y) {
+        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+            const int x = i % rx - rs, yq = i / rx;  // y + R = yq
+            const unsigned char *row = cp + (long long)zp * pxy + (long long)yq * px + 16 + x;
+            int s = 0;
+            for (int j = -rp; j <= rp; ++j) {
+                const int v = row[j];
+                s += v * v;
+            }
+            X1[(long long)zp * n + i] = s;
+        }
+    }
+}
+
+__global__ void u8_sq_y(const int *X1, int *X2, int rx, int ry1, int ry2, int rz1, int rp, int rs, int R)
+{
+    const int n2 = rx * ry2;
+    for (int zp = blockIdx.y; zp < rz1; zp += gridDim.y) {
+        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += gridDim.x * blockDim.x) {
+            const int x = i % rx, y = i / rx - rs;
+            const int *col = X1 + (long long)zp * rx * ry1 + (long long)(y + R) * rx + x;
+            int s = 0;
+            for (int j = -rp; j <= rp; ++j) s += col[(long long)j * rx];
+            X2[(long long)zp * n2 + i] = s;
+        }
+    }
+}
+
+__global__ void u8_sq_z(const int *X2, const unsigned char *cp, int px, int pxy, int2 *rec, int rx, int ry2, int rz,
+                        int rp, int rs, int R)
+{
+    const int n2 = rx * ry2;
+    for (int zq = blockIdx.y; zq < rz; zq += gridDim.y) {
+        const int z = zq - rs;
+        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += gridDim.x * blockDim.x) {
+            const int x = i % rx - rs, y = i / rx - rs;
+            const int *col = X2 + (long long)(z + R) * n2 + i;
+            int s = 0;
+            for (int j = -rp; j <= rp; ++j) s += col[(long long)j * n2];
+            const unsigned char u = cp[(long long)(z + R) * pxy + (long long)(y + R) * px + 16 + x];
+            rec[(long long)zq * n2 + i] = make_int2(s, __float_as_int((float)u));
+        }
+    }
+}
+
+template <int RP, int RS>
+constexpr size_t u8_lds()
+{
+    return U8Cfg<RP, RS>::LDS_INTS * sizeof(int);
+}
+
+// Geometry of one launch: tile counts and buffer extents.
+struct U8Geom {
+    int ntx, nty, ntz, px, py, pz, rx, ry, rz, ry1, rz1;
+    long long copy_bytes, x1_ints, x2_ints, rec_bytes;
+};
+
+U8Geom u8_geom(int nx, int ny, int nz, int rp, int rs)
+{
+    U8Geom g;
+    const int TY = 32 - 2 * rp, TZ = TY, R = rp + rs;
+    g.ntx = (nx + 15) / 16;
+    g.nty = (ny + TY - 1) / TY;
+    g.ntz = (nz + TZ - 1) / TZ;
+    const int nxr = g.ntx * 16, nyr = g.nty * TY, nzr = g.ntz * TZ;
+    g.px = nxr + 32;
+    g.py = nyr + 2 * R;
+    g.pz = nzr + 2 * R;
+    g.rx = nxr + 2 * rs;
+    g.ry = nyr + 2 * rs;
+    g.rz = nzr + 2 * rs;
+    g.ry1 = g.py;
+    g.rz1 = g.pz;
+    g.copy_bytes = (long long)g.px * g.py * g.pz;
+    g.x1_ints = (long long)g.rx * g.ry1 * g.rz1;
+    g.x2_ints = (long long)g.rx * g.ry * g.rz1;
+    g.rec_bytes = 8LL * g.rx * g.ry * g.rz;
+    return g;
+}
+
+template <int RP, int RS>
+hipError_t u8_launch_t(const InU8 &in, float *out, float *Zout, float k, hipStream_t s)
+{
+    using C = U8Cfg<RP, RS>;
+    const U8Geom g = u8_geom(in.nx, in.ny, in.nz, RP, RS);
+    const long long nv = (long long)in.nx * in.ny * in.nz;
+    auto al = [](long long b) { return (b + 255) / 256 * 256; };
+    const long long off_x1 = al(g.copy_bytes), off_x2 = off_x1 + al(4 * g.x1_ints),
+                    off_rec = off_x2 + al(4 * g.x2_ints), off_z = off_rec + al(g.rec_bytes);
+    const long long bytes = off_z + (Zout ? 0 : 4 * nv);
+    char *ws = nullptr;
+    hipError_t e = hipMallocAsync((void **)&ws, (size_t)bytes, s);
+    if (e != hipSuccess) return e;
+    unsigned char *cp = (unsigned char *)ws;
+    int *X1 = (int *)(ws + off_x1), *X2 = (int *)(ws + off_x2);
+    int2 *rec = (int2 *)(ws + off_rec);
+    U8Args a;
+    a.v = U8Vol{cp, g.px, g.px * g.py, (unsigned)g.copy_bytes};
+    a.rc = RecVol{(const int *)rec, g.rx, g.rx * g.ry, (unsigned)g.rec_bytes};
+    a.out = out;
+    a.Zacc = Zout ? Zout : (float *)(ws + off_z);
+    a.nx = in.nx;
+    a.ny = in.ny;
+    a.nz = in.nz;
+    a.k = k;
+    a.ntx = g.ntx;
+    a.nty = g.nty;
+    a.ntz = g.ntz;
+    auto grid = [](long long n, int planes) {
+        const long long gx = (n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024;
+        return dim3((unsigned)gx, (unsigned)(planes < 65535 ? planes : 65535));
+    };
+    hipLaunchKernelGGL(u8_pad, grid((long long)g.px * g.py, g.pz), dim3(256), 0, s, in, cp, g.px, g.py, g.pz, C::R);
+    hipLaunchKernelGGL(u8_sq_x, grid((long long)g.rx * g.ry1, g.rz1), dim3(256), 0, s, cp, g.px, g.px * g.py, X1, g.rx,
+                       g.ry1, g.rz1, RP, RS, C::R);
+    hipLaunchKernelGGL(u8_sq_y, grid((long long)g.rx * g.ry, g.rz1), dim3(256), 0, s, X1, X2, g.rx, g.ry1, g.ry, g.rz1,
+                       RP, RS, C::R);
+    hipLaunchKernelGGL(u8_sq_z, grid((long long)g.rx * g.ry, g.rz), dim3(256), 0, s, X2, cp, g.px, g.px * g.py, rec,
+                       g.rx, g.ry, g.rz, RP, RS, C::R);
+    e = hipGetLastError();
+    if (e == hipSuccess) {
+        const size_t lds = u8_lds<RP, RS>();
+        void (*kern)(U8Args) = nlm3d_fwd_u8<RP, RS>;
+        e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(kern, dim3(g.ntx * g.nty * g.ntz), dim3(C::NT), lds, s, a);
+            e = hipGetLastError();
+        }
+    }
+    const hipError_t e2 = hipFreeAsync(ws, s);
+    return e != hipSuccess ? e : e2;
+}
+
+// Instantiated radii: r_p 7 and 8 (NS 15, 17) with r_s 1-7.  For r_p 4-6 the
+// tiles grow to 16 x 20^2 .. 16 x 24^2 voxels on the same 1024 rows and the
+// phase-3 state no longer fits the registers; there the FP32 kernels (on a
+// float copy) ran in 0.67-1.03x of its time on MI355X (512^2 x 64, r_p 4-6,
+// r_s 3-6), and gnlm3d_forward_u8 uses them.
+#define GNLM_U8_CASES(X)                                                                \
+    X(7, 1) X(7, 2) X(7, 3) X(7, 4) X(7, 5) X(7, 6) X(7, 7) X(8, 1) X(8, 2) X(8, 3)     \
+    X(8, 4) X(8, 5) X(8, 6) X(8, 7)
+
+}  // namespace
+
+bool fwd3d_u8_supported(int r_p, int r_s)
+{
+#define GNLM_U8_SUP(RP, RS) \
+    if (r_p == RP && r_s == RS) return u8_lds<RP, RS>() <= kLdsLimitBytes;
+    GNLM_U8_CASES(GNLM_U8_SUP)
+#undef GNLM_U8_SUP
+    return false;
+}
+
+bool fwd3d_u8_fits(int nx, int ny, int nz, int r_p, int r_s)
+{
+    if (!fwd3d_u8_supported(r_p, r_s)) return false;
+    const U8Geom g = u8_geom(nx, ny, nz, r_p, r_s);
+    const long long nv = (long long)nx * ny * nz;
+    // 32-bit byte offsets into the copy and the records; 32-bit voxel indices
+    return g.copy_bytes < (1LL << 32) && g.rec_bytes < (1LL << 32) && nv < (1LL << 31) &&
+           (long long)g.ntx * g.nty * g.ntz < (1LL << 31);
+}
+
+hipError_t fwd3d_u8_launch(const unsigned char *d_in, int in_pad, long long sy, long long sz, float *out, float *Zout,
+                           int nx, int ny, int nz, int r_p, int r_s, float k, hipStream_t s)
+{
+    if (!fwd3d_u8_fits(nx, ny, nz, r_p, r_s)) return hipErrorInvalidValue;
+    if (in_pad > 0 && in_pad < r_p + r_s) return hipErrorInvalidValue;
+    const InU8 in{d_in, nx, ny, nz, in_pad, sy, sz};
+#define GNLM_U8_RUN(RP, RS) \
+    if (r_p == RP && r_s == RS) return u8_launch_t<RP, RS>(in, out, Zout, k, s);
+    GNLM_U8_CASES(GNLM_U8_RUN)
+#undef GNLM_U8_RUN
+    return hipErrorInvalidValue;
+}
+
+}  // namespace gnlm

@@ -299,35 +299,39 @@ def bench_sweep(args):
     """The reference's parameter sweep (3DNLM/run.sh:25-34): a 512 x 512 8-bit
     image stacked `depth` times (3DNLM/nlm.cpp:53-59), SS in {11, 13}, NS in
     {13, 15, 17}, depth in {16, 32, 64, 128}, h = PARAMETER = 10 on 0-255.
-    The image is synthetic (a 512^2 phantom quantised to 8 bits).  Each point
-    is timed with HIP events over `steps` forwards after `warmup`."""
+    The image is synthetic (a 512^2 phantom quantised to 8 bits) and stays
+    8-bit on the device, as the reference's does (gnlm3d_forward_u8).  Each
+    point is timed with HIP events over `steps` forwards after `warmup`."""
     import torch
     import gnlm
     dev = torch.device("cuda:0")
     img, _ = gnlm.phantom2d(512, 512, sigma=0.05, seed=20261209)
-    img = np.clip(np.rint(img * 255.0), 0, 255).astype(np.float32)
+    img = np.clip(np.rint(img * 255.0), 0, 255).astype(np.uint8)
     steps, warmup = max(1, min(args.steps, 3)), max(0, min(args.warmup, 1))
     points = []
     t_all = time.perf_counter()
     for depth in (16, 32, 64, 128):
         vol = torch.from_numpy(np.ascontiguousarray(np.broadcast_to(img, (depth, 512, 512)))).to(dev)
-        out = torch.empty_like(vol)
+        out = torch.empty(vol.shape, dtype=torch.float32, device=dev)
         for ss in (11, 13):
             for ns in (13, 15, 17):
                 rs, rp = ss // 2, ns // 2
                 for _ in range(warmup):
-                    gnlm.nlm3d(vol, rp, rs, 10.0, out=out)
+                    gnlm.nlm3d_u8(vol, rp, rs, 10.0, out=out)
                 e0 = torch.cuda.Event(enable_timing=True)
                 e1 = torch.cuda.Event(enable_timing=True)
                 torch.cuda.synchronize()
                 e0.record()
                 for _ in range(steps):
-                    gnlm.nlm3d(vol, rp, rs, 10.0, out=out)
+                    gnlm.nlm3d_u8(vol, rp, rs, 10.0, out=out)
                 e1.record()
                 torch.cuda.synchronize()
                 ms = e0.elapsed_time(e1) / steps
                 nvox = 512 * 512 * depth
-                points.append({"SS": ss, "NS": ns, "depth": depth, "variant": gnlm.tuned_variant(rp, rs),
+                integer = gnlm.forward_u8_route(512, 512, depth, rp, rs)[1] == gnlm.KERNEL_U8_INTEGER
+                points.append({"SS": ss, "NS": ns, "depth": depth,
+                               "kernel": "8-bit integer distances" if integer else
+                               f"FP32 variant {gnlm.tuned_variant(rp, rs)} on a float copy",
                                "ms": round(ms, 3), "Mvoxels_s": round(nvox / ms / 1e3, 1),
                                "Gpairs_s": round(nvox * ss ** 3 / ms / 1e6, 1)})
                 print(f"sweep SS={ss} NS={ns} depth={depth}: {ms:.2f} ms", file=sys.stderr, flush=True)
@@ -366,8 +370,10 @@ def pmc_pass(args):
             u, _ = gnlm.phantom3d(nx, ny, nz, n_ellipsoids=wl["n_ell"], axis_lo=wl["ax"][0], axis_hi=wl["ax"][1],
                                   background=0.05, sigma=wl["sigma"], seed=wl["seed"])
             if wl.get("u8"):
-                u = np.clip(np.rint(u * 255.0), 0, 255).astype(np.float32)
-            gnlm.nlm3d(torch.from_numpy(u).to(dev), wl["rp"], wl["rs"], wl["h"])
+                u8 = np.clip(np.rint(u * 255.0), 0, 255).astype(np.uint8)
+                gnlm.nlm3d_u8(torch.from_numpy(u8).to(dev), wl["rp"], wl["rs"], wl["h"])
+            else:
+                gnlm.nlm3d(torch.from_numpy(u).to(dev), wl["rp"], wl["rs"], wl["h"])
         else:
             u, _ = gnlm.phantom2d(*wl["dims2"], sigma=wl["sigma"], seed=wl["seed"])
             if wl.get("u8"):
@@ -441,15 +447,19 @@ def main():
         unit_vox = "Mpixels/s"
     nvox = nx * ny * nz
     if wl.get("u8"):
-        # 8-bit input resident in HBM; each step converts it and filters it
+        # 8-bit input resident in HBM.  3D: the 8-bit entry point
+        # (gnlm3d_forward_u8: integer patch distances); 2D: each step converts
+        # the image and filters it
         u = np.clip(np.rint(u * 255.0), 0, 255).astype(np.float32)
         u8 = torch.from_numpy(u.astype(np.uint8)).to(dev)
         f32 = torch.empty(u.shape, dtype=torch.float32, device=dev)
 
         def run(t, o):
+            if nz > 1:
+                return gnlm.nlm3d_u8(u8, rp, rs, h, out=o)
             gnlm.check(gnlm.lib().gnlm_u8_to_f32(u8.data_ptr(), f32.data_ptr(), f32.numel(),
                                                  torch.cuda.current_stream(dev).cuda_stream), "gnlm_u8_to_f32")
-            return gnlm.nlm3d(f32, rp, rs, h, out=o) if nz > 1 else gnlm.nlm2d(f32, rp, rs, h)
+            return gnlm.nlm2d(f32, rp, rs, h)
     ud = torch.from_numpy(u).to(dev)
     out = torch.empty_like(ud)
     torch.cuda.synchronize()
@@ -476,7 +486,9 @@ def main():
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": _traffic_from_profiles(args.workload),
                 "kernel_ms": round(ms_kernel, 4), "flop_per_launch": flop}
-    if nz > 1:
+    if nz > 1 and wl.get("u8") and gnlm.forward_u8_route(nx, ny, nz, rp, rs)[1] == gnlm.KERNEL_U8_INTEGER:
+        roofline["kernel"] = "nlm3d_fwd_u8 (8-bit samples, integer patch distances)"
+    elif nz > 1:
         # the tile variant and the launch shape the first call's tuning chose
         shape, nshapes = gnlm.forward_shape((nz, ny, nx), rp, rs)
         roofline["variant"] = gnlm.tuned_variant(rp, rs)

@@ -265,6 +265,28 @@ int gnlm_set_forward_shape(int nx, int ny, int nz, int r_p, int r_s, int shape);
  * the reference's (3DNLM/main.cu:7620 copies unsigned char volumes). */
 int gnlm_u8_to_f32(const unsigned char *d_in, float *d_out, long long n, void *stream);
 
+/* 3D forward of an 8-bit volume (the reference's input type: its launcher
+ * copies an unsigned char volume to the device, 3DNLM/main.cu:7620, and its
+ * kernels difference the samples as ints, 3DNLM/main.cu:59).  Replaces the
+ * kernel launches of 3DNLM/main.cu:7622-7667 for 8-bit data.  Layout and
+ * arguments as gnlm3d_forward_ex (in_pad, in_sy, in_sz in bytes = elements;
+ * h on the samples' 0-255 scale, the reference's PARAMETER); d_out and d_Z
+ * are FP32.  For r_p 7-8 (NS 15-17) with r_s 1-7 the patch distances are
+ * formed exactly in 32-bit integers (an 8-bit patch SSD is an integer below
+ * 2^31 for NS <= 17) and only the weights are floating point; other radii,
+ * and volumes past that kernel's 32-bit offsets (about 800^3), run the FP32
+ * kernels on a float copy made in a stream-ordered workspace.  On 8-bit
+ * samples both give the same definition to FP32 rounding. */
+int gnlm3d_forward_u8(const unsigned char *d_in, int in_pad, long long in_sy, long long in_sz, float *d_out,
+                      float *d_Z, int nx, int ny, int nz, int r_p, int r_s, float h, void *stream);
+
+/* Which kernel gnlm3d_forward_u8 runs for this geometry (host logic only, no
+ * device work; no counterpart in the reference): GNLM_KERNEL_U8_INTEGER for
+ * the integer-distance kernel, else what gnlm3d_forward_route gives for the
+ * FP32 kernels on the float copy.  Same statuses as gnlm3d_forward_route. */
+enum { GNLM_KERNEL_U8_INTEGER = 110 };
+int gnlm3d_forward_u8_route(int nx, int ny, int nz, int r_p, int r_s, int *kernel);
+
 /* ---- host-pointer, synchronous entry points (the drop-in's backing) ----
  * They allocate device memory for one call, copy in, run, copy out and free,
  * as the reference's launchers do (3DNLM/main.cu:7609-7676).  (Parameter

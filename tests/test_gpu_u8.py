@@ -1,0 +1,114 @@
+"""GPU parity of the 8-bit forward (gnlm3d_forward_u8) against the CPU oracle.
+
+For r_p 7-8 the 8-bit kernel forms every patch distance exactly in 32-bit
+integers (S = A(x) + A(x+d) - 2 C_d(x)); only the weights and their sums are
+FP32.  Bars, stated per comparison:
+  * vs the oracle's FP64 separable path and its FP32 path, on the float
+    values of the same 8-bit samples: max relative error
+    |a-b|/max(|b|,1e-6) <= 1e-5 (outputs are weighted means of samples in
+    0-255 and sit well away from 0 on these inputs);
+  * the normaliser Z vs FP64: <= 1e-5 relative;
+  * a caller-padded input (the reference's layout), the host-pointer drop-in
+    and the unpadded device entry point: bit-identical (one kernel, one copy);
+  * radii outside r_p 7-8 run the FP32 kernels on a float copy: bit-identical
+    to gnlm.nlm3d on the converted volume.
+"""
+import numpy as np
+import pytest
+import torch
+
+import gnlm
+import oracle
+from nlm_metrics import max_rel, report
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def _u8(dev, a):
+    return torch.from_numpy(np.ascontiguousarray(a, np.uint8)).to(dev)
+
+
+def _phantom_u8(shape, seed, sigma=0.05):
+    nz, ny, nx = shape
+    u, _ = gnlm.phantom3d(nx, ny, nz, n_ellipsoids=8, axis_lo=3.0, axis_hi=12.0, background=0.05, sigma=sigma,
+                          seed=seed)
+    return np.clip(np.rint(u * 255.0), 0, 255).astype(np.uint8)
+
+
+U8_CASES = [
+    # (nz, ny, nx), r_p, r_s, h (0-255 scale), input
+    ((24, 22, 21), 8, 6, 10.0, "phantom"),   # the reference's default NS = 17, SS = 13, PARAMETER = 10
+    ((24, 22, 21), 8, 6, 90.0, "uniform"),
+    ((40, 36, 33), 8, 3, 60.0, "uniform"),   # several tiles per axis, partial tiles
+    ((33, 35, 21), 7, 5, 10.0, "phantom"),   # NS = 15, SS = 11 (the reference's sweep)
+    ((17, 19, 23), 8, 7, 70.0, "uniform"),   # largest instantiated r_s
+    ((20, 37, 18), 7, 7, 70.0, "uniform"),
+    ((18, 16, 19), 7, 2, 80.0, "uniform"),
+    ((10, 9, 8), 8, 1, 90.0, "uniform"),     # smaller than one tile: reflection folds more than once
+    ((1, 30, 37), 7, 3, 60.0, "uniform"),    # one plane
+    ((35, 17, 50), 8, 4, 40.0, "uniform"),   # a partial last z layer of tiles
+]
+
+
+@pytest.mark.parametrize("shape,rp,rs,h,kind", U8_CASES)
+def test_u8_matches_oracle(dev, shape, rp, rs, h, kind):
+    """Integer patch distances, FP32 weights: within 1e-5 of the FP64 and the
+    CPU FP32 separable paths on the float values of the same samples."""
+    seed = sum(shape) + 31 * rp + rs
+    if kind == "phantom":
+        img = _phantom_u8(shape, seed)
+    else:
+        img = np.random.default_rng(seed).integers(0, 256, shape, dtype=np.uint8)
+    print(f"case u8 shape={shape} rp={rp} rs={rs} h={h} {kind}", flush=True)
+    nz, ny, nx = shape
+    assert gnlm.forward_u8_route(nx, ny, nz, rp, rs) == (gnlm.OK, gnlm.KERNEL_U8_INTEGER)
+    y, Z = gnlm.nlm3d_u8(_u8(dev, img), rp, rs, h, want_Z=True)
+    torch.cuda.synchronize()
+    y, Z = y.cpu().numpy(), Z.cpu().numpy()
+    u = img.astype(np.float32)
+    ref64, Z64, _ = oracle.nlm3d_separable_f64_za(u.astype(np.float64), rp, rs, h)
+    ref32 = oracle.nlm3d_separable_f32(u, rp, rs, h)
+    print("u8 vs fp64", report(y, ref64), "vs fp32", report(y, ref32), "Z vs fp64", report(Z, Z64))
+    assert max_rel(y, ref64) <= TOL, report(y, ref64)
+    assert max_rel(y, ref32) <= TOL, report(y, ref32)
+    assert max_rel(Z, Z64) <= TOL, report(Z, Z64)
+
+
+def test_u8_padded_and_dropin_equal_unpadded(dev):
+    """The reference's padded 8-bit layout, through the device entry point
+    and through the host-pointer drop-in (gnlm_nl_means_3d_u8), gives the
+    same bits as the unpadded input with reflect-101 on device."""
+    rng = np.random.default_rng(21)
+    nz, ny, nx = 19, 26, 35
+    img = rng.integers(0, 256, (nz, ny, nx), dtype=np.uint8)
+    rp, rs, h = 8, 6, 10.0
+    R = rp + rs
+    padded = np.pad(img, R, mode="reflect")
+    a = gnlm.nlm3d_u8(_u8(dev, img), rp, rs, h).cpu().numpy()
+    b = gnlm.nl_means_3d_u8(padded, nx, ny, nz, 2 * rs + 1, 2 * rp + 1, h)
+    assert np.array_equal(a, b.reshape(a.shape)), report(b.reshape(a.shape), a)
+
+
+@pytest.mark.parametrize("rp,rs", [(1, 3), (2, 5), (3, 2), (4, 3), (6, 5), (9, 1)])
+def test_u8_other_radii_run_float_kernels(dev, rp, rs):
+    """Radii outside r_p 7-8 run the FP32 kernels on a float copy: the same
+    bits as gnlm.nlm3d on the converted volume."""
+    rng = np.random.default_rng(7 + rp)
+    img = rng.integers(0, 256, (13, 17, 21), dtype=np.uint8)
+    assert gnlm.forward_u8_route(21, 17, 13, rp, rs)[1] != gnlm.KERNEL_U8_INTEGER
+    a = gnlm.nlm3d_u8(_u8(dev, img), rp, rs, 40.0).cpu().numpy()
+    b = gnlm.nlm3d(torch.from_numpy(img.astype(np.float32)).to(dev), rp, rs, 40.0).cpu().numpy()
+    assert np.array_equal(a, b), report(a, b)
+
+
+def test_u8_close_to_float_big_tiles(dev):
+    """On 8-bit samples the integer kernel and the FP32 large-patch tiles
+    compute the same definition: within 1e-5 of each other (not bit for bit:
+    the FP32 tiles round their patch sums)."""
+    img = _phantom_u8((30, 40, 37), 5)
+    rp, rs, h = 8, 6, 10.0
+    a = gnlm.nlm3d_u8(_u8(dev, img), rp, rs, h).cpu().numpy()
+    b = gnlm.nlm3d(torch.from_numpy(img.astype(np.float32)).to(dev), rp, rs, h).cpu().numpy()
+    assert max_rel(a, b) <= TOL, report(a, b)

@@ -93,3 +93,41 @@ def test_backward_form_switch_routes():
 def test_route_rejects_bad_geometry(args):
     assert gnlm.forward_route(*args)[0] == gnlm.EINVAL
     assert gnlm.backward_route(*args)[0] == gnlm.EINVAL
+
+
+def _u8_record_bytes(n_xy, n_z, rp, rs):
+    """Size of the 8-byte (A, u) records the 8-bit forward builds: the volume
+    rounded up to 16 x (32 - 2 r_p)^2 tiles and grown by r_s per side."""
+    t = 32 - 2 * rp
+    rx = -(-n_xy // 16) * 16 + 2 * rs
+    ry = -(-n_xy // t) * t + 2 * rs
+    rz = -(-n_z // t) * t + 2 * rs
+    return 8 * rx * ry * rz
+
+
+def test_u8_route():
+    """gnlm3d_forward_u8: integer patch distances for r_p 7-8 (the reference's
+    NS 15-17) while the records fit 32-bit byte offsets; otherwise the FP32
+    route on a float copy, with its statuses."""
+    assert gnlm.forward_u8_route(512, 512, 128, 8, 6) == (gnlm.OK, gnlm.KERNEL_U8_INTEGER)
+    assert gnlm.forward_u8_route(512, 512, 16, 7, 5) == (gnlm.OK, gnlm.KERNEL_U8_INTEGER)
+    for rp, rs in ((1, 3), (2, 5), (4, 3), (6, 5), (8, 8), (9, 2)):
+        assert gnlm.forward_u8_route(64, 64, 64, rp, rs) == gnlm.forward_route(64, 64, 64, rp, rs), (rp, rs)
+    # 780^3 at NS 17 / SS 13: records below 4 GiB; 820^3: past them
+    assert _u8_record_bytes(780, 780, 8, 6) < 2 ** 32 <= _u8_record_bytes(820, 820, 8, 6)
+    assert gnlm.forward_u8_route(780, 780, 780, 8, 6) == (gnlm.OK, gnlm.KERNEL_U8_INTEGER)
+    assert gnlm.forward_u8_route(820, 820, 820, 8, 6) == gnlm.forward_route(820, 820, 820, 8, 6)
+    st, _ = gnlm.forward_u8_route(1 << 14, 1 << 14, 1 << 13, 8, 6)
+    assert st == gnlm.EUNSUPPORTED
+    assert gnlm.forward_u8_route(0, 8, 8, 8, 6)[0] == gnlm.EINVAL
+
+
+def test_u8_call_rejects_before_the_device():
+    L = gnlm.lib()
+    fake = ctypes.c_void_p(0x1000)
+    assert L.gnlm3d_forward_u8(None, 0, 0, 0, fake, None, 8, 8, 8, 8, 6, 10.0, None) == gnlm.EINVAL
+    assert L.gnlm3d_forward_u8(fake, 0, 0, 0, fake, None, 8, 8, 8, 8, 6, 0.0, None) == gnlm.EINVAL
+    # padding smaller than the halo r_p + r_s
+    assert L.gnlm3d_forward_u8(fake, 13, 0, 0, fake, None, 8, 8, 8, 8, 6, 10.0, None) == gnlm.EINVAL
+    assert L.gnlm3d_forward_u8(fake, 0, 0, 0, fake, None, 1 << 14, 1 << 14, 1 << 13, 8, 6, 10.0,
+                               None) == gnlm.EUNSUPPORTED
