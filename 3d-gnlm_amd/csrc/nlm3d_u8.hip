@@ -299,7 +299,7 @@ __device__ __forceinline__ void u8_tile(const U8Args &a, int x0, int y0, int z0,
             }
             static_for<-RS, RS>([&](auto dxc) {
                 constexpr int DX = decltype(dxc)::value;
-                if (p1ok) u8_phase1<C, RS + DX>(cen, win, SX + sxw);
+                u8_phase1<C, RS + DX>(cen, win, SX + sxw);
                 // candidate records (A, u) at x + d, in flight across the barriers
                 unsigned cA[RZ], cU[RZ];
 #pragma unroll
@@ -312,14 +312,14 @@ __device__ __forceinline__ void u8_tile(const U8Args &a, int x0, int y0, int z0,
                 if constexpr (C::WL) {
                     // the wave's own phase-1 stores are complete before its reads
                     asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-                    if (p2ok) u8_phase2<C>(SX + p2src, SXY + sbuf + p2dst);
+                    u8_phase2<C>(SX + p2src, SXY + sbuf + p2dst);
                     lds_barrier();
                 } else {
                     lds_barrier();
                     if (p2ok) u8_phase2<C>(SX + p2src, SXY + p2dst);
                     lds_barrier();
                 }
-                if (p3ok) {
+                {
                     int v[C::NV3];
                     const int *s3 = SXY + sbuf + p3r;
 #pragma unroll
