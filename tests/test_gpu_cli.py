@@ -72,6 +72,30 @@ def test_cli_3d_output_bin_and_slices(tmp_path):
     _close_bytes(got, ref, "trunc")
 
 
+def test_cli_3d_reference_default_geometry(tmp_path):
+    """The reference's default SS = 13 / NS = 17 (3DNLM/config.h:4-5) on its
+    own 128.bmp, 6 slices deep: the command line reaches the 8-bit kernel
+    with integer patch distances (gnlm_nl_means_3d_u8 -> gnlm3d_forward_u8),
+    and output.bin holds the floor of the FP64 oracle (one byte of slack only
+    within 1e-3 of an integer)."""
+    import gnlm
+    img = _gray(FIXTURE, tmp_path)
+    depth, ss, ns = 6, 13, 17
+    assert gnlm.forward_u8_route(128, 128, depth, ns // 2, ss // 2)[1] == gnlm.KERNEL_U8_INTEGER
+    out = tmp_path / "output.bin"
+    print(f"case cli 3d ss={ss} ns={ns} depth={depth}", flush=True)
+    r = subprocess.run([NLM, FIXTURE, "--ss", str(ss), "--ns", str(ns), "--depth", str(depth), "--out", str(out)],
+                       capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    b = out.read_bytes()
+    assert tuple(np.frombuffer(b[:12], np.int32)) == (depth, 128, 128)
+    got = np.frombuffer(b[12:], np.uint8).reshape(depth, 128, 128)
+    vol = np.repeat(img[None].astype(np.float32), depth, axis=0)
+    ref = oracle.nlm3d_separable_f64(vol, ns // 2, ss // 2, 10.0)
+    _close_bytes(got, ref, "trunc")
+
+
 def test_cli_2d_output_and_padded_bmp(tmp_path):
     img = _gray(FIXTURE, tmp_path)
     sr, nb = 7, 3
