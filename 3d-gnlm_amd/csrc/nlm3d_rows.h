@@ -33,10 +33,12 @@
 //     conflict-free for 64 banks (row pitches of 20 floats, plane paddings
 //     of 16 floats).
 //   * Weights accumulate in two levels (partial over the (dy, dx) offsets of
-//     one dz, then the total over dz) as in the other variants; the totals
-//     live in the output buffers between dz steps (one read-modify-write of
-//     two floats per voxel per dz), which keeps the register file for the
-//     rows and partials.
+//     one dz, then the total over dz) as in the other variants.  The r_p 1-4
+//     tiles keep the totals in registers and write y and Z once per voxel;
+//     the big tiles (1024 threads, 128 VGPRs) keep them in the output
+//     buffers between dz steps (one read-modify-write of two floats per
+//     voxel per dz).  Either way each total is (old total + partial), in dz
+//     order, so both give the same bits.
 // Box sums add non-negative terms only (no running add/subtract windows), in
 // shared pairwise trees instead of the CPU oracle's left-to-right chain, so the
 // results agree with the oracle to rounding (~1e-7 relative in S) rather
@@ -85,6 +87,8 @@ template <int RP_, int RS_, bool SHORT_ = false, int ZM_ = 1>
 struct RowsCfg {
     static constexpr int RP = RP_, RS = RS_, ZM = ZM_;
     static constexpr bool SHORT = SHORT_;
+    // Y, Z totals over dz stay in registers: one write of y and Z per voxel
+    static constexpr bool REG_TOTALS = true;
     static_assert(!SHORT || RP == 2, "short tiles are for r_p = 2");
     static_assert(ZM == 1 || (ZM == 2 && (RP != 2 || SHORT)), "deep tiles: r_p 1, 3, 4 or short r_p 2");
     static constexpr int NT = kRowsThreads * ZM;         // threads per workgroup
@@ -136,6 +140,10 @@ struct RowsBigCfg {
     static constexpr int RP = RP_, RS = RS_;
     static_assert(RP >= 4 && RP <= 8, "big tiles are for r_p 4-8");
     static constexpr bool SHORT = false;
+    // 1024 threads leave 128 VGPRs: the totals live in the output buffers
+    // between dz steps (registers for them cost more in spills than the
+    // read-modify-write of two floats per voxel and dz)
+    static constexpr bool REG_TOTALS = false;
     static constexpr int NT = 1024;
     static constexpr int W = 2 * RP + 1;
     static constexpr int R = RP + RS;
@@ -270,8 +278,8 @@ __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(
 
 struct RowsTile {
     PadVol v;
-    float *out;    // y, and the running Y total between dz steps
-    float *Zacc;   // running Z total between dz steps; Z at the end
+    float *out;    // y (the big tiles: the running Y total between dz steps)
+    float *Zacc;   // Z (the big tiles: the running Z total between dz steps)
     int nx, ny, nz;
     float k;       // w = exp2(S * k), k = -log2(e) / (P h^2)
     int x0, y0, z0;
@@ -428,6 +436,83 @@ __device__ __forceinline__ void rows_tile(const RowsTile &ta, float *lds)
     const unsigned vbytes = vec_ok ? (unsigned)(4 * nvox) : 0u;
     const PadRsrc ro = buf_rsrc(a.out, vbytes), rz = buf_rsrc(a.Zacc, vbytes);
 
+    // Write (Z, Y) sums of this thread's items: with acc, added to the totals
+    // already in the output buffers (Zacc, out); with last, y = Y / Z is stored.
+    auto store = [&](float(&Zs)[C::P3R][V3][RZ], float(&Ys)[C::P3R][V3][RZ], bool acc, bool last) {
+#pragma unroll
+        for (int k = 0; k < C::P3R; ++k) {
+            const int it = tid + k * NT;
+            const int x = (it % NXG) * V3, y = (it / NXG) % C::TY, zr = it / (NXG * C::TY);
+            const int gy = a.y0 + y, gz = a.z0 + zr * RZ;
+            if (!p3ok[k] || it >= C::P3N || gy >= a.ny) continue;
+            if constexpr (V3 > 1) {
+                // a whole group at an aligned position: one 64/128-bit access per
+                // buffer and plane (vec_ok: aligned buffers, rows of a multiple of
+                // V3 samples, byte offsets within 32 bits)
+                if (vec_ok && a.x0 + x + V3 - 1 < a.nx) {
+#pragma unroll
+                    for (int o = 0; o < RZ; ++o) {
+                        if (gz + o >= a.nz) continue;
+                        const unsigned boff = 4u * (unsigned)(a.x0 + x + a.nx * (gy + a.ny * (gz + o)));
+                        float Z[V3], Y[V3];
+#pragma unroll
+                        for (int c = 0; c < V3; ++c) {
+                            Z[c] = Zs[k][c][o];
+                            Y[c] = Ys[k][c][o];
+                        }
+                        if (acc) {
+                            unsigned zt[V3], yt[V3];
+                            buf_load_v<V3>(rz, boff, 0, zt);
+                            buf_load_v<V3>(ro, boff, 0, yt);
+#pragma unroll
+                            for (int c = 0; c < V3; ++c) {
+                                Z[c] = __uint_as_float(zt[c]) + Z[c];
+                                Y[c] = __uint_as_float(yt[c]) + Y[c];
+                            }
+                        }
+                        unsigned zs[V3], ys[V3];
+#pragma unroll
+                        for (int c = 0; c < V3; ++c) {
+                            zs[c] = __float_as_uint(Z[c]);
+                            ys[c] = __float_as_uint(last ? Y[c] / Z[c] : Y[c]);
+                        }
+                        buf_store_v<V3>(rz, boff, zs);
+                        buf_store_v<V3>(ro, boff, ys);
+                    }
+                    continue;
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < V3; ++c) {
+                const int gx = a.x0 + x + c;
+                if (gx >= a.nx) continue;
+#pragma unroll
+                for (int o = 0; o < RZ; ++o) {
+                    if (gz + o >= a.nz) continue;
+                    const int oi = gx + a.nx * (gy + a.ny * (gz + o));
+                    float Z = Zs[k][c][o], Y = Ys[k][c][o];
+                    if (acc) {
+                        Z = Zacc[oi] + Z;
+                        Y = out[oi] + Y;
+                    }
+                    Zacc[oi] = Z;
+                    out[oi] = last ? Y / Z : Y;
+                }
+            }
+        }
+    };
+
+    // totals over dz, in registers when the tile keeps them there
+    float Zt[C::P3R][V3][RZ], Yt[C::P3R][V3][RZ];
+#pragma unroll
+    for (int k = 0; k < C::P3R; ++k)
+#pragma unroll
+        for (int c = 0; c < V3; ++c)
+#pragma unroll
+            for (int o = 0; o < RZ; ++o) {
+                Zt[k][c][o] = 0.0f;
+                Yt[k][c][o] = 0.0f;
+            }
     for (int dz = -RS; dz <= RS; ++dz) {
         float Zd[C::P3R][V3][RZ], Yd[C::P3R][V3][RZ];
 #pragma unroll
@@ -510,69 +595,23 @@ __device__ __forceinline__ void rows_tile(const RowsTile &ta, float *lds)
                 }
             });
         }
-        // close the dz block: totals += partials (kept in the output buffers)
+        // close the dz block: totals += partials (old total + partial, in
+        // this order, in registers or in the output buffers)
+        if constexpr (C::REG_TOTALS) {
 #pragma unroll
-        for (int k = 0; k < C::P3R; ++k) {
-            const int it = tid + k * NT;
-            const int x = (it % NXG) * V3, y = (it / NXG) % C::TY, zr = it / (NXG * C::TY);
-            const int gy = a.y0 + y, gz = a.z0 + zr * RZ;
-            if (!p3ok[k] || it >= C::P3N || gy >= a.ny) continue;
-            if constexpr (V3 > 1) {
-                // a whole group at an aligned position: one 64/128-bit access per
-                // buffer and plane (vec_ok: aligned buffers, rows of a multiple of
-                // V3 samples, byte offsets within 32 bits)
-                if (vec_ok && a.x0 + x + V3 - 1 < a.nx) {
+            for (int k = 0; k < C::P3R; ++k)
+#pragma unroll
+                for (int c = 0; c < V3; ++c)
 #pragma unroll
                     for (int o = 0; o < RZ; ++o) {
-                        if (gz + o >= a.nz) continue;
-                        const unsigned boff = 4u * (unsigned)(a.x0 + x + a.nx * (gy + a.ny * (gz + o)));
-                        float Z[V3], Y[V3];
-#pragma unroll
-                        for (int c = 0; c < V3; ++c) {
-                            Z[c] = Zd[k][c][o];
-                            Y[c] = Yd[k][c][o];
-                        }
-                        if (dz != -RS) {
-                            unsigned zt[V3], yt[V3];
-                            buf_load_v<V3>(rz, boff, 0, zt);
-                            buf_load_v<V3>(ro, boff, 0, yt);
-#pragma unroll
-                            for (int c = 0; c < V3; ++c) {
-                                Z[c] = __uint_as_float(zt[c]) + Z[c];
-                                Y[c] = __uint_as_float(yt[c]) + Y[c];
-                            }
-                        }
-                        unsigned zs[V3], ys[V3];
-#pragma unroll
-                        for (int c = 0; c < V3; ++c) {
-                            zs[c] = __float_as_uint(Z[c]);
-                            ys[c] = __float_as_uint((dz == RS) ? Y[c] / Z[c] : Y[c]);
-                        }
-                        buf_store_v<V3>(rz, boff, zs);
-                        buf_store_v<V3>(ro, boff, ys);
+                        Zt[k][c][o] = Zt[k][c][o] + Zd[k][c][o];
+                        Yt[k][c][o] = Yt[k][c][o] + Yd[k][c][o];
                     }
-                    continue;
-                }
-            }
-#pragma unroll
-            for (int c = 0; c < V3; ++c) {
-                const int gx = a.x0 + x + c;
-                if (gx >= a.nx) continue;
-#pragma unroll
-                for (int o = 0; o < RZ; ++o) {
-                    if (gz + o >= a.nz) continue;
-                    const int oi = gx + a.nx * (gy + a.ny * (gz + o));
-                    float Z = Zd[k][c][o], Y = Yd[k][c][o];
-                    if (dz != -RS) {
-                        Z = Zacc[oi] + Z;
-                        Y = out[oi] + Y;
-                    }
-                    Zacc[oi] = Z;
-                    out[oi] = (dz == RS) ? Y / Z : Y;
-                }
-            }
+        } else {
+            store(Zd, Yd, dz != -RS, dz == RS);
         }
     }
+    if constexpr (C::REG_TOTALS) store(Zt, Yt, false, true);
 }
 
 }  // namespace gnlm
