@@ -30,14 +30,14 @@
 //     compile-time register indices;
 //   * phase 1: prefix sums of the products along the row give the x boxes of
 //     -2 u(x) u(x + d) -> LDS; phase 2: sliding y boxes of (x, z_ext)
-//     columns -> LDS (for r_p = 8 each wave boxes the two planes of rows it
+//     columns -> LDS (for r_p 7-8 each wave boxes the two planes of rows it
 //     wrote itself, into one of two buffers by the offset's parity, so one
 //     barrier per offset suffices); phase 3: (x, y, run of RZ z) items,
 //     sliding z boxes read with 128/64-bit accesses (z-fastest layout), then
 //     S = A(x) + A(x + d) + box, w = exp2(S k), per-dz partial sums in
 //     registers; (A, u) of the candidate x + d arrive as one 64-bit record;
-//   * weights accumulate in two levels (per dz, then over dz) through the
-//     output buffers, as in the float kernels.
+//   * weights accumulate in two levels (per dz, then over dz), the totals in
+//     registers, written once per voxel.
 #include <hip/hip_runtime.h>
 #include "gnlm_internal.h"
 #include "nlm3d_rows.h"
@@ -70,9 +70,13 @@ struct U8Cfg {
     // mod 64) as phase 1 writes it; s_xy [y][x][z_ext], z fastest, so that a
     // phase-3 item reads its z run with 64/128-bit accesses (x pitch 36: the
     // 16 x of one y fall on distinct bank quads; y pitch 580: the two halves
-    // of a wave-local phase 2 write 32 banks apart)
+    // of a wave-local phase 2 write 32 banks apart).  r_p = 7 (18 y rows)
+    // takes x pitch 34 (distinct bank pairs for its 64-bit reads), the
+    // widest with which s_x and two s_xy buffers still fit 160 KiB: the
+    // two-barrier form holds more registers live across the offset loop
+    // than 128 VGPRs and spilled about 400 of them, 10x slower.
     static constexpr int SXR = 20, SXP = SXR * EY + 16, SXN = SXP * EZ;
-    static constexpr int SYX = 36, SYY = 16 * SYX + 4, SYN = SYY * TY;
+    static constexpr int SYX = RP == 7 ? 34 : 36, SYY = 16 * SYX + 4, SYN = SYY * TY;
     // WL ("wave-local"): where s_x and two s_xy buffers fit the LDS, each wave
     // computes the y boxes of the two z_ext planes of rows it wrote itself in
     // phase 1 -- no barrier between phases 1 and 2 -- into the s_xy buffer of
@@ -80,6 +84,7 @@ struct U8Cfg {
     // orders everything; otherwise waves 0-7 run phase 2 on whole columns
     // between two barriers.
     static constexpr bool WL = (SXN + 2 * SYN) * 4 <= 160 * 1024;
+    static_assert(WL || RP < 7, "r_p 7-8 run the wave-local form");
     // phase 2 items: (x, z_ext) columns, whole (NP2 = 1) or, wave-local, in
     // two runs of RY outputs (64 items per wave: 16 x, 2 planes, 2 runs)
     static constexpr int NP2 = WL ? 2 : 1;
@@ -116,8 +121,8 @@ struct RecVol {
 struct U8Args {
     U8Vol v;
     RecVol rc;
-    float *out;   // y, and the running Y total between dz steps
-    float *Zacc;  // running Z total between dz steps; Z at the end
+    float *out;   // y
+    float *Zacc;  // Z (the normaliser)
     int nx, ny, nz;
     float k;
     int ntx, nty, ntz;
@@ -281,6 +286,12 @@ __device__ __forceinline__ void u8_tile(const U8Args &a, int x0, int y0, int z0,
     for (int o = 0; o < RZ; ++o)
         Ax[o] = (int)__builtin_amdgcn_raw_buffer_load_b32(rr.r, (int)cb, 8 * ((RS + o) * rxy + RS * rx + RS), 0);
 
+    float Zt[RZ], Yt[RZ];  // totals over dz: old total + partial, in dz order
+#pragma unroll
+    for (int o = 0; o < RZ; ++o) {
+        Zt[o] = 0.0f;
+        Yt[o] = 0.0f;
+    }
     for (int dz = -RS; dz <= RS; ++dz) {
         float Zd[RZ], Yd[RZ];
 #pragma unroll
@@ -353,21 +364,20 @@ __device__ __forceinline__ void u8_tile(const U8Args &a, int x0, int y0, int z0,
                 if constexpr (C::WL) sbuf = C::SYN - sbuf;
             });
         }
-        // close the dz block: totals += partials (kept in the output buffers)
-        const int gx = x0 + p3x, gy = y0 + p3y, gz = z0 + p3zr * RZ;
-        if (p3ok && tid < C::P3N && gx < a.nx && gy < a.ny) {
 #pragma unroll
-            for (int o = 0; o < RZ; ++o) {
-                if (gz + o >= a.nz) continue;
-                const int oi = gx + a.nx * (gy + a.ny * (gz + o));
-                float Z = Zd[o], Y = Yd[o];
-                if (dz != -RS) {
-                    Z = Zacc[oi] + Z;
-                    Y = out[oi] + Y;
-                }
-                Zacc[oi] = Z;
-                out[oi] = (dz == RS) ? Y / Z : Y;
-            }
+        for (int o = 0; o < RZ; ++o) {
+            Zt[o] = Zt[o] + Zd[o];
+            Yt[o] = Yt[o] + Yd[o];
+        }
+    }
+    const int gx = x0 + p3x, gy = y0 + p3y, gz = z0 + p3zr * RZ;
+    if (p3ok && tid < C::P3N && gx < a.nx && gy < a.ny) {
+#pragma unroll
+        for (int o = 0; o < RZ; ++o) {
+            if (gz + o >= a.nz) continue;
+            const int oi = gx + a.nx * (gy + a.ny * (gz + o));
+            Zacc[oi] = Zt[o];
+            out[oi] = Yt[o] / Zt[o];
         }
     }
 }
