@@ -112,3 +112,36 @@ def test_u8_close_to_float_big_tiles(dev):
     a = gnlm.nlm3d_u8(_u8(dev, img), rp, rs, h).cpu().numpy()
     b = gnlm.nlm3d(torch.from_numpy(img.astype(np.float32)).to(dev), rp, rs, h).cpu().numpy()
     assert max_rel(a, b) <= TOL, report(a, b)
+
+
+def _median_ms(fn, reps=5):
+    fn()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ts.append(e0.elapsed_time(e1))
+    return float(np.median(ts))
+
+
+def test_u8_rp7_rate_matches_rp8(dev):
+    """Guard against register spills in the offset loop: r_p = 7 tiles
+    (16 x 18 x 18) must process voxels at least about as fast as r_p = 8
+    tiles (16 x 16 x 16) at the same search radius.  A form of the r_p = 7
+    kernel that spilled about 400 registers ran 10x slower than r_p = 8; the
+    kept one runs within 0.9-1.1x of its time (256 x 256 x 64, r_s 3: 2.51
+    against 2.31 ms).  The bar (1.6x) leaves room for
+    timing noise only."""
+    img = np.random.default_rng(3).integers(0, 256, (64, 256, 256), dtype=np.uint8)
+    t8 = _u8(dev, img)
+    out = torch.empty(img.shape, dtype=torch.float32, device=dev)
+    assert gnlm.forward_u8_route(256, 256, 64, 7, 3)[1] == gnlm.KERNEL_U8_INTEGER
+    ms7 = _median_ms(lambda: gnlm.nlm3d_u8(t8, 7, 3, 10.0, out=out))
+    ms8 = _median_ms(lambda: gnlm.nlm3d_u8(t8, 8, 3, 10.0, out=out))
+    print(f"u8 256x256x64 r_s 3: r_p 7 {ms7:.3f} ms, r_p 8 {ms8:.3f} ms", flush=True)
+    assert ms7 < 1.6 * ms8, (ms7, ms8)
