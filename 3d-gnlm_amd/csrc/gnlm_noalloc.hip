@@ -2,9 +2,9 @@
 // cannot allocate device memory (code under stream capture, tight serving
 // loops, harnesses that forbid hipMalloc).  It runs the same kernels as
 // gnlm3d_forward and gives the same bits; the one workspace those kernels need
-// (the rows variants' padded copy of the input, and the running Z total when
-// the caller passes no Z buffer) comes from a fixed device array instead of
-// hipMallocAsync.
+// (the rows variants' padded copy of the input, and for the large-patch tiles
+// the running Z total when the caller passes no Z buffer) comes from a fixed
+// device array instead of hipMallocAsync.
 //
 // The array is `scratch` below: every device the library runs on holds its
 // own copy, at its own address, until the process ends.  Host code must never

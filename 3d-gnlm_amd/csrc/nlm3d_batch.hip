@@ -43,8 +43,8 @@ struct BatchVol {
     float sigma;
     int variant;
     int ntx, nty;
-    // rows-variant volumes: padded copy (filled by batch_pad inside the graph) and
-    // the Z accumulator (Zout, or plan workspace when the caller wants no Z)
+    // rows-variant volumes: padded copy (filled by batch_pad inside the graph);
+    // Zacc = Zout (NULL when the caller wants no Z: the tiles write y alone)
     PadVol pv;
     float *Zacc;
     int px, py, pz, lp, halo;
@@ -75,7 +75,13 @@ __device__ __noinline__ void tile_direct(const Fwd3Tile &a, float *lds) { fwd3_t
 // LIGHT instances are called from the three-waves-per-SIMD kernel only (a
 // separate function, so that the register limit of that kernel is theirs).
 template <int RP, int RS, bool SH, bool LIGHT>
-__device__ __noinline__ void tile_rows(const RowsTile &a, float *lds) { rows_tile<RowsCfg<RP, RS, SH>>(a, lds); }
+__device__ __noinline__ void tile_rows(const RowsTile &a, float *lds)
+{
+    // volumes without a Z buffer pass Zacc = NULL: the tiles must hold their
+    // totals in registers
+    static_assert(RowsCfg<RP, RS, SH>::REG_TOTALS, "batched rows tiles keep their totals in registers");
+    rows_tile<RowsCfg<RP, RS, SH>>(a, lds);
+}
 
 // kVRows / kVRowsShort tile: r_s is a template parameter of the tile body (its
 // dx loop is unrolled), so dispatch on it; the cases match nlm3d_rows.hip.
@@ -281,7 +287,7 @@ struct gnlm_batch_plan {
     int *d_padids = nullptr;   // their indices
     int padmax_xy = 0;         // largest plane of a padded copy (floats)
     int padmax_z = 0;          // most planes of a padded copy
-    float *d_ws = nullptr;     // padded copies + Z accumulators
+    float *d_ws = nullptr;     // padded copies
     bool light = false;        // every volume fits the three-waves-per-SIMD kernel
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -342,7 +348,7 @@ int gnlm_batch_plan_create(const gnlm_batch_volume *vols, int n, gnlm_batch_plan
     std::vector<std::vector<BatchTile>> vt;  // tiles of each volume, x fastest
     std::vector<double> cost;                 // per-tile cost of each volume
     std::vector<int> padids;
-    std::vector<long long> wsoff((size_t)n, -1), zoff((size_t)n, -1);
+    std::vector<long long> wsoff((size_t)n, -1);
     long long wsn = 0;
     int padmax_xy = 0, padmax_z = 0;
     size_t lds = 0;
@@ -386,10 +392,6 @@ int gnlm_batch_plan_create(const gnlm_batch_volume *vols, int n, gnlm_batch_plan
             b.pv.bytes = (unsigned)(sizeof(float) * np);
             wsoff[(size_t)i] = wsn;
             wsn += (np + 63) / 64 * 64;
-            if (!v.d_Z) {
-                zoff[(size_t)i] = wsn;
-                wsn += (nv + 63) / 64 * 64;
-            }
             padmax_xy = std::max(padmax_xy, b.px * b.py);
             padmax_z = std::max(padmax_z, b.pz);
             padids.push_back(i);
@@ -444,7 +446,7 @@ int gnlm_batch_plan_create(const gnlm_batch_volume *vols, int n, gnlm_batch_plan
         BatchVol &b = hv[(size_t)i];
         if (!fwd3_is_rows(b.variant)) continue;
         b.pv.p = p->d_ws + wsoff[(size_t)i];
-        b.Zacc = b.Zout ? b.Zout : p->d_ws + zoff[(size_t)i];
+        b.Zacc = b.Zout;  // NULL: y alone (the tiles keep their totals in registers)
     }
     if (e == hipSuccess) e = hipMalloc((void **)&p->d_vols, sizeof(BatchVol) * hv.size());
     if (e == hipSuccess) e = hipMalloc((void **)&p->d_tiles, sizeof(BatchTile) * sorted.size());

@@ -279,7 +279,8 @@ __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(
 struct RowsTile {
     PadVol v;
     float *out;    // y (the big tiles: the running Y total between dz steps)
-    float *Zacc;   // Z (the big tiles: the running Z total between dz steps)
+    float *Zacc;   // Z, or NULL for no Z (the big tiles: the running Z total
+                   // between dz steps, never NULL)
     int nx, ny, nz;
     float k;       // w = exp2(S * k), k = -log2(e) / (P h^2)
     int x0, y0, z0;
@@ -438,6 +439,8 @@ __device__ __forceinline__ void rows_tile(const RowsTile &ta, float *lds)
 
     // Write (Z, Y) sums of this thread's items: with acc, added to the totals
     // already in the output buffers (Zacc, out); with last, y = Y / Z is stored.
+    // No Z buffer (register totals only): y alone.
+    const bool wz = !C::REG_TOTALS || a.Zacc != nullptr;
     auto store = [&](float(&Zs)[C::P3R][V3][RZ], float(&Ys)[C::P3R][V3][RZ], bool acc, bool last) {
 #pragma unroll
         for (int k = 0; k < C::P3R; ++k) {
@@ -476,7 +479,7 @@ __device__ __forceinline__ void rows_tile(const RowsTile &ta, float *lds)
                             zs[c] = __float_as_uint(Z[c]);
                             ys[c] = __float_as_uint(last ? Y[c] / Z[c] : Y[c]);
                         }
-                        buf_store_v<V3>(rz, boff, zs);
+                        if (wz) buf_store_v<V3>(rz, boff, zs);
                         buf_store_v<V3>(ro, boff, ys);
                     }
                     continue;
@@ -495,7 +498,7 @@ __device__ __forceinline__ void rows_tile(const RowsTile &ta, float *lds)
                         Z = Zacc[oi] + Z;
                         Y = out[oi] + Y;
                     }
-                    Zacc[oi] = Z;
+                    if (wz) Zacc[oi] = Z;
                     out[oi] = last ? Y / Z : Y;
                 }
             }

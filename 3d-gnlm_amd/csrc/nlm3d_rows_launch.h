@@ -69,12 +69,14 @@ constexpr size_t rows_lds_cfg()
     return C::LDS_FLOATS * sizeof(float);
 }
 
-// Workspace = padded copy (+ the Z accumulator when the caller wants no Z),
-// allocated and released on the stream around the two launches -- or, when
-// the caller hands one in (gws), taken from it: nothing is allocated then,
-// and a workspace too small is hipErrorOutOfMemory.  Every float of the
-// padded copy is written by rows_pad and the Z accumulator is written before
-// it is read (first dz), so a reused workspace needs no clearing.
+// Workspace = padded copy (+ the Z accumulator when the caller wants no Z
+// and the tiles keep their totals in the output buffers), allocated and
+// released on the stream around the two launches -- or, when the caller
+// hands one in (gws), taken from it: nothing is allocated then, and a
+// workspace too small is hipErrorOutOfMemory.  Every float of the padded copy
+// is written by rows_pad and the Z accumulator is written before it is read
+// (first dz), so a reused workspace needs no clearing.  Tiles with register
+// totals and no Z wanted get Zacc = NULL and write y alone.
 template <class C>
 hipError_t rows_launch_cfg(const InDesc &in, float *out, float *Zout, float k, hipStream_t s, void (*kernel)(RowsArgs),
                            const Workspace *gws = nullptr)
@@ -94,7 +96,8 @@ hipError_t rows_launch_cfg(const InDesc &in, float *out, float *Zout, float k, h
     const long long npad = (long long)px * py * pz;
     const long long nv = (long long)in.nx * in.ny * in.nz;
     if (npad >= (1LL << 30) || nv >= (1LL << 31)) return hipErrorInvalidValue;
-    const size_t bytes = sizeof(float) * (size_t)(npad + (Zout ? 0 : nv));
+    const bool zws = !Zout && !C::REG_TOTALS;  // Z accumulator in the workspace
+    const size_t bytes = sizeof(float) * (size_t)(npad + (zws ? nv : 0));
     float *ws = nullptr;
     hipError_t e = hipSuccess;
     if (gws) {
@@ -108,7 +111,7 @@ hipError_t rows_launch_cfg(const InDesc &in, float *out, float *Zout, float k, h
     a.v.px = px;
     a.v.pxy = px * py;
     a.v.bytes = (unsigned)(sizeof(float) * npad);
-    a.Zacc = Zout ? Zout : ws + npad;
+    a.Zacc = Zout ? Zout : (zws ? ws + npad : nullptr);
     const int gx = (px * py + 255) / 256 < 1024 ? (px * py + 255) / 256 : 1024;
     const int gy = pz < 65535 ? pz : 65535;
     hipLaunchKernelGGL(rows_pad, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, s, in, ws, px, py, pz, C::LP, C::R);

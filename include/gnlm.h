@@ -78,10 +78,10 @@ int gnlm3d_forward_ex(const float *d_in, int in_pad, long long in_sy, long long 
  * callers that cannot call hipMalloc (code under stream capture, tight
  * serving loops, harnesses that forbid allocation).  Same arguments, kernels
  * and bits as gnlm3d_forward.  The workspace the kernels need (the padded
- * copy of the input, and the running Z total when d_Z is NULL) comes from a
- * fixed device array of GNLM_NOALLOC_SCRATCH_BYTES that each device the
- * library runs on holds until the process ends: about 192^3 voxels at patch
- * 5^3 / search 11^3 with d_Z NULL.  GNLM_ENOMEM (before anything is
+ * copy of the input; for the large-patch tiles of r_p 5-8 also the running Z
+ * total when d_Z is NULL) comes from a fixed device array of
+ * GNLM_NOALLOC_SCRATCH_BYTES that each device the library runs on holds
+ * until the process ends: about 240^3 voxels at patch 5^3 / search 11^3.  GNLM_ENOMEM (before anything is
  * launched) for a volume that needs more; GNLM_EUNSUPPORTED where the route
  * is the generic per-offset path, which allocates.  All calls on one device
  * share its array, so calls that overlap on one device (other streams, other

@@ -40,14 +40,23 @@ def _both(dev, shape, rp, rs, h, want_Z, seed):
 
 
 def test_noalloc_matches_forward_two_calls(dev):
-    # first call: the 96x96x48 configuration, no Z buffer (the running Z total
-    # lives in the scratch array too)
+    # first call: the 96x96x48 configuration, no Z buffer (the rows tiles then
+    # write y alone)
     y1 = _both(dev, (48, 96, 96), 1, 3, 0.05, False, 20261204)
     assert torch.isfinite(y1).all()
     # second call, smaller and with other radii, on the array the first left dirty
     _both(dev, (30, 36, 40), 2, 5, 0.05, True, 11)
     # and a third on yet another shape class (big-patch tiles, no Z)
     _both(dev, (20, 24, 28), 8, 2, 0.2, False, 12)
+
+
+def test_noalloc_capacity_without_Z(dev):
+    """The rows tiles keep their totals in registers, so without a Z buffer
+    the scratch array holds the padded copy alone: 232^3 at patch 5^3 /
+    search 11^3 (a 66 MB copy) fits the 64 MiB array; with a Z buffer the
+    caller's own Z takes the totals, so the same volume fits as well."""
+    _both(dev, (232, 232, 232), 2, 5, 0.05, False, 13)
+    _both(dev, (232, 232, 232), 2, 5, 0.05, True, 14)
 
 
 def test_noalloc_refusals(dev):
