@@ -54,14 +54,19 @@ WORKLOADS = {
 }
 
 
-def _traffic_from_profiles(name):
-    """HBM bytes per launch measured by rocprofv3 PMC runs, if recorded."""
+def _traffic_from_profiles(name, shape=None):
+    """HBM bytes per launch measured by rocprofv3 PMC runs, if recorded: one
+    figure per workload, or one per launch shape of its tile family (the
+    shapes give the same bits but differ in register spills, so in traffic)."""
     p = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(p) as f:
-            return json.load(f).get(name)
+            t = json.load(f).get(name)
     except (OSError, ValueError):
         return None
+    if isinstance(t, dict):
+        return t.get(str(shape))
+    return t
 
 
 def cpu_sample(nx, ny, nz, rp, rs, planes=0, budget_s=4.0):
@@ -484,15 +489,19 @@ def main():
     flop = float(nvox) * pairs * fpp
     achieved = flop / (ms_kernel * 1e-3) / 1e12
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": _traffic_from_profiles(args.workload),
+                "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
                 "kernel_ms": round(ms_kernel, 4), "flop_per_launch": flop}
     if nz > 1 and wl.get("u8") and gnlm.forward_u8_route(nx, ny, nz, rp, rs)[1] == gnlm.KERNEL_U8_INTEGER:
         roofline["kernel"] = "nlm3d_fwd_u8 (8-bit samples, integer patch distances)"
+        roofline["traffic"] = _traffic_from_profiles(args.workload)
     elif nz > 1:
         # the tile variant and the launch shape the first call's tuning chose
         shape, nshapes = gnlm.forward_shape((nz, ny, nx), rp, rs)
         roofline["variant"] = gnlm.tuned_variant(rp, rs)
         roofline["launch_shape"] = {"chosen": shape, "candidates": nshapes}
+        roofline["traffic"] = _traffic_from_profiles(args.workload, shape)
+    else:
+        roofline["traffic"] = _traffic_from_profiles(args.workload)
 
     cpu = None
     parity = None

@@ -47,3 +47,15 @@ def test_window_equals_whole_volume_result():
     got64 = oracle.nlm3d_separable_f64(crop, rp, rs, h, z0=z0 - cz0, z1=z0 - cz0 + n)[win]
     ref64 = oracle.nlm3d_separable_f64(u, rp, rs, h, z0=z0, z1=z0 + n)[:, y0:y0 + wy, x0:x0 + wx]
     assert np.array_equal(got64, ref64)
+
+
+def test_traffic_figures_per_launch_shape():
+    """The headline's byte counters are recorded per launch shape (the shapes
+    give the same bits but not the same register spills); the figure the bench
+    line carries is the chosen shape's, and an unknown shape or workload gives
+    None rather than another shape's figure."""
+    figs = [bench._traffic_from_profiles("nlm3d_256_p5_s11", c) for c in range(2)]
+    assert all(isinstance(f, int) and f > 1.342e8 for f in figs), figs
+    assert bench._traffic_from_profiles("nlm3d_256_p5_s11", 7) is None
+    assert bench._traffic_from_profiles("no such workload") is None
+    assert isinstance(bench._traffic_from_profiles("nlm3d_512x512x128_p17_s13_u8"), int)
