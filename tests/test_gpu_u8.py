@@ -145,3 +145,41 @@ def test_u8_rp7_rate_matches_rp8(dev):
     ms8 = _median_ms(lambda: gnlm.nlm3d_u8(t8, 8, 3, 10.0, out=out))
     print(f"u8 256x256x64 r_s 3: r_p 7 {ms7:.3f} ms, r_p 8 {ms8:.3f} ms", flush=True)
     assert ms7 < 1.6 * ms8, (ms7, ms8)
+
+
+def test_u8_reference_default_full_size(dev):
+    """The reference's default configuration at its full size: 512 x 512 x
+    128 8-bit samples, NS 17 / SS 13, PARAMETER 10 (3DNLM/config.h:4-9).
+    Checked two ways, each within 1e-5 relative:
+      * the whole volume against the FP32 large-patch tiles run on the float
+        values of the same samples (the same definition computed with
+        rounded patch sums: a size-independent consistency check; on this
+        phantom every patch sum stays below 2^24, and the two gave the same
+        bits);
+      * one plane of a centred 48 x 48 window against the oracle's FP64
+        separable path, run on the window grown by R = r_p + r_s (its
+        outputs see exactly the samples the whole volume gives them)."""
+    import sys, os
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    import bench
+    nx, ny, nz, rp, rs, h = 512, 512, 128, 8, 6, 10.0
+    u, _ = gnlm.phantom3d(nx, ny, nz, n_ellipsoids=64, axis_lo=5.0, axis_hi=40.0, background=0.05, sigma=0.05,
+                          seed=20261210)
+    img = np.clip(np.rint(u * 255.0), 0, 255).astype(np.uint8)
+    del u
+    print("case u8 full size 512x512x128 rp=8 rs=6", flush=True)
+    assert gnlm.forward_u8_route(nx, ny, nz, rp, rs) == (gnlm.OK, gnlm.KERNEL_U8_INTEGER)
+    y8 = gnlm.nlm3d_u8(_u8(dev, img), rp, rs, h).cpu().numpy()
+    yf = gnlm.nlm3d(torch.from_numpy(img.astype(np.float32)).to(dev), rp, rs, h).cpu().numpy()
+    torch.cuda.synchronize()
+    print("u8 vs FP32 tiles", report(y8, yf))
+    assert max_rel(y8, yf) <= TOL, report(y8, yf)
+    del yf
+    (z0, n, y0, wy, x0, wx), (cz0, cz1, cy0, cy1, cx0, cx1) = bench.cpu_sample(nx, ny, nz, rp, rs, budget_s=1.5)
+    assert n == 1 and wx == wy and wx >= 32
+    crop = img[cz0:cz1, cy0:cy1, cx0:cx1].astype(np.float32)
+    ref = oracle.nlm3d_separable_f64(crop, rp, rs, h, z0 - cz0, z0 - cz0 + 1)
+    ref = ref[:, y0 - cy0:y0 - cy0 + wy, x0 - cx0:x0 - cx0 + wx]
+    got = y8[z0:z0 + 1, y0:y0 + wy, x0:x0 + wx]
+    print(f"window {wx}x{wy} of plane {z0} vs FP64", report(got, ref))
+    assert max_rel(got, ref) <= TOL, report(got, ref)
