@@ -116,7 +116,7 @@ int device_cus();
 hipError_t bwd3d_reduce_dh_launch(const float *part, int n, float scale, float *out, hipStream_t s);
 
 // ---- 8-bit forward with integer patch distances (nlm3d_u8.hip) -------------
-// r_p 7-8 with r_s 1-7; the input is the caller's 8-bit volume (in_pad = 0:
+// r_p 6-8 with r_s 1-7; the input is the caller's 8-bit volume (in_pad = 0:
 // unpadded, reflect-101 on device; in_pad >= r_p + r_s: padded with strides
 // sy, sz in elements).  fwd3d_u8_fits: the radii are instantiated and the
 // volume's copy and records fit 32-bit byte offsets.

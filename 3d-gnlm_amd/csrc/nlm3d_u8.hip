@@ -2,7 +2,7 @@
 // distances: the reference's own input type (3DNLM/main.cu:7620 copies an
 // unsigned char volume; its kernels difference the samples as ints before
 // squaring, 3DNLM/main.cu:59) at the large patches of its default
-// configuration and sweeps (NS 15-17, 3DNLM/config.h:4-5, 3DNLM/run.sh:25-34).
+// configuration and sweeps (NS 13-17, 3DNLM/config.h:4-5, 3DNLM/run.sh:25-34).
 //
 // Same definition as every other forward (3DNLM/main.cu:52-68 patch SSD over
 // the full offset cube, :107-111 weights, :125 normalisation).  For 8-bit
@@ -18,9 +18,10 @@
 // each box is a prefix or sliding sum, which integers allow without error.
 // Only the weight is floating point: w = exp2(float(S) * k).
 //
-// Tiles: 16 x (32 - 2 r_p) x (32 - 2 r_p) voxels on 1024 threads, the grown
-// rows forming a 32 x 32 grid with one row per thread (the geometry of the
-// float large-patch tiles):
+// Tiles: 16 x (32 - 2 r_p) x (32 - 2 r_p) voxels on 1024 threads for r_p 7-8,
+// the grown rows forming a 32 x 32 grid with one row per thread (the geometry
+// of the float large-patch tiles); 16 x 16 x 16 for r_p 6, on 28 x 28 grown
+// rows in the same thread grid:
 //   * a pre-pass copies the volume with its reflect-101 border into an 8-bit
 //     buffer whose rows start on 16-byte boundaries (x origin 16 >= r_p + r_s);
 //     a row's 48 bytes around the tile come in three 128-bit loads;
@@ -30,7 +31,7 @@
 //     compile-time register indices;
 //   * phase 1: prefix sums of the products along the row give the x boxes of
 //     -2 u(x) u(x + d) -> LDS; phase 2: sliding y boxes of (x, z_ext)
-//     columns -> LDS (for r_p 7-8 each wave boxes the two planes of rows it
+//     columns -> LDS (for r_p 6-8 each wave boxes the two planes of rows it
 //     wrote itself, into one of two buffers by the offset's parity, so one
 //     barrier per offset suffices); phase 3: (x, y, run of RZ z) items,
 //     sliding z boxes read with 128/64-bit accesses (z-fastest layout), then
@@ -50,18 +51,26 @@ namespace {
 template <int RP>
 constexpr int u8_rz()
 {
-    return RP == 8 ? 4 : (RP == 7 ? 6 : (RP == 6 ? 10 : (RP == 5 ? 11 : 12)));
+    return RP == 7 ? 6 : 4;
 }
+
+// y and z extent of the grown tile: 32 rows (16 x (32 - 2 r_p)^2 tiles) for
+// r_p 7-8; 16 + 2 r_p for smaller patches, whose tiles stay 16 x 16 x 16 so
+// that the phase-3 state (z runs of 4) fits the registers
+constexpr int u8_ext(int rp) { return rp >= 7 ? 32 : 16 + 2 * rp; }
 
 template <int RP_, int RS_>
 struct U8Cfg {
     static constexpr int RP = RP_, RS = RS_;
     static constexpr int W = 2 * RP + 1, R = RP + RS;
-    static constexpr int NT = 1024, TX = 16, EY = 32, EZ = 32;
+    static constexpr int NT = 1024, TX = 16, EY = u8_ext(RP), EZ = EY;
+    // thread -> phase-1 row: 32 row slots per plane, 32 planes (lane =
+    // y_ext + 32 z_ext); slots past EY, EZ hold rows no output reads
+    static constexpr int EYP = 32, EZP = 32;
     static constexpr int TY = EY - 2 * RP, TZ = EZ - 2 * RP;
     static constexpr int EX = TX + 2 * RP;   // centre samples per row
     static constexpr int LPB = 16;           // x origin of the 8-bit copy
-    static_assert(RP >= 4 && RP <= 8 && RS >= 1 && R <= LPB, "32 - 2 r_p rows, window inside the 48 loaded bytes");
+    static_assert(RP >= 4 && RP <= 8 && RS >= 1 && R <= LPB, "grown rows within 32, window inside the 48 loaded bytes");
     static constexpr int NW = TX + 2 * R;    // shifted-row samples per (dy, dz)
     static constexpr int CB0 = LPB - RP;     // byte of centre sample 0 in the 48 loaded bytes
     static constexpr int WB0 = LPB - R;      // byte of window sample 0
@@ -75,7 +84,7 @@ struct U8Cfg {
     // widest with which s_x and two s_xy buffers still fit 160 KiB: the
     // two-barrier form holds more registers live across the offset loop
     // than 128 VGPRs and spilled about 400 of them, 10x slower.
-    static constexpr int SXR = 20, SXP = SXR * EY + 16, SXN = SXP * EZ;
+    static constexpr int SXR = 20, SXP = SXR * EYP + 16, SXN = SXP * EZP;
     static constexpr int SYX = RP == 7 ? 34 : 36, SYY = 16 * SYX + 4, SYN = SYY * TY;
     // WL ("wave-local"): where s_x and two s_xy buffers fit the LDS, each wave
     // computes the y boxes of the two z_ext planes of rows it wrote itself in
@@ -84,13 +93,13 @@ struct U8Cfg {
     // orders everything; otherwise waves 0-7 run phase 2 on whole columns
     // between two barriers.
     static constexpr bool WL = (SXN + 2 * SYN) * 4 <= 160 * 1024;
-    static_assert(WL || RP < 7, "r_p 7-8 run the wave-local form");
+    static_assert(WL || RP < 6, "r_p 6-8 run the wave-local form");
     // phase 2 items: (x, z_ext) columns, whole (NP2 = 1) or, wave-local, in
     // two runs of RY outputs (64 items per wave: 16 x, 2 planes, 2 runs)
     static constexpr int NP2 = WL ? 2 : 1;
     static_assert(TY % NP2 == 0, "whole y runs");
     static constexpr int RY = TY / NP2;
-    static constexpr int P2N = TX * EZ * NP2;
+    static constexpr int P2N = TX * (WL ? EZP : EZ) * NP2;
     static constexpr int RZ = u8_rz<RP>();
     static_assert(TZ % RZ == 0, "whole z runs");
     static constexpr int P3N = TX * TY * (TZ / RZ);
@@ -122,7 +131,7 @@ struct U8Args {
     U8Vol v;
     RecVol rc;
     float *out;   // y
-    float *Zacc;  // Z (the normaliser)
+    float *Zacc;  // Z (the normaliser), or NULL for no Z
     int nx, ny, nz;
     float k;
     int ntx, nty, ntz;
@@ -248,9 +257,13 @@ __device__ __forceinline__ void u8_tile(const U8Args &a, int x0, int y0, int z0,
     const int tzv = min(C::TZ, a.nz - z0);
 
     // ---- phase-1 row (y_ext = tid % 32, z_ext = tid / 32) ----
-    const int yp = tid % C::EY, zp = tid / C::EY;
+    const int yp = tid % C::EYP, zp = tid / C::EYP;
     // z_ext grows with the lane index: the first lane decides for the wave
     const bool p1ok = __builtin_amdgcn_readfirstlane(zp < tzv + 2 * RP);
+    // (EZ < 32: the waves whose two planes lie past the grown tile run phases
+    // 1 and 2 on rows no output reads, into s_x and s_xy slots no one reads;
+    // a wave-uniform branch around those phases made the compiler spill
+    // 70-320 VGPRs)
     // bytes x0 - 16 .. x0 + 32 of the row at (dy, dz) = (-RS, -RS)
     const unsigned rbase = (unsigned)((z0 - RP + zp + C::R - RS) * pxy + (y0 - RP + yp + C::R - RS) * px + x0);
     const int sxw = zp * C::SXP + yp * C::SXR;
@@ -376,7 +389,7 @@ __device__ __forceinline__ void u8_tile(const U8Args &a, int x0, int y0, int z0,
         for (int o = 0; o < RZ; ++o) {
             if (gz + o >= a.nz) continue;
             const int oi = gx + a.nx * (gy + a.ny * (gz + o));
-            Zacc[oi] = Zt[o];
+            if (Zacc) Zacc[oi] = Zt[o];
             out[oi] = Yt[o] / Zt[o];
         }
     }
@@ -499,7 +512,7 @@ struct U8Geom {
 U8Geom u8_geom(int nx, int ny, int nz, int rp, int rs)
 {
     U8Geom g;
-    const int TY = 32 - 2 * rp, TZ = TY, R = rp + rs;
+    const int TY = u8_ext(rp) - 2 * rp, TZ = TY, R = rp + rs;
     g.ntx = (nx + 15) / 16;
     g.nty = (ny + TY - 1) / TY;
     g.ntz = (nz + TZ - 1) / TZ;
@@ -524,11 +537,10 @@ hipError_t u8_launch_t(const InU8 &in, float *out, float *Zout, float k, hipStre
 {
     using C = U8Cfg<RP, RS>;
     const U8Geom g = u8_geom(in.nx, in.ny, in.nz, RP, RS);
-    const long long nv = (long long)in.nx * in.ny * in.nz;
     auto al = [](long long b) { return (b + 255) / 256 * 256; };
     const long long off_x1 = al(g.copy_bytes), off_x2 = off_x1 + al(4 * g.x1_ints),
-                    off_rec = off_x2 + al(4 * g.x2_ints), off_z = off_rec + al(g.rec_bytes);
-    const long long bytes = off_z + (Zout ? 0 : 4 * nv);
+                    off_rec = off_x2 + al(4 * g.x2_ints);
+    const long long bytes = off_rec + g.rec_bytes;
     char *ws = nullptr;
     hipError_t e = hipMallocAsync((void **)&ws, (size_t)bytes, s);
     if (e != hipSuccess) return e;
@@ -539,7 +551,7 @@ hipError_t u8_launch_t(const InU8 &in, float *out, float *Zout, float k, hipStre
     a.v = U8Vol{cp, g.px, g.px * g.py, (unsigned)g.copy_bytes};
     a.rc = RecVol{(const int *)rec, g.rx, g.rx * g.ry, (unsigned)g.rec_bytes};
     a.out = out;
-    a.Zacc = Zout ? Zout : (float *)(ws + off_z);
+    a.Zacc = Zout;  // NULL: y alone (the totals live in registers)
     a.nx = in.nx;
     a.ny = in.ny;
     a.nz = in.nz;
@@ -572,14 +584,14 @@ hipError_t u8_launch_t(const InU8 &in, float *out, float *Zout, float k, hipStre
     return e != hipSuccess ? e : e2;
 }
 
-// Instantiated radii: r_p 7 and 8 (NS 15, 17) with r_s 1-7.  For r_p 4-6 the
-// tiles grow to 16 x 20^2 .. 16 x 24^2 voxels on the same 1024 rows and the
-// phase-3 state no longer fits the registers; there the FP32 kernels (on a
-// float copy) ran in 0.67-1.03x of its time on MI355X (512^2 x 64, r_p 4-6,
-// r_s 3-6), and gnlm3d_forward_u8 uses them.
+// Instantiated radii: r_p 6-8 (NS 13-17, the reference's default and sweeps)
+// with r_s 1-7.  r_p 6 runs 16 x 16 x 16 tiles on 28 x 28 grown rows (two of
+// the 16 waves idle in phases 1 and 2).  For r_p 4-5 the FP32 kernels on a
+// float copy serve the call.
 #define GNLM_U8_CASES(X)                                                                \
-    X(7, 1) X(7, 2) X(7, 3) X(7, 4) X(7, 5) X(7, 6) X(7, 7) X(8, 1) X(8, 2) X(8, 3)     \
-    X(8, 4) X(8, 5) X(8, 6) X(8, 7)
+    X(6, 1) X(6, 2) X(6, 3) X(6, 4) X(6, 5) X(6, 6) X(6, 7) X(7, 1) X(7, 2) X(7, 3)     \
+    X(7, 4) X(7, 5) X(7, 6) X(7, 7) X(8, 1) X(8, 2) X(8, 3) X(8, 4) X(8, 5) X(8, 6)     \
+    X(8, 7)
 
 }  // namespace
 
