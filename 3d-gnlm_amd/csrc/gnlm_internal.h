@@ -116,9 +116,9 @@ int device_cus();
 hipError_t bwd3d_reduce_dh_launch(const float *part, int n, float scale, float *out, hipStream_t s);
 
 // ---- 8-bit forward with integer patch distances (nlm3d_u8.hip) -------------
-// r_p 6-8 with r_s 1-7; the input is the caller's 8-bit volume (in_pad = 0:
-// unpadded, reflect-101 on device; in_pad >= r_p + r_s: padded with strides
-// sy, sz in elements).  fwd3d_u8_fits: the radii are instantiated and the
+// r_p 6-8 with r_s 1-7 and r_p 5 with r_s 3-7; the input is the caller's
+// 8-bit volume (in_pad = 0: unpadded, reflect-101 on device; in_pad >= r_p +
+// r_s: padded with strides sy, sz in elements).  fwd3d_u8_fits: the radii are instantiated and the
 // volume's copy and records fit 32-bit byte offsets.
 bool fwd3d_u8_supported(int r_p, int r_s);
 bool fwd3d_u8_fits(int nx, int ny, int nz, int r_p, int r_s);

@@ -585,11 +585,14 @@ hipError_t u8_launch_t(const InU8 &in, float *out, float *Zout, float k, hipStre
 }
 
 // Instantiated radii: r_p 6-8 (NS 13-17, the reference's default and sweeps)
-// with r_s 1-7.  r_p 6 runs 16 x 16 x 16 tiles on 28 x 28 grown rows (two of
-// the 16 waves idle in phases 1 and 2).  For r_p 4-5 the FP32 kernels on a
-// float copy serve the call.
+// with r_s 1-7, and r_p 5 (NS 11) with r_s 3-7.  r_p 5-6 run 16 x 16 x 16
+// tiles on 26^2 / 28^2 grown rows (the waves past the grown planes run phases
+// 1 and 2 on rows no output reads).  Measured against the FP32 kernels on the
+// same samples (512^2 x 128): r_p 5 r_s 3 / 5 14.5 / 51.7 against 15.2 /
+// 64.9 ms; r_p 4 r_s 3 / 5 14.1 / 50.6 against 13.4 / 50.9 ms, so r_p 4 and
+// r_p 5 with r_s 1-2 stay on the FP32 kernels (on a float copy).
 #define GNLM_U8_CASES(X)                                                                \
-    X(6, 1) X(6, 2) X(6, 3) X(6, 4) X(6, 5) X(6, 6) X(6, 7) X(7, 1) X(7, 2) X(7, 3)     \
+    X(5, 3) X(5, 4) X(5, 5) X(5, 6) X(5, 7) X(6, 1) X(6, 2) X(6, 3) X(6, 4) X(6, 5) X(6, 6) X(6, 7) X(7, 1) X(7, 2) X(7, 3)     \
     X(7, 4) X(7, 5) X(7, 6) X(7, 7) X(8, 1) X(8, 2) X(8, 3) X(8, 4) X(8, 5) X(8, 6)     \
     X(8, 7)
 

@@ -271,9 +271,10 @@ int gnlm_u8_to_f32(const unsigned char *d_in, float *d_out, long long n, void *s
  * kernel launches of 3DNLM/main.cu:7622-7667 for 8-bit data.  Layout and
  * arguments as gnlm3d_forward_ex (in_pad, in_sy, in_sz in bytes = elements;
  * h on the samples' 0-255 scale, the reference's PARAMETER); d_out and d_Z
- * are FP32.  For r_p 6-8 (NS 13-17) with r_s 1-7 the patch distances are
- * formed exactly in 32-bit integers (an 8-bit patch SSD is an integer below
- * 2^31 for NS <= 17) and only the weights are floating point; other radii,
+ * are FP32.  For r_p 6-8 (NS 13-17) with r_s 1-7, and r_p 5 (NS 11) with
+ * r_s 3-7, the patch distances are formed exactly in 32-bit integers (an
+ * 8-bit patch SSD is an integer below 2^31 for NS <= 17) and only the
+ * weights are floating point; other radii,
  * and volumes past that kernel's 32-bit offsets (about 800^3), run the FP32
  * kernels on a float copy made in a stream-ordered workspace.  On 8-bit
  * samples both give the same definition to FP32 rounding. */

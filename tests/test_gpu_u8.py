@@ -1,8 +1,8 @@
 """GPU parity of the 8-bit forward (gnlm3d_forward_u8) against the CPU oracle.
 
-For r_p 6-8 the 8-bit kernel forms every patch distance exactly in 32-bit
-integers (S = A(x) + A(x+d) - 2 C_d(x)); only the weights and their sums are
-FP32.  Bars, stated per comparison:
+For r_p 6-8 (and r_p 5 with r_s >= 3) the 8-bit kernel forms every patch
+distance exactly in 32-bit integers (S = A(x) + A(x+d) - 2 C_d(x)); only the
+weights and their sums are FP32.  Bars, stated per comparison:
   * vs the oracle's FP64 separable path and its FP32 path, on the float
     values of the same 8-bit samples: max relative error
     |a-b|/max(|b|,1e-6) <= 1e-5 (outputs are weighted means of samples in
@@ -10,8 +10,8 @@ FP32.  Bars, stated per comparison:
   * the normaliser Z vs FP64: <= 1e-5 relative;
   * a caller-padded input (the reference's layout), the host-pointer drop-in
     and the unpadded device entry point: bit-identical (one kernel, one copy);
-  * radii outside r_p 6-8 run the FP32 kernels on a float copy: bit-identical
-    to gnlm.nlm3d on the converted volume.
+  * other radii run the FP32 kernels on a float copy: bit-identical to
+    gnlm.nlm3d on the converted volume.
 """
 import numpy as np
 import pytest
@@ -53,6 +53,8 @@ U8_CASES = [
     ((40, 36, 33), 6, 6, 60.0, "uniform"),   # 16^3 tiles on 28^2 grown rows, partial tiles
     ((17, 35, 18), 6, 7, 70.0, "uniform"),
     ((9, 10, 8), 6, 1, 90.0, "uniform"),
+    ((24, 22, 21), 5, 3, 10.0, "phantom"),   # NS = 11: 16^3 tiles on 26^2 grown rows
+    ((33, 20, 40), 5, 7, 60.0, "uniform"),
 ]
 
 
@@ -95,10 +97,10 @@ def test_u8_padded_and_dropin_equal_unpadded(dev):
     assert np.array_equal(a, b.reshape(a.shape)), report(b.reshape(a.shape), a)
 
 
-@pytest.mark.parametrize("rp,rs", [(1, 3), (2, 5), (3, 2), (4, 3), (5, 4), (9, 1)])
+@pytest.mark.parametrize("rp,rs", [(1, 3), (2, 5), (3, 2), (4, 3), (5, 2), (9, 1)])
 def test_u8_other_radii_run_float_kernels(dev, rp, rs):
-    """Radii outside r_p 6-8 run the FP32 kernels on a float copy: the same
-    bits as gnlm.nlm3d on the converted volume."""
+    """Radii the integer kernel does not take run the FP32 kernels on a float
+    copy: the same bits as gnlm.nlm3d on the converted volume."""
     rng = np.random.default_rng(7 + rp)
     img = rng.integers(0, 256, (13, 17, 21), dtype=np.uint8)
     assert gnlm.forward_u8_route(21, 17, 13, rp, rs)[1] != gnlm.KERNEL_U8_INTEGER

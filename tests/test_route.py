@@ -97,7 +97,7 @@ def test_route_rejects_bad_geometry(args):
 
 def _u8_record_bytes(n_xy, n_z, rp, rs):
     """Size of the 8-byte (A, u) records the 8-bit forward builds: the volume
-    rounded up to 16 x (32 - 2 r_p)^2 tiles (r_p 7-8; 16^3 tiles at r_p 6)
+    rounded up to 16 x (32 - 2 r_p)^2 tiles (r_p 7-8; 16^3 tiles at r_p 5-6)
     and grown by r_s per side."""
     t = 32 - 2 * rp if rp >= 7 else 16
     rx = -(-n_xy // 16) * 16 + 2 * rs
@@ -108,12 +108,13 @@ def _u8_record_bytes(n_xy, n_z, rp, rs):
 
 def test_u8_route():
     """gnlm3d_forward_u8: integer patch distances for r_p 6-8 (the reference's
-    NS 13-17) while the records fit 32-bit byte offsets; otherwise the FP32
-    route on a float copy, with its statuses."""
+    NS 13-17) and r_p 5 with r_s >= 3, while the records fit 32-bit byte
+    offsets; otherwise the FP32 route on a float copy, with its statuses."""
     assert gnlm.forward_u8_route(512, 512, 128, 8, 6) == (gnlm.OK, gnlm.KERNEL_U8_INTEGER)
     assert gnlm.forward_u8_route(512, 512, 16, 7, 5) == (gnlm.OK, gnlm.KERNEL_U8_INTEGER)
     assert gnlm.forward_u8_route(512, 512, 64, 6, 6) == (gnlm.OK, gnlm.KERNEL_U8_INTEGER)
-    for rp, rs in ((1, 3), (2, 5), (4, 3), (5, 4), (6, 8), (8, 8), (9, 2)):
+    assert gnlm.forward_u8_route(512, 512, 64, 5, 3) == (gnlm.OK, gnlm.KERNEL_U8_INTEGER)
+    for rp, rs in ((1, 3), (2, 5), (4, 3), (4, 5), (5, 2), (6, 8), (8, 8), (9, 2)):
         assert gnlm.forward_u8_route(64, 64, 64, rp, rs) == gnlm.forward_route(64, 64, 64, rp, rs), (rp, rs)
     # 780^3 at NS 17 / SS 13: records below 4 GiB; 820^3: past them
     assert _u8_record_bytes(780, 780, 8, 6) < 2 ** 32 <= _u8_record_bytes(820, 820, 8, 6)
