@@ -82,14 +82,16 @@ def test_u8_matches_oracle(dev, shape, rp, rs, h, kind):
     assert max_rel(Z, Z64) <= TOL, report(Z, Z64)
 
 
-def test_u8_padded_and_dropin_equal_unpadded(dev):
+@pytest.mark.parametrize("rp,rs", [(8, 6), (6, 5), (5, 3)])
+def test_u8_padded_and_dropin_equal_unpadded(dev, rp, rs):
     """The reference's padded 8-bit layout, through the device entry point
     and through the host-pointer drop-in (gnlm_nl_means_3d_u8), gives the
-    same bits as the unpadded input with reflect-101 on device."""
-    rng = np.random.default_rng(21)
+    same bits as the unpadded input with reflect-101 on device (both tile
+    geometries: 16 x (32 - 2 r_p)^2 and 16^3 on smaller grown rows)."""
+    rng = np.random.default_rng(21 + rp)
     nz, ny, nx = 19, 26, 35
     img = rng.integers(0, 256, (nz, ny, nx), dtype=np.uint8)
-    rp, rs, h = 8, 6, 10.0
+    h = 10.0
     R = rp + rs
     padded = np.pad(img, R, mode="reflect")
     a = gnlm.nlm3d_u8(_u8(dev, img), rp, rs, h).cpu().numpy()
