@@ -14,6 +14,8 @@ fails at least one of them:
   * constant input                returns the constant
   * x-step volume, r_p = r_s = 1  closed form y = (a(1+q) + q b)/(1+2q),
                                      q = exp(-(b-a)^2/(3 h^2)) at the step
+  * single impulse, r_p = r_s = 1 closed forms at the impulse and at its
+                                     x neighbours (worked by hand below)
   * z-invariant volume            3D output equals the 2D output per plane
   * flips / axis transposes       output flips / transposes with the input
   * convex combination            min u <= y <= max u
@@ -87,6 +89,32 @@ def test_step_closed_form():
     assert np.allclose(y[:, :, m], want_right, rtol=1e-13)
     assert np.allclose(y[:, :, 2:m - 1], a, rtol=1e-13)
     assert np.allclose(y[:, :, m + 1:10], b, rtol=1e-13)
+
+
+def test_impulse_closed_form():
+    """A single impulse c at x0 in a zero 9^3 volume, patch and search radius
+    1 (P = 27), worked by hand:
+      * at x0 every d != 0 has S_d = 2 c^2 (the impulse enters the patch
+        difference twice, at p = 0 and p = -d), so y = c / (1 + 26 w2);
+      * at x1 = x0 + e (e = +x or -x) only d = -e reaches the impulse, with
+        S = 2 c^2; of the other offsets, the 17 with d.x != e.x have S =
+        2 c^2 and the 9 with d.x = e.x have S = c^2 (the second copy of the
+        impulse falls outside the patch), so y = c w2 / (1 + 17 w2 + 9 w1);
+      * two or more voxels from the impulse (outside every search window that
+        holds it), y = 0 exactly;
+    with w_k = exp(-k c^2 / (27 h^2))."""
+    c, h = 0.8, 0.35
+    u = np.zeros((9, 9, 9))
+    u[4, 4, 4] = c
+    y = oracle.nlm3d_brute(u, 1, 1, h)
+    w1 = np.exp(-c * c / (27 * h * h))
+    w2 = np.exp(-2 * c * c / (27 * h * h))
+    assert abs(y[4, 4, 4] - c / (1 + 26 * w2)) <= 1e-14
+    for x in (3, 5):
+        assert abs(y[4, 4, x] - c * w2 / (1 + 17 * w2 + 9 * w1)) <= 1e-14, x
+    assert y[4, 4, 6] == 0.0 and y[4, 4, 2] == 0.0 and y[4, 6, 4] == 0.0 and y[6, 4, 4] == 0.0
+    y32 = oracle.nlm3d_brute(u.astype(np.float32), 1, 1, h)
+    assert abs(y32[4, 4, 4] - y[4, 4, 4]) <= 1e-6 * y[4, 4, 4]
 
 
 def test_z_invariant_3d_equals_2d():
