@@ -81,8 +81,8 @@ int gnlm3d_forward_ex(const float *d_in, int in_pad, long long in_sy, long long 
  * copy of the input; for the large-patch tiles of r_p 5-8 also the running Z
  * total when d_Z is NULL) comes from a fixed device array of
  * GNLM_NOALLOC_SCRATCH_BYTES that each device the library runs on holds
- * until the process ends: about 240^3 voxels at patch 5^3 / search 11^3.  GNLM_ENOMEM (before anything is
- * launched) for a volume that needs more; GNLM_EUNSUPPORTED where the route
+ * until the process ends: about 240^3 voxels at patch 5^3 / search 11^3.
+ * GNLM_ENOMEM (before anything is launched) for a volume that needs more; GNLM_EUNSUPPORTED where the route
  * is the generic per-offset path, which allocates.  All calls on one device
  * share its array, so calls that overlap on one device (other streams, other
  * host threads) race: serialise them. */
