@@ -20,8 +20,8 @@
 //
 // Tiles: 16 x (32 - 2 r_p) x (32 - 2 r_p) voxels on 1024 threads for r_p 7-8,
 // the grown rows forming a 32 x 32 grid with one row per thread (the geometry
-// of the float large-patch tiles); 16 x 16 x 16 for r_p 6, on 28 x 28 grown
-// rows in the same thread grid:
+// of the float large-patch tiles); 16 x 16 x 16 for r_p 5-6, on (16 + 2 r_p)^2
+// grown rows in the same thread grid:
 //   * a pre-pass copies the volume with its reflect-101 border into an 8-bit
 //     buffer whose rows start on 16-byte boundaries (x origin 16 >= r_p + r_s);
 //     a row's 48 bytes around the tile come in three 128-bit loads;
@@ -31,7 +31,7 @@
 //     compile-time register indices;
 //   * phase 1: prefix sums of the products along the row give the x boxes of
 //     -2 u(x) u(x + d) -> LDS; phase 2: sliding y boxes of (x, z_ext)
-//     columns -> LDS (for r_p 6-8 each wave boxes the two planes of rows it
+//     columns -> LDS (for r_p 5-8 each wave boxes the two planes of rows it
 //     wrote itself, into one of two buffers by the offset's parity, so one
 //     barrier per offset suffices); phase 3: (x, y, run of RZ z) items,
 //     sliding z boxes read with 128/64-bit accesses (z-fastest layout), then
@@ -93,7 +93,7 @@ struct U8Cfg {
     // orders everything; otherwise waves 0-7 run phase 2 on whole columns
     // between two barriers.
     static constexpr bool WL = (SXN + 2 * SYN) * 4 <= 160 * 1024;
-    static_assert(WL || RP < 6, "r_p 6-8 run the wave-local form");
+    static_assert(WL || RP < 5, "r_p 5-8 run the wave-local form");
     // phase 2 items: (x, z_ext) columns, whole (NP2 = 1) or, wave-local, in
     // two runs of RY outputs (64 items per wave: 16 x, 2 planes, 2 runs)
     static constexpr int NP2 = WL ? 2 : 1;
