@@ -15,6 +15,12 @@ forward is N * (2 r_s + 1)^3 * 13 flop; achieved = that / kernel time,
 peak = 157.3 TFLOP/s (MI355X FP32 vector).  HBM traffic per launch comes
 from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs kept in profiles/.
 
+The default line also carries, as objects of their own: the batched
+persistent launch of 64 uneven volumes against per-volume launches
+("batched"), fwd+bwd against fwd on 8 x 128^3 ("grad"), the reference's own
+default configuration on its 8-bit data type (512 x 512 x 128, NS 17 /
+SS 13: "reference_default") and its run.sh sweep ("sweep").
+
 cpu_baseline: the CPU FP32 path of the oracle (separable, OpenMP), timed on
 this host's cores on a bounded sample of the same volume (a slab of z-planes,
 or for large patches one x-y window of a plane); the sample's outputs are also
@@ -166,6 +172,52 @@ def grad_summary(r):
     return {"workload": "nlm3d_grad_8x128", "ratio": r["value"], "fwd_ms": r["fwd_ms"],
             "fwd_bwd_ms": r["fwd_bwd_ms"], "steps": r["steps"], "warmup": r["warmup"], "fit_h": r["fit_h"],
             "target": "fwd+bwd <= 3x fwd"}
+
+
+def ref_default_summary(steps=3, warmup=1):
+    """The reference's own default configuration (3DNLM/config.h:4-9: 512 x
+    512 x 128, SS 13, NS 17, PARAMETER 10) on its data type, 8-bit samples
+    resident on the device, through gnlm3d_forward_u8 -- the object the
+    headline line carries for it (kernel time by HIP events, no CPU work;
+    parity at this size is a GPU test)."""
+    import torch
+    import gnlm
+    wl = WORKLOADS["nlm3d_512x512x128_p17_s13_u8"]
+    nx, ny, nz = wl["dims"]
+    rp, rs, h = wl["rp"], wl["rs"], wl["h"]
+    dev = torch.device("cuda:0")
+    u, _ = gnlm.phantom3d(nx, ny, nz, n_ellipsoids=wl["n_ell"], axis_lo=wl["ax"][0], axis_hi=wl["ax"][1],
+                          background=0.05, sigma=wl["sigma"], seed=wl["seed"])
+    u8 = torch.from_numpy(np.clip(np.rint(u * 255.0), 0, 255).astype(np.uint8)).to(dev)
+    out = torch.empty(u8.shape, dtype=torch.float32, device=dev)
+    for _ in range(warmup):
+        gnlm.nlm3d_u8(u8, rp, rs, h, out=out)
+    stream = torch.cuda.current_stream(dev)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(steps):
+        gnlm.nlm3d_u8(u8, rp, rs, h, out=out)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    nvox = nx * ny * nz
+    tflops = nvox * (2 * rs + 1) ** 3 * FLOP_PER_PAIR_3D / (ms * 1e-3) / 1e12
+    integer = gnlm.forward_u8_route(nx, ny, nz, rp, rs)[1] == gnlm.KERNEL_U8_INTEGER
+    return {"workload": "nlm3d_512x512x128_p17_s13_u8", "ms": round(ms, 3),
+            "Mvoxels_s": round(nvox / ms / 1e3, 1),
+            "kernel": "nlm3d_fwd_u8 (integer patch distances)" if integer else "FP32 kernels on a float copy",
+            "achieved_TFLOPs": round(tflops, 3), "frac": round(tflops / PEAK_FP32_TFLOPS, 4),
+            "steps": steps, "warmup": warmup, "traffic": _traffic_from_profiles("nlm3d_512x512x128_p17_s13_u8")}
+
+
+def sweep_summary(r):
+    """The run.sh sweep object carried on the headline line."""
+    gp = [p["Gpairs_s"] for p in r["points"]]
+    return {"workload": "nlm3d_sweep_512", "points": len(r["points"]), "total_ms": r["ms_per_step"],
+            "Mvoxels_s": r["value"], "Gpairs_s_min": min(gp), "Gpairs_s_max": max(gp),
+            "steps": r["steps"], "warmup": r["warmup"]}
 
 
 def bench_batch(args):
@@ -405,8 +457,8 @@ def main():
                          "allows -- all 256 planes of the headline volume)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
-                    help="headline only: skip the batched and differentiable workloads the default run "
-                         "appends to the line (for profiler runs)")
+                    help="headline only: skip the batched, differentiable, reference-default and sweep "
+                         "workloads the default run appends to the line (for profiler runs)")
     ap.add_argument("--pmc-pass", action="store_true",
                     help="one launch of the workload plus a 256 MiB calibration copy (for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -612,6 +664,11 @@ def main():
         sub.steps, sub.warmup = 3, 1
         line["batched"] = batch_summary(bench_batch(sub))
         line["grad"] = grad_summary(bench_grad(sub))
+        # the reference's own default configuration and its run.sh sweep, on
+        # its 8-bit data type
+        line["reference_default"] = ref_default_summary()
+        sub.steps, sub.warmup = 1, 1
+        line["sweep"] = sweep_summary(bench_sweep(sub))
     print(json.dumps(line))
 
 
