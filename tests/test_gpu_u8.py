@@ -55,6 +55,8 @@ U8_CASES = [
     ((9, 10, 8), 6, 1, 90.0, "uniform"),
     ((24, 22, 21), 5, 3, 10.0, "phantom"),   # NS = 11: 16^3 tiles on 26^2 grown rows
     ((33, 20, 40), 5, 7, 60.0, "uniform"),
+    ((1, 30, 37), 5, 4, 60.0, "uniform"),    # one plane, 16^3 tiles
+    ((3, 5, 70), 6, 6, 80.0, "uniform"),     # thinner than the halo in y and z
 ]
 
 
