@@ -80,6 +80,25 @@ __device__ __forceinline__ void lds_barrier2()
     __builtin_amdgcn_sched_barrier(0);
 }
 
+// An LDS address the compiler cannot see through: loads from it take their
+// offsets as immediates (up to 1020 bytes for paired 32-bit reads) from this
+// one base, instead of one address add per read pair, which it emits when the
+// base is a known constant plus a lane offset.
+typedef const __attribute__((address_space(3))) float lds_cfloat;
+typedef __attribute__((address_space(3))) float lds_float;
+__device__ __forceinline__ lds_cfloat *lds_opaque(const float *p)
+{
+    unsigned a = (unsigned)(size_t)(lds_cfloat *)p;
+    asm volatile("" : "+v"(a));
+    return (lds_cfloat *)(size_t)a;
+}
+__device__ __forceinline__ lds_float *lds_opaque_w(float *p)
+{
+    unsigned a = (unsigned)(size_t)(lds_float *)p;
+    asm volatile("" : "+v"(a));
+    return (lds_float *)(size_t)a;
+}
+
 // compile-time loop: f(std::integral_constant<int, I>) for I = I0 .. I1
 template <int I, int I1, class F>
 __device__ __forceinline__ void static_for2(F &&f)
@@ -170,23 +189,34 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(waves2<RP, R
             Zd[m] = 0.0f;
             Yd[m] = 0.0f;
         }
+        // the s_x buffer of the dy's first offset; the offsets alternate between
+        // the two (compile-time-radius builds: by the parity of dx, so that every
+        // LDS address of an offset is a per-dy base plus an immediate)
+        float *const SXe = SXB + (it & 1) * G::SXN, *const SXo = SXB + ((it & 1) ^ 1) * G::SXN;
+        lds_cfloat *const colE = lds_opaque(SXe + cx * G::EYP + cy0), *const colO = lds_opaque(SXo + cx * G::EYP + cy0);
+        lds_float *dstE[G::ROWS], *dstO[G::ROWS];
+#pragma unroll
+        for (int q = 0; q < G::ROWS; ++q) {
+            const int r = tid + q * NT;
+            const int rc = own[q] ? r : G::NR - 1;
+            const int o = (rc / G::EY) * G::SEG * G::EYP + rc % G::EY;
+            dstE[q] = lds_opaque_w(SXe + o);
+            dstO[q] = lds_opaque_w(SXo + o);
+        }
         // one offset: phase 1 with shifted sample i of row item q from shv(q, i),
-        // the barrier, phase 2
-        auto offset = [&](int dx, auto &&shv) {
-            float *SX = SXB + (it & 1) * G::SXN;
+        // the barrier, phase 2; odd: the offset takes the other buffer than the
+        // dy's first offset
+        auto offset = [&](int dx, bool odd, auto &&shv) {
             const int doff = dy * LX + dx;
 #pragma unroll
             for (int q = 0; q < G::ROWS; ++q) {
                 if (wav[q]) {
-                    const int r = tid + q * NT;
-                    const int rc = own[q] ? r : G::NR - 1;
-                    const int ry = rc % G::EY, sg = rc / G::EY;
                     float e[G::ES];
 #pragma unroll
                     for (int i = 0; i < G::ES; ++i) e[i] = cen[q][i] - shv(q, i);
                     float sx[G::SEG];
                     box_tree_sq<2 * RP + 1, G::SEG>(e, sx);
-                    float *dst = SX + (sg * G::SEG) * G::EYP + ry;
+                    lds_float *dst = odd ? dstO[q] : dstE[q];
                     if (own[q]) {
 #pragma unroll
                         for (int j = 0; j < G::SEG; ++j) dst[j * G::EYP] = sx[j];
@@ -196,7 +226,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(waves2<RP, R
             lds_barrier2();
 
             float col[G::YR + 2 * RP];
-            const float *src = SX + cx * G::EYP + cy0;
+            lds_cfloat *src = odd ? colO : colE;
 #pragma unroll
             for (int j = 0; j < G::YR + 2 * RP; ++j) col[j] = src[j];
             const float *cand = U + candbase + doff;
@@ -232,11 +262,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(waves2<RP, R
             }
             static_for2<-RSC, RSC>([&](auto dxc) {
                 constexpr int DX = decltype(dxc)::value;
-                offset(DX, [&](int q, int i) { return win[q][i + DX + RSC]; });
+                offset(DX, ((DX + RSC) & 1) != 0, [&](int q, int i) { return win[q][i + DX + RSC]; });
             });
         } else {
             for (int dx = -rs; dx <= rs; ++dx)
-                offset(dx, [&](int q, int i) { return U[base[q] + dy * LX + dx + i]; });
+                offset(dx, ((dx + rs) & 1) != 0, [&](int q, int i) { return U[base[q] + dy * LX + dx + i]; });
         }
 #pragma unroll
         for (int m = 0; m < G::YR; ++m) {
