@@ -561,7 +561,7 @@ def main():
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle
-        from nlm_metrics import phantom_parity
+        from nlm_metrics import fp32_units, phantom_parity
         yc = y.cpu().numpy()
         threads = _threads()
         model, phys = _cpu_info()
@@ -634,7 +634,7 @@ def main():
         dc = np.abs(ref32_64.astype(np.float64) - ref64)
         parity["cpu_fp32_max_rel_vs_fp64"] = float((dc / np.maximum(np.abs(ref64), 1e-6)).max())
         parity["cpu_fp32_max_abs_vs_fp64"] = float(dc.max())
-        parity["cpu_fp32_err_in_fp32_units_of_A_over_Z"] = float((dc / (2.0 ** -24 * A64 / Z64)).max())
+        parity["cpu_fp32_err_in_fp32_units_of_A_over_Z"] = float(fp32_units(dc, A64, Z64).max())
 
     line = {
         "metric": ((f"Mvoxels/s 3D NLM {nx}^3" if nx == ny == nz else f"Mvoxels/s 3D NLM {nx}x{ny}x{nz}") +

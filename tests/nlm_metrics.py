@@ -47,6 +47,19 @@ def report(a, b):
     return {"max_abs": max_abs(a, b), "max_rel": max_rel(a, b), "max_rel_floored": max_rel_fl(a, b)}
 
 
+def fp32_units(err, A64, Z64):
+    """|err| in units of 2^-24 A/Z.  Where A = 0 (every candidate of the voxel
+    is 0, as in the black regions of 8-bit images) y is exactly 0: an error of
+    0 there is 0 units, any other error infinitely many."""
+    err = np.abs(np.asarray(err, np.float64))
+    unit = 2.0 ** -24 * np.asarray(A64, np.float64) / np.asarray(Z64, np.float64)
+    out = np.zeros_like(err)
+    pos = unit > 0.0
+    out[pos] = err[pos] / unit[pos]
+    out[~pos & (err > 0.0)] = np.inf
+    return out
+
+
 def phantom_parity(got, ref32, ref64, Z64, A64, tol=1e-5, tol32=None):
     """The phantom-workload bar (module notes).  tol32 (default tol) is the
     bar against the CPU FP32 path above tau.  Returns the report; the caller
@@ -59,7 +72,7 @@ def phantom_parity(got, ref32, ref64, Z64, A64, tol=1e-5, tol32=None):
     big = np.abs(r64) >= tau
     rel32 = np.abs(g - r32) / np.maximum(np.abs(r32), 1e-30)
     rel64 = np.abs(g - r64) / np.maximum(np.abs(r64), 1e-30)
-    cond = np.abs(g - r64) / (2.0 ** -24 * np.asarray(A64, np.float64) / np.asarray(Z64, np.float64))
+    cond = fp32_units(g - r64, A64, Z64)
     rep = {
         "tau": tau,
         "voxels_below_tau": int((~big).sum()),

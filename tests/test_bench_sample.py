@@ -59,3 +59,16 @@ def test_traffic_figures_per_launch_shape():
     assert bench._traffic_from_profiles("nlm3d_256_p5_s11", 7) is None
     assert bench._traffic_from_profiles("no such workload") is None
     assert isinstance(bench._traffic_from_profiles("nlm3d_512x512x128_p17_s13_u8"), int)
+
+
+def test_parity_units_where_every_candidate_is_zero():
+    """The A/Z bar on 8-bit images with black regions: where A = 0 the output
+    is exactly 0, so an exact 0 is 0 units (not 0/0) and anything else fails."""
+    from nlm_metrics import fp32_units, phantom_parity
+    A = np.array([0.0, 2.0, 0.0])
+    Z = np.array([1.0, 1.0, 3.0])
+    u = fp32_units(np.array([0.0, 2.0 ** -24, 1e-9]), A, Z)
+    assert u[0] == 0.0 and u[1] == 0.5 and np.isinf(u[2])
+    ref = np.array([0.0, 1.0, 2.0])
+    rep = phantom_parity(ref.copy(), ref, ref, np.ones(3), np.array([0.0, 1.0, 2.0]))
+    assert rep["ok"] and rep["max_err_in_fp32_units_of_A_over_Z"] == 0.0
