@@ -342,21 +342,26 @@ int oracle_nlm3d_separable_f32(const float *u, int nx, int ny, int nz,
  * Z(x) = sum_d w_d(x) and A(x) = sum_d w_d(x) |u~(x+d)|; A/Z is the scale of
  * the rounding error of any FP32 evaluation of y = sum_d w_d u~(x+d) / Z (the
  * sum of the magnitudes of its terms), which the parity tests use to bound
- * the error at outputs near zero. */
+ * the error at outputs near zero.  Bout (nullable) receives
+ * B(x) = sum_d w_d(x) a_d(x) (|u~(x+d)| + |y(x)|), a_d = S_d / (P h^2) the
+ * exponent of w_d = exp(-a_d): an FP32 evaluation of a_d carries a relative
+ * error of a few 2^-24, so w_d one of a few a_d 2^-24, and y moves by up to
+ * sum_d w_d a_d |u~(x+d) - y| / Z <= B / Z of those units (large for 8-bit
+ * data, where a_d reaches 80 and more). */
 int oracle_nlm3d_separable_f64_za(const float *u, int nx, int ny, int nz,
                                   int rp, int rs, double h, double *y, double *Zout, double *Aout,
-                                  int z0, int z1, int nthreads);
+                                  double *Bout, int z0, int z1, int nthreads);
 
 int oracle_nlm3d_separable_f64(const float *u, int nx, int ny, int nz,
                                int rp, int rs, double h, double *y,
                                int z0, int z1, int nthreads)
 {
-    return oracle_nlm3d_separable_f64_za(u, nx, ny, nz, rp, rs, h, y, NULL, NULL, z0, z1, nthreads);
+    return oracle_nlm3d_separable_f64_za(u, nx, ny, nz, rp, rs, h, y, NULL, NULL, NULL, z0, z1, nthreads);
 }
 
 int oracle_nlm3d_separable_f64_za(const float *u, int nx, int ny, int nz,
                                   int rp, int rs, double h, double *y, double *Zout, double *Aout,
-                                  int z0, int z1, int nthreads)
+                                  double *Bout, int z0, int z1, int nthreads)
 {
     const int R = rp + rs;
     const long ex = nx + 2L * R, ey = ny + 2L * R, ez = nz + 2L * R;
@@ -382,7 +387,9 @@ int oracle_nlm3d_separable_f64_za(const float *u, int nx, int ny, int nz,
         double *Zs = (double *)malloc(sizeof(double) * (long)nx * ny * SLAB);
         double *Ys = (double *)malloc(sizeof(double) * (long)nx * ny * SLAB);
         double *As = (double *)malloc(sizeof(double) * (long)nx * ny * SLAB);
-        if (!sx || !sxy || !Zs || !Ys || !As) {
+        double *WAs = (double *)malloc(sizeof(double) * (long)nx * ny * SLAB);
+        double *WUs = (double *)malloc(sizeof(double) * (long)nx * ny * SLAB);
+        if (!sx || !sxy || !Zs || !Ys || !As || !WAs || !WUs) {
 #pragma omp atomic write
             fail = 1;
         } else {
@@ -396,6 +403,8 @@ int oracle_nlm3d_separable_f64_za(const float *u, int nx, int ny, int nz,
                 memset(Zs, 0, sizeof(double) * (long)nx * ny * SLAB);
                 memset(Ys, 0, sizeof(double) * (long)nx * ny * SLAB);
                 memset(As, 0, sizeof(double) * (long)nx * ny * SLAB);
+                memset(WAs, 0, sizeof(double) * (long)nx * ny * SLAB);
+                memset(WUs, 0, sizeof(double) * (long)nx * ny * SLAB);
                 for (int dz = -rs; dz <= rs; ++dz)
                     for (int dy = -rs; dy <= rs; ++dy)
                         for (int dx = -rs; dx <= rs; ++dx) {
@@ -432,14 +441,19 @@ int oracle_nlm3d_separable_f64_za(const float *u, int nx, int ny, int nz,
                                     double *Zr = Zs + (long)nx * (jy + (long)ny * kz);
                                     double *Yr = Ys + (long)nx * (jy + (long)ny * kz);
                                     double *Ar = As + (long)nx * (jy + (long)ny * kz);
+                                    double *WAr = WAs + (long)nx * (jy + (long)ny * kz);
+                                    double *WUr = WUs + (long)nx * (jy + (long)ny * kz);
                                     for (int x = 0; x < nx; ++x) {
                                         double S = 0.0;
                                         for (int pz = 0; pz <= 2 * rp; ++pz)
                                             S += sxy[x + wx * (jy + (long)ny * (kz + pz))];
-                                        double w = exp(-(S / P) * inv);
+                                        const double arg = (S / P) * inv;
+                                        double w = exp(-arg);
                                         Zr[x] += w;
                                         Yr[x] += w * cand[x];
                                         Ar[x] += w * fabs(cand[x]);
+                                        WAr[x] += w * arg;
+                                        WUr[x] += w * arg * fabs(cand[x]);
                                     }
                                 }
                         }
@@ -451,21 +465,22 @@ int oracle_nlm3d_separable_f64_za(const float *u, int nx, int ny, int nz,
                             y[o] = Ys[i] / Zs[i];
                             if (Zout) Zout[o] = Zs[i];
                             if (Aout) Aout[o] = As[i];
+                            if (Bout) Bout[o] = WUs[i] + fabs(y[o]) * WAs[i];
                         }
             }
         }
-        free(sx); free(sxy); free(Zs); free(Ys); free(As);
+        free(sx); free(sxy); free(Zs); free(Ys); free(As); free(WAs); free(WUs);
     }
     free(pu);
     return fail ? -1 : 0;
 }
 
 /* ------------------------------------------------------------------ */
-/* 2D separable FP64 (plain double sums), OpenMP over rows; Zout, Aout  */
-/* (nullable) as in oracle_nlm3d_separable_f64_za.                      */
+/* 2D separable FP64 (plain double sums), OpenMP over rows; Zout, Aout, */
+/* Bout (nullable) as in oracle_nlm3d_separable_f64_za.                 */
 /* ------------------------------------------------------------------ */
 int oracle_nlm2d_separable_f64(const float *u, int nx, int ny, int rp, int rs,
-                               double h, double *y, double *Zout, double *Aout, int nthreads)
+                               double h, double *y, double *Zout, double *Aout, double *Bout, int nthreads)
 {
     const int R = rp + rs;
     const long ex = nx + 2L * R, ey = ny + 2L * R;
@@ -488,7 +503,9 @@ int oracle_nlm2d_separable_f64(const float *u, int nx, int ny, int rp, int rs,
         double *Zs = (double *)malloc(sizeof(double) * nx * BAND);
         double *Ys = (double *)malloc(sizeof(double) * nx * BAND);
         double *As = (double *)malloc(sizeof(double) * nx * BAND);
-        if (!sx || !Zs || !Ys || !As) {
+        double *WAs = (double *)malloc(sizeof(double) * nx * BAND);
+        double *WUs = (double *)malloc(sizeof(double) * nx * BAND);
+        if (!sx || !Zs || !Ys || !As || !WAs || !WUs) {
 #pragma omp atomic write
             fail = 1;
         } else {
@@ -502,6 +519,8 @@ int oracle_nlm2d_separable_f64(const float *u, int nx, int ny, int rp, int rs,
                 memset(Zs, 0, sizeof(double) * nx * BAND);
                 memset(Ys, 0, sizeof(double) * nx * BAND);
                 memset(As, 0, sizeof(double) * nx * BAND);
+                memset(WAs, 0, sizeof(double) * nx * BAND);
+                memset(WUs, 0, sizeof(double) * nx * BAND);
                 for (int dy = -rs; dy <= rs; ++dy)
                     for (int dx = -rs; dx <= rs; ++dx) {
                         for (int jy = 0; jy < ty + 2 * rp; ++jy) {
@@ -523,11 +542,14 @@ int oracle_nlm2d_separable_f64(const float *u, int nx, int ny, int rp, int rs,
                             for (int x = 0; x < nx; ++x) {
                                 double S = 0.0;
                                 for (int py = 0; py <= 2 * rp; ++py) S += sx[x + (long)nx * (jy + py)];
-                                const double w = exp(-(S / P) * inv);
+                                const double arg = (S / P) * inv;
+                                const double w = exp(-arg);
                                 const long i = x + (long)nx * jy;
                                 Zs[i] += w;
                                 Ys[i] += w * cand[x];
                                 As[i] += w * fabs(cand[x]);
+                                WAs[i] += w * arg;
+                                WUs[i] += w * arg * fabs(cand[x]);
                             }
                         }
                     }
@@ -537,10 +559,11 @@ int oracle_nlm2d_separable_f64(const float *u, int nx, int ny, int rp, int rs,
                         y[o] = Ys[i] / Zs[i];
                         if (Zout) Zout[o] = Zs[i];
                         if (Aout) Aout[o] = As[i];
+                        if (Bout) Bout[o] = WUs[i] + fabs(y[o]) * WAs[i];
                     }
             }
         }
-        free(sx); free(Zs); free(Ys); free(As);
+        free(sx); free(Zs); free(Ys); free(As); free(WAs); free(WUs);
     }
     free(pu);
     return fail ? -1 : 0;

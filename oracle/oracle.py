@@ -45,9 +45,9 @@ def lib():
         L.oracle_nlm3d_separable_f64.restype = _i
         _vp = ctypes.c_void_p
         L.oracle_nlm3d_separable_f64_za.argtypes = [_f32p, _i, _i, _i, _i, _i, ctypes.c_double, _f64p, _vp, _vp,
-                                                    _i, _i, _i]
+                                                    _vp, _i, _i, _i]
         L.oracle_nlm3d_separable_f64_za.restype = _i
-        L.oracle_nlm2d_separable_f64.argtypes = [_f32p, _i, _i, _i, _i, ctypes.c_double, _f64p, _vp, _vp, _i]
+        L.oracle_nlm2d_separable_f64.argtypes = [_f32p, _i, _i, _i, _i, ctypes.c_double, _f64p, _vp, _vp, _vp, _i]
         L.oracle_nlm2d_separable_f64.restype = _i
         L.oracle_nlm2d_separable_f32.argtypes = [_f32p, _i, _i, _i, _i, ctypes.c_float, _f32p, _i]
         L.oracle_nlm2d_separable_f32.restype = _i
@@ -126,34 +126,38 @@ def nlm3d_separable_f64(u, rp, rs, h, z0=0, z1=None, nthreads=0):
     return y
 
 
-def nlm3d_separable_f64_za(u, rp, rs, h, z0=0, z1=None, nthreads=0):
+def nlm3d_separable_f64_za(u, rp, rs, h, z0=0, z1=None, nthreads=0, with_b=False):
     """FP64 re-run of planes [z0, z1) with the normaliser Z = sum_d w_d and
-    A = sum_d w_d |u~(x+d)|: returns (y, Z, A), float64."""
+    A = sum_d w_d |u~(x+d)|: returns (y, Z, A), float64; with_b also B =
+    sum_d w_d a_d (|u~(x+d)| + |y|), a_d the exponent of w_d: (y, Z, A, B)."""
     u = np.ascontiguousarray(u, dtype=np.float32)
     nz, ny, nx = u.shape
     z1 = nz if z1 is None else z1
     y = np.empty((z1 - z0, ny, nx), np.float64)
     Z = np.empty_like(y)
     A = np.empty_like(y)
+    B = np.empty_like(y) if with_b else None
     rc = lib().oracle_nlm3d_separable_f64_za(u, nx, ny, nz, rp, rs, float(h), y, Z.ctypes.data, A.ctypes.data,
-                                             z0, z1, int(nthreads))
+                                             B.ctypes.data if with_b else None, z0, z1, int(nthreads))
     if rc != 0:
         raise MemoryError("oracle_nlm3d_separable_f64_za failed")
-    return y, Z, A
+    return (y, Z, A, B) if with_b else (y, Z, A)
 
 
-def nlm2d_separable_f64(u, rp, rs, h, nthreads=0):
-    """2D FP64 separable path: returns (y, Z, A) as nlm3d_separable_f64_za."""
+def nlm2d_separable_f64(u, rp, rs, h, nthreads=0, with_b=False):
+    """2D FP64 separable path: returns (y, Z, A), or (y, Z, A, B) with with_b,
+    as nlm3d_separable_f64_za."""
     u = np.ascontiguousarray(u, dtype=np.float32)
     ny, nx = u.shape
     y = np.empty((ny, nx), np.float64)
     Z = np.empty_like(y)
     A = np.empty_like(y)
+    B = np.empty_like(y) if with_b else None
     rc = lib().oracle_nlm2d_separable_f64(u, nx, ny, rp, rs, float(h), y, Z.ctypes.data, A.ctypes.data,
-                                          int(nthreads))
+                                          B.ctypes.data if with_b else None, int(nthreads))
     if rc != 0:
         raise MemoryError("oracle_nlm2d_separable_f64 failed")
-    return y, Z, A
+    return (y, Z, A, B) if with_b else (y, Z, A)
 
 
 def nlm2d_separable_f32(u, rp, rs, h, nthreads=0):

@@ -23,9 +23,9 @@ void oracle_nlm2d_brute_f32(const float *u, int nx, int ny, int rp, int rs, floa
 int oracle_nlm3d_separable_f32(const float *u, int nx, int ny, int nz, int rp, int rs, float h, float *y, int z0,
                                int z1, int nthreads);
 int oracle_nlm3d_separable_f64_za(const float *u, int nx, int ny, int nz, int rp, int rs, double h, double *y,
-                                  double *Zout, double *Aout, int z0, int z1, int nthreads);
+                                  double *Zout, double *Aout, double *Bout, int z0, int z1, int nthreads);
 int oracle_nlm2d_separable_f64(const float *u, int nx, int ny, int rp, int rs, double h, double *y, double *Zout,
-                               double *Aout, int nthreads);
+                               double *Aout, double *Bout, int nthreads);
 int oracle_nlm2d_separable_f32(const float *u, int nx, int ny, int rp, int rs, float h, float *y, int nthreads);
 void oracle_nlm3d_grad_f64(const double *u, const double *g, int nx, int ny, int nz, int rp, int rs, double h,
                            double *gu, double *gh);
@@ -51,6 +51,7 @@ static void case3d(int nx, int ny, int nz, int rp, int rs)
     float *ys = malloc(sizeof(float) * n);
     double *u64 = malloc(sizeof(double) * n), *y64 = malloc(sizeof(double) * n), *z64 = malloc(sizeof(double) * n);
     double *ysep = malloc(sizeof(double) * n), *zsep = malloc(sizeof(double) * n), *asep = malloc(sizeof(double) * n);
+    double *bsep = malloc(sizeof(double) * n);
     double *g = malloc(sizeof(double) * n), *gu = malloc(sizeof(double) * n), *cl = malloc(sizeof(double) * n);
     for (long i = 0; i < n; ++i) {
         u32[i] = (float)rnd();
@@ -68,7 +69,7 @@ static void case3d(int nx, int ny, int nz, int rp, int rs)
     }
     /* planes [z0, z1) of the separable FP64 path, the rest of y left alone */
     const int z0 = nz / 3, z1 = nz - nz / 4;
-    if (oracle_nlm3d_separable_f64_za(u32, nx, ny, nz, rp, rs, h, ysep, zsep, asep, z0, z1, 2) != 0) ++fails;
+    if (oracle_nlm3d_separable_f64_za(u32, nx, ny, nz, rp, rs, h, ysep, zsep, asep, bsep, z0, z1, 2) != 0) ++fails;
     const long off = (long)z0 * nx * ny;
     for (long i = 0; i < (long)(z1 - z0) * nx * ny; ++i) {
         if (fabs(ysep[i] - y64[off + i]) > 1e-12 * fabs(y64[off + i]) + 1e-15 ||
@@ -83,7 +84,7 @@ static void case3d(int nx, int ny, int nz, int rp, int rs)
     const double loss = oracle_nlm3d_mse_f64(u64, cl, nx, ny, nz, rp, rs, h);
     if (!(loss >= 0.0) || !isfinite(gh)) ++fails;
     free(u32), free(y32), free(z32), free(ys), free(u64), free(y64), free(z64);
-    free(ysep), free(zsep), free(asep), free(g), free(gu), free(cl);
+    free(ysep), free(zsep), free(asep), free(bsep), free(g), free(gu), free(cl);
 }
 
 static void case2d(int nx, int ny, int rp, int rs)
@@ -93,6 +94,7 @@ static void case2d(int nx, int ny, int rp, int rs)
     float *ys = malloc(sizeof(float) * n);
     double *u64 = malloc(sizeof(double) * n), *y64 = malloc(sizeof(double) * n), *z64 = malloc(sizeof(double) * n);
     double *ysep = malloc(sizeof(double) * n), *zsep = malloc(sizeof(double) * n), *asep = malloc(sizeof(double) * n);
+    double *bsep = malloc(sizeof(double) * n);
     for (long i = 0; i < n; ++i) {
         u32[i] = (float)rnd();
         u64[i] = u32[i];
@@ -101,7 +103,7 @@ static void case2d(int nx, int ny, int rp, int rs)
     oracle_nlm2d_brute_f64(u64, nx, ny, rp, rs, h, y64, z64);
     oracle_nlm2d_brute_f32(u32, nx, ny, rp, rs, (float)h, y32, z32);
     if (oracle_nlm2d_separable_f32(u32, nx, ny, rp, rs, (float)h, ys, 2) != 0) ++fails;
-    if (oracle_nlm2d_separable_f64(u32, nx, ny, rp, rs, h, ysep, zsep, asep, 2) != 0) ++fails;
+    if (oracle_nlm2d_separable_f64(u32, nx, ny, rp, rs, h, ysep, zsep, asep, bsep, 2) != 0) ++fails;
     for (long i = 0; i < n; ++i) {
         if (fabs(ysep[i] - y64[i]) > 1e-12 * fabs(y64[i]) + 1e-15) {
             fprintf(stderr, "2D f64 separable != brute at %dx%d rp %d rs %d\n", nx, ny, rp, rs);
@@ -109,7 +111,8 @@ static void case2d(int nx, int ny, int rp, int rs)
             break;
         }
     }
-    free(u32), free(y32), free(z32), free(ys), free(u64), free(y64), free(z64), free(ysep), free(zsep), free(asep);
+    free(u32), free(y32), free(z32), free(ys), free(u64), free(y64), free(z64), free(ysep), free(zsep), free(asep),
+        free(bsep);
 }
 
 int main(void)
