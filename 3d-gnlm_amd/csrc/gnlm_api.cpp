@@ -135,6 +135,36 @@ int gnlm2d_forward(const float *d_in, float *d_out, int nx, int ny, int r_p, int
     return gnlm2d_forward_ex(d_in, 0, 0, d_out, nullptr, nx, ny, r_p, r_s, h, GNLM_PATH_AUTO, stream);
 }
 
+int gnlm2d_forward_u8(const unsigned char *d_in, int in_pad, long long in_sy, float *d_out, float *d_Z, int nx,
+                      int ny, int r_p, int r_s, float h, void *stream)
+{
+    if (!d_in || !d_out) return GNLM_EINVAL;
+    if (nx < 1 || ny < 1 || r_p < 0 || r_s < 0 || bad_h(h)) return GNLM_EINVAL;
+    const int R = r_p + r_s;
+    long long in_sz = 0;
+    int st = check_padded(in_pad, in_sy, in_sz, nx, ny, R);
+    if (st != GNLM_OK) return st;
+    hipStream_t s = (hipStream_t)stream;
+    if (fwd2d_tiled_supported(r_p, r_s)) {
+        InDesc in{nullptr, nx, ny, 1, in_pad, in_sy, 0};
+        const float k = (float)weight_scale((2 * r_p + 1) * (2 * r_p + 1), h);
+        return hip_status(fwd2d_tiled_u8_launch(in, d_in, d_out, d_Z, r_p, r_s, k, s));
+    }
+    // other radii: the FP32 path on a float copy (a padded input is converted
+    // up to its last sample)
+    const long long n = in_pad == 0 ? (long long)nx * ny
+                                    : (long long)nx + 2LL * in_pad + in_sy * ((long long)ny + 2LL * in_pad - 1);
+    float *d_f = nullptr;
+    hipError_t e = hipMallocAsync((void **)&d_f, sizeof(float) * (size_t)n, s);
+    if (e != hipSuccess) return hip_status(e);
+    e = u8_to_f32_any_launch(d_in, d_f, n, s);
+    st = e == hipSuccess ? gnlm2d_forward_ex(d_f, in_pad, in_sy, d_out, d_Z, nx, ny, r_p, r_s, h, GNLM_PATH_AUTO,
+                                             stream)
+                         : hip_status(e);
+    const hipError_t e2 = hipFreeAsync(d_f, s);
+    return st != GNLM_OK ? st : hip_status(e2);
+}
+
 int gnlm3d_backward(const float *d_in, const float *d_out, const float *d_Z, const float *d_gout, int nx, int ny,
                     int nz, int r_p, int r_s, float h, float *d_gin, float *d_gh, void *stream)
 {
@@ -236,19 +266,16 @@ int gnlm_nl_means_2d_u8(const unsigned char *h_in, float *h_out, int W, int H, i
     // would overlap; the reference's driver only produces X <= Y (square).
     if (image_size_x > image_size_y) return GNLM_EINVAL;
     const long long nin = (long long)W * H;
-    const long long nin_al = (nin + 15) / 16 * 16;
     // The reference writes rows of stride IMAGE_SIZE_Y ("2D NLM/11x11/main.cu:64").
     const long long nout = (long long)image_size_y * image_size_y;
     unsigned char *d_u8 = nullptr;
-    float *d_f = nullptr, *d_o = nullptr;
+    float *d_o = nullptr;
     int st = GNLM_OK;
-    hipError_t e = hipMalloc((void **)&d_u8, (size_t)nin_al);
-    if (e == hipSuccess) e = hipMalloc((void **)&d_f, sizeof(float) * (size_t)nin);
+    hipError_t e = hipMalloc((void **)&d_u8, (size_t)nin);
     if (e == hipSuccess) e = hipMalloc((void **)&d_o, sizeof(float) * (size_t)nx * ny);
     if (e == hipSuccess) e = hipMemcpy(d_u8, h_in, (size_t)nin, hipMemcpyHostToDevice);
     if (e != hipSuccess) st = hip_status(e);
-    if (st == GNLM_OK) st = gnlm_u8_to_f32(d_u8, d_f, nin, nullptr);
-    if (st == GNLM_OK) st = gnlm2d_forward_ex(d_f, pad, W, d_o, nullptr, nx, ny, r_p, r_s, h, GNLM_PATH_AUTO, nullptr);
+    if (st == GNLM_OK) st = gnlm2d_forward_u8(d_u8, pad, W, d_o, nullptr, nx, ny, r_p, r_s, h, nullptr);
     if (st == GNLM_OK) {
         if (image_size_x == image_size_y) {
             e = hipMemcpy(h_out, d_o, sizeof(float) * (size_t)nout, hipMemcpyDeviceToHost);
@@ -259,7 +286,6 @@ int gnlm_nl_means_2d_u8(const unsigned char *h_in, float *h_out, int W, int H, i
         if (e != hipSuccess) st = hip_status(e);
     }
     if (d_u8) (void)hipFree(d_u8);
-    if (d_f) (void)hipFree(d_f);
     if (d_o) (void)hipFree(d_o);
     return st;
 }

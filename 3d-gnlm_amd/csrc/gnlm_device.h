@@ -33,17 +33,22 @@ __device__ __forceinline__ int reflect101(int i, int n)
 }
 
 // Sample u~ at (x, y, z) in volume coordinates (may lie outside the volume by
-// up to the padding radius in padded mode, by any amount in reflect mode).
-__device__ __forceinline__ float load_in(const InDesc &d, int x, int y, int z)
+// up to the padding radius in padded mode, by any amount in reflect mode),
+// from data laid out as d describes (load_at: samples of another type, e.g.
+// an 8-bit image, with d's sizes, padding and strides in elements).
+template <class T>
+__device__ __forceinline__ T load_at(const T *data, const InDesc &d, int x, int y, int z)
 {
     if (d.pad == 0) {
         x = reflect101(x, d.nx);
         y = reflect101(y, d.ny);
         z = reflect101(z, d.nz);
-        return d.data[(long long)x + (long long)d.nx * ((long long)y + (long long)d.ny * z)];
+        return data[(long long)x + (long long)d.nx * ((long long)y + (long long)d.ny * z)];
     }
-    return d.data[(long long)(x + d.pad) + d.sy * (long long)(y + d.pad) + d.sz * (long long)(z + d.pad)];
+    return data[(long long)(x + d.pad) + d.sy * (long long)(y + d.pad) + d.sz * (long long)(z + d.pad)];
 }
+
+__device__ __forceinline__ float load_in(const InDesc &d, int x, int y, int z) { return load_at(d.data, d, x, y, z); }
 
 // v_exp_f32: 2^x.  The NLM weight exp(-S/(P h^2)) is evaluated as
 // exp2(S * k) with k = -log2(e)/(P h^2) folded on the host.

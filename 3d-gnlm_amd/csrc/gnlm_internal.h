@@ -77,6 +77,9 @@ bool fwd3d_tuning_lazy();  // gnlm_set_tuning(GNLM_TUNE_LAZY) in force
 bool fwd2d_tiled_supported(int r_p, int r_s);
 hipError_t fwd2d_tiled_launch(const InDesc &in, float *out, float *Zout, int r_p, int r_s, float k,
                               hipStream_t s);
+// the 8-bit build: samples from in8, laid out as in describes (in.data unused)
+hipError_t fwd2d_tiled_u8_launch(const InDesc &in, const unsigned char *in8, float *out, float *Zout, int r_p,
+                                 int r_s, float k, hipStream_t s);
 
 // ---- generic per-offset forward (nlm_generic.hip) --------------------------
 // Any radii; 2D when rpz = rsz = 0 and nz = 1.  Workspace is allocated inside.

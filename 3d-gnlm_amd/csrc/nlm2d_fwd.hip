@@ -27,6 +27,12 @@
 // those to rounding (~1e-7 relative in S); for integer-valued input (the
 // uint8 images of the reference) every S is an exact integer in FP32 and the
 // results are identical bit for bit.
+//
+// 8-bit build (INT8): the tile is staged from the caller's 8-bit image, and
+// since every partial patch sum of squared 8-bit differences is an integer
+// below 2^24 (11 x 11 x 255^2 < 2^23), both box sums run as sliding windows
+// (nlm_box.h:box_run*), 2 operations per output instead of about 3: the same
+// S, so the same bits as the FP32 build on the converted image.
 #include <type_traits>
 #include "gnlm_internal.h"
 #include "nlm_box.h"
@@ -51,7 +57,8 @@ struct G2 {
 };
 
 struct Fwd2Args {
-    InDesc in;
+    InDesc in;                 // layout (and, FP32 build, the samples)
+    const unsigned char *in8;  // the samples, 8-bit build
     float *out;
     float *Zout;
     int rs;
@@ -122,11 +129,14 @@ __device__ __forceinline__ void static_for2(F &&f)
 template <int RP, int RSC>
 constexpr int waves2() { return RSC == 0 ? 1 : (RP >= 4 ? 3 : 4); }
 
-template <int RP, int RSC>
+template <int RP, int RSC, bool INT8>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(waves2<RP, RSC>()))) void nlm2d_fwd_tiled(
     Fwd2Args a)
 {
     using G = G2<RP>;
+    // the sliding windows are exact: a patch sum plus one x-row sum stays
+    // below 2^24
+    static_assert(!INT8 || (2 * RP + 2) * (2 * RP + 1) * 65025 < (1 << 24), "8-bit sums must stay exact");
     extern __shared__ float lds[];
     const int R = RP + (RSC > 0 ? RSC : a.rs);
     const int LXr = G::TX + 2 * R;
@@ -143,7 +153,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(waves2<RP, R
     for (int i = tid; i < LX * LY; i += NT) {
         const int lx = i % LX, ly = i / LX;
         float v = 0.0f;
-        if (lx < LXr) v = load_in(a.in, x0 - R + lx, y0 - R + ly, 0);
+        if (lx < LXr) {
+            if constexpr (INT8) v = (float)load_at(a.in8, a.in, x0 - R + lx, y0 - R + ly, 0);
+            else v = load_in(a.in, x0 - R + lx, y0 - R + ly, 0);
+        }
         U[i] = v;
     }
     __syncthreads();
@@ -215,7 +228,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(waves2<RP, R
 #pragma unroll
                     for (int i = 0; i < G::ES; ++i) e[i] = cen[q][i] - shv(q, i);
                     float sx[G::SEG];
-                    box_tree_sq<2 * RP + 1, G::SEG>(e, sx);
+                    if constexpr (INT8) box_run_sq<2 * RP + 1, G::SEG>(e, sx);
+                    else box_tree_sq<2 * RP + 1, G::SEG>(e, sx);
                     lds_float *dst = odd ? dstO[q] : dstE[q];
                     if (own[q]) {
 #pragma unroll
@@ -231,7 +245,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(waves2<RP, R
             for (int j = 0; j < G::YR + 2 * RP; ++j) col[j] = src[j];
             const float *cand = U + candbase + doff;
             float S[G::YR];
-            box_tree<2 * RP + 1, G::YR>(col, S);
+            if constexpr (INT8) box_run<2 * RP + 1, G::YR>(col, S);
+            else box_tree<2 * RP + 1, G::YR>(col, S);
 #pragma unroll
             for (int m = 0; m < G::YR; ++m) {
                 const float w = exp2_fast(S[m] * a.k);
@@ -289,12 +304,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(waves2<RP, R
     }
 }
 
-template <int RP, int RSC>
-hipError_t launch2_k(const InDesc &in, float *out, float *Zout, int rs, float k, hipStream_t s)
+template <int RP, int RSC, bool INT8>
+hipError_t launch2_k(const InDesc &in, const unsigned char *in8, float *out, float *Zout, int rs, float k,
+                     hipStream_t s)
 {
     using G = G2<RP>;
     Fwd2Args a;
     a.in = in;
+    a.in8 = in8;
     a.out = out;
     a.Zout = Zout;
     a.rs = rs;
@@ -302,25 +319,41 @@ hipError_t launch2_k(const InDesc &in, float *out, float *Zout, int rs, float k,
     a.ntx = (in.nx + G::TX - 1) / G::TX;
     a.nty = (in.ny + G::TY - 1) / G::TY;
     const size_t lds = fwd2_lds_bytes<RP>(rs);
-    hipError_t e = hipFuncSetAttribute((const void *)nlm2d_fwd_tiled<RP, RSC>,
+    hipError_t e = hipFuncSetAttribute((const void *)nlm2d_fwd_tiled<RP, RSC, INT8>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((nlm2d_fwd_tiled<RP, RSC>), dim3(a.ntx * a.nty), dim3(NT), lds, s, a);
+    hipLaunchKernelGGL((nlm2d_fwd_tiled<RP, RSC, INT8>), dim3(a.ntx * a.nty), dim3(NT), lds, s, a);
     return hipGetLastError();
 }
 
 // Search radii with a register-window build (the reference's own 2D geometry,
 // patch 11 / search 21, and patch 5 / search 11); the others run the
 // run-time-radius kernel.
-template <int RP>
-hipError_t launch2_rp(const InDesc &in, float *out, float *Zout, int rs, float k, hipStream_t s)
+template <int RP, bool INT8>
+hipError_t launch2_rp(const InDesc &in, const unsigned char *in8, float *out, float *Zout, int rs, float k,
+                      hipStream_t s)
 {
     if constexpr (RP == 5) {
-        if (rs == 10) return launch2_k<RP, 10>(in, out, Zout, rs, k, s);
+        if (rs == 10) return launch2_k<RP, 10, INT8>(in, in8, out, Zout, rs, k, s);
     } else if constexpr (RP == 2) {
-        if (rs == 5) return launch2_k<RP, 5>(in, out, Zout, rs, k, s);
+        if (rs == 5) return launch2_k<RP, 5, INT8>(in, in8, out, Zout, rs, k, s);
     }
-    return launch2_k<RP, 0>(in, out, Zout, rs, k, s);
+    return launch2_k<RP, 0, INT8>(in, in8, out, Zout, rs, k, s);
+}
+
+template <bool INT8>
+hipError_t launch2(const InDesc &in, const unsigned char *in8, float *out, float *Zout, int r_p, int r_s, float k,
+                   hipStream_t s)
+{
+    switch (r_p) {
+    case 0: return launch2_rp<0, INT8>(in, in8, out, Zout, r_s, k, s);
+    case 1: return launch2_rp<1, INT8>(in, in8, out, Zout, r_s, k, s);
+    case 2: return launch2_rp<2, INT8>(in, in8, out, Zout, r_s, k, s);
+    case 3: return launch2_rp<3, INT8>(in, in8, out, Zout, r_s, k, s);
+    case 4: return launch2_rp<4, INT8>(in, in8, out, Zout, r_s, k, s);
+    case 5: return launch2_rp<5, INT8>(in, in8, out, Zout, r_s, k, s);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 constexpr size_t kLdsLimit = 160 * 1024;
@@ -344,15 +377,13 @@ bool fwd2d_tiled_supported(int r_p, int r_s)
 hipError_t fwd2d_tiled_launch(const InDesc &in, float *out, float *Zout, int r_p, int r_s, float k,
                               hipStream_t s)
 {
-    switch (r_p) {
-    case 0: return launch2_rp<0>(in, out, Zout, r_s, k, s);
-    case 1: return launch2_rp<1>(in, out, Zout, r_s, k, s);
-    case 2: return launch2_rp<2>(in, out, Zout, r_s, k, s);
-    case 3: return launch2_rp<3>(in, out, Zout, r_s, k, s);
-    case 4: return launch2_rp<4>(in, out, Zout, r_s, k, s);
-    case 5: return launch2_rp<5>(in, out, Zout, r_s, k, s);
-    default: return hipErrorInvalidValue;
-    }
+    return launch2<false>(in, nullptr, out, Zout, r_p, r_s, k, s);
+}
+
+hipError_t fwd2d_tiled_u8_launch(const InDesc &in, const unsigned char *in8, float *out, float *Zout, int r_p,
+                                 int r_s, float k, hipStream_t s)
+{
+    return launch2<true>(in, in8, out, Zout, r_p, r_s, k, s);
 }
 
 }  // namespace gnlm

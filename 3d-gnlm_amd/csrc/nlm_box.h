@@ -141,4 +141,41 @@ __device__ __forceinline__ void box_tree(const float *v, float *s)
     }
 }
 
+// Sliding-window box sums: s[0] = v[0] + ... + v[W-1] left to right, then
+// s[i] = (s[i-1] + v[i+W-1]) - v[i-1] -- 2 operations per output, against
+// about 3 for the trees.  Only for integer-valued inputs whose partial sums
+// all stay below 2^24 in magnitude (squared differences of 8-bit samples over
+// patches of up to 11 x 11): there every sum is exact in FP32, so the result
+// is the trees' (and any order's) bit for bit; on general floating-point data
+// the running window would accumulate rounding and cancellation.
+template <int W, int N>
+__device__ __forceinline__ void box_run(const float *v, float *s)
+{
+    float a = v[0];
+#pragma unroll
+    for (int i = 1; i < W; ++i) a = a + v[i];
+    s[0] = a;
+#pragma unroll
+    for (int i = 1; i < N; ++i) {
+        a = (a + v[i + W - 1]) - v[i - 1];
+        s[i] = a;
+    }
+}
+
+// box_run over the squares of e (same exactness condition), the squares
+// folded into fused multiply-adds: W + 2 (N - 1) operations for N outputs.
+template <int W, int N>
+__device__ __forceinline__ void box_run_sq(const float *e, float *s)
+{
+    float a = e[0] * e[0];
+#pragma unroll
+    for (int i = 1; i < W; ++i) a = fmaf(e[i], e[i], a);
+    s[0] = a;
+#pragma unroll
+    for (int i = 1; i < N; ++i) {
+        a = fmaf(-e[i - 1], e[i - 1], fmaf(e[i + W - 1], e[i + W - 1], a));
+        s[i] = a;
+    }
+}
+
 }  // namespace gnlm

@@ -46,6 +46,7 @@ def lib():
     L.gnlm2d_forward_ex.argtypes = [vp, i, ll, vp, vp, i, i, i, i, f, i, vp]
     L.gnlm_u8_to_f32.argtypes = [vp, vp, ll, vp]
     L.gnlm3d_forward_u8.argtypes = [vp, i, ll, ll, vp, vp, i, i, i, i, i, f, vp]
+    L.gnlm2d_forward_u8.argtypes = [vp, i, ll, vp, vp, i, i, i, i, f, vp]
     L.gnlm_nl_means_3d_u8.argtypes = [vp, vp, i, i, i, i, i, f]
     L.gnlm_nl_means_2d_u8.argtypes = [vp, vp, i, i, i, i, i, i, f]
     L.gnlm_phantom3d.argtypes = [vp, vp, i, i, i, i, f, f, f, f, ctypes.c_ulonglong]
@@ -53,7 +54,7 @@ def lib():
     L.gnlm_hash_f32.restype = ctypes.c_ulonglong
     L.gnlm_hash_f32.argtypes = [vp, ll]
     for name in ("gnlm3d_forward", "gnlm3d_forward_ex", "gnlm2d_forward", "gnlm2d_forward_ex",
-                 "gnlm_u8_to_f32", "gnlm3d_forward_u8", "gnlm_nl_means_3d_u8", "gnlm_nl_means_2d_u8", "gnlm_phantom3d",
+                 "gnlm_u8_to_f32", "gnlm3d_forward_u8", "gnlm2d_forward_u8", "gnlm_nl_means_3d_u8", "gnlm_nl_means_2d_u8", "gnlm_phantom3d",
                  "gnlm_phantom2d"):
         getattr(L, name).restype = i
     _extend(L)
@@ -325,6 +326,29 @@ def forward_u8_route(nx, ny, nz, r_p, r_s):
     k = ctypes.c_int(-1)
     st = lib().gnlm3d_forward_u8_route(int(nx), int(ny), int(nz), int(r_p), int(r_s), ctypes.byref(k))
     return st, k.value
+
+
+def nlm2d_u8(u8, r_p, r_s, h, want_Z=False, out=None):
+    """2D NLM of an 8-bit image u8[y, x] (uint8 CUDA tensor), reflect-101
+    borders, h on the 0-255 scale; returns float32 (gnlm2d_forward_u8; the
+    same bits as nlm2d on the image as floats)."""
+    import torch
+    _check_vol(u8, 2, "u8")
+    if not (isinstance(u8, torch.Tensor) and u8.is_cuda and u8.dtype == torch.uint8 and u8.is_contiguous()):
+        raise TypeError("u8 must be a contiguous uint8 CUDA tensor")
+    ny, nx = u8.shape
+    if out is not None:
+        _check_like(out, u8, "out", "u8")
+        _check_dev([out], ["out"])
+        if out.device != u8.device:
+            raise ValueError(f"out is on {out.device}, u8 on {u8.device}")
+    y = torch.empty(u8.shape, dtype=torch.float32, device=u8.device) if out is None else out
+    Z = torch.empty_like(y) if want_Z else None
+    st = lib().gnlm2d_forward_u8(ctypes.c_void_p(u8.data_ptr()), 0, 0, _dev_f32(y, "out"),
+                                 _dev_f32(Z, "Z") if Z is not None else None,
+                                 nx, ny, int(r_p), int(r_s), float(h), _stream(u8))
+    check(st, "gnlm2d_forward_u8")
+    return (y, Z) if want_Z else y
 
 
 def nlm2d(u, r_p, r_s, h, want_Z=False, path=PATH_AUTO):

@@ -132,6 +132,18 @@ int gnlm2d_forward(const float *d_in, float *d_out, int nx, int ny, int r_p, int
 int gnlm2d_forward_ex(const float *d_in, int in_pad, long long in_sy, float *d_out, float *d_Z,
                       int nx, int ny, int r_p, int r_s, float h, int path, void *stream);
 
+/* 2D forward of an 8-bit image, the reference's input type: its launcher
+ * takes the padded unsigned char image ("2D NLM/11x11/main.cu:472-539"), h
+ * on the 0-255 scale.  Layout as gnlm2d_forward_ex (in_pad = 0: unpadded,
+ * reflect-101 on the device; in_pad >= r_p + r_s: padded, row stride in_sy
+ * bytes).  The tiled kernel (r_p <= 5) reads the bytes itself; every partial
+ * patch sum of squared 8-bit differences is an exact integer in FP32, so its
+ * box sums run as sliding windows, and the result is bit-identical to
+ * gnlm2d_forward_ex on the same samples as floats.  Other radii run the FP32
+ * path on a float copy made in a stream-ordered workspace. */
+int gnlm2d_forward_u8(const unsigned char *d_in, int in_pad, long long in_sy, float *d_out, float *d_Z,
+                      int nx, int ny, int r_p, int r_s, float h, void *stream);
+
 /* 3D adjoint (no counterpart in the reference, which has no backward pass;
  * it differentiates the forward above).  Given the forward's input d_in,
  * output d_out and normaliser d_Z (from gnlm3d_forward) and d_gout = dL/dy,

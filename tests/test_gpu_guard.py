@@ -147,6 +147,21 @@ def test_2d_guard_bands(dev, shape, rp, rs):
     assert torch.equal(gy.view(shape), y_ref)
 
 
+@pytest.mark.parametrize("shape,rp,rs", [((77, 200), 5, 10), ((130, 33), 2, 5), ((40, 41), 4, 3), ((31, 20), 6, 2)])
+def test_2d_u8_guard_bands(dev, shape, rp, rs):
+    img = np.random.default_rng(sum(shape) + rp).integers(0, 256, shape, dtype=np.uint8)
+    ny, nx = shape
+    print(f"case guard 2d u8 shape={shape} rp={rp} rs={rs}", flush=True)
+    gi = _guarded_input(dev, img, fill=255, dtype=torch.uint8)
+    gy, gz = Guarded(dev, img.size), Guarded(dev, img.size)
+    st = gnlm.lib().gnlm2d_forward_u8(gi.ptr(), 0, 0, gy.ptr(), gz.ptr(), nx, ny, rp, rs, 20.0, _stream(dev))
+    torch.cuda.synchronize()
+    assert st == gnlm.OK, st
+    assert gi.margins_intact() and gy.margins_intact() and gz.margins_intact()
+    y_ref, Z_ref = gnlm.nlm2d_u8(torch.from_numpy(img).to(dev), rp, rs, 20.0, want_Z=True)
+    assert torch.equal(gy.view(shape), y_ref) and torch.equal(gz.view(shape), Z_ref)
+
+
 @pytest.mark.parametrize("shape,rp,rs", [((15, 13, 17), 1, 3), ((11, 19, 9), 2, 3), ((9, 8, 10), 0, 2),
                                          ((12, 10, 11), 3, 2), ((7, 8, 6), 4, 1)])
 def test_backward_guard_bands(dev, shape, rp, rs):
