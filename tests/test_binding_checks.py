@@ -65,3 +65,16 @@ def test_autograd_checks():
         gnlm.nlm3d_autograd(_t(8, 8), 0.1, 1, 2)
     with pytest.raises(TypeError):
         gnlm.nlm3d_autograd(_t(8, 8, 8), 0.1, 1, 2)
+
+
+@pytest.mark.parametrize("fn,ndim", [(gnlm.nlm3d_u8, 3), (gnlm.nlm2d_u8, 2)])
+def test_u8_checks(fn, ndim):
+    u8 = lambda *s: torch.zeros(s, dtype=torch.uint8)  # noqa: E731
+    with pytest.raises(ValueError):
+        fn(u8(*([5] * (ndim + 1))), 1, 2, 10.0)             # wrong rank
+    with pytest.raises(ValueError):
+        fn(u8(*([0] + [5] * (ndim - 1))), 1, 2, 10.0)       # empty
+    with pytest.raises(TypeError):
+        fn(_t(*([5] * ndim)), 1, 2, 10.0)                    # float32, not uint8
+    with pytest.raises(TypeError):
+        fn(u8(*([5] * ndim)), 1, 2, 10.0)                    # right shape, not on the GPU
