@@ -72,6 +72,7 @@ int fwd3d_pick_variant(int r_p, int r_s);
 int fwd3d_pick_shape(const InDesc &in, float *out, float *Zout, int r_p, int r_s, float k, int variant,
                      hipStream_t s, bool may_tune);
 bool fwd3d_tuning_lazy();  // gnlm_set_tuning(GNLM_TUNE_LAZY) in force
+void fwd3d_set_dz_split(bool on);  // gnlm_set_dz_split
 
 // ---- tiled 2D forward (nlm2d_fwd.hip) --------------------------------------
 bool fwd2d_tiled_supported(int r_p, int r_s);

@@ -229,6 +229,13 @@ int gnlm_set_backward_form(int form) { return bwd3d_set_form(form) == 0 ? GNLM_O
 
 int gnlm_set_backward_parts(int nparts) { return bwd3d_set_parts(nparts) == 0 ? GNLM_OK : GNLM_EINVAL; }
 
+int gnlm_set_dz_split(int enable)
+{
+    if (enable != 0 && enable != 1) return GNLM_EINVAL;
+    fwd3d_set_dz_split(enable == 1);
+    return GNLM_OK;
+}
+
 void gnlm_reset_tuning(void)
 {
     std::lock_guard<std::mutex> lk(g_mu);

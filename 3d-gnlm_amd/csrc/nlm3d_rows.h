@@ -286,6 +286,17 @@ struct RowsTile {
     int x0, y0, z0;
 };
 
+// A dz-split launch (small volumes: several workgroups per tile, one per
+// range of dz): the workgroup visits dz in [dzlo, dzhi] only and writes each
+// dz's partial sums into slot dz + r_s of `slots` (slot j: Y at slots + 2 j
+// nvox, Z at slots + (2 j + 1) nvox, voxel-indexed); a combine pass then
+// forms 0 + partial(-r_s) + partial(-r_s + 1) + ... in dz order -- the
+// additions a single workgroup makes -- so the output bits are the same.
+struct RowsSplit {
+    float *slots;
+    int dzlo, dzhi;
+};
+
 // phase 1 for the offset whose first shifted sample is sh[SH]
 template <class C, int SH>
 __device__ __forceinline__ void rows_phase1(const float (&cen)[C::ROWS][C::EX], const float (&sh)[C::ROWS][C::NS],
@@ -350,9 +361,10 @@ __device__ __forceinline__ void rows_phase2(const float *src, float *dst)
     }
 }
 
-template <class C>
-__device__ __forceinline__ void rows_tile(const RowsTile &ta, float *lds)
+template <class C, bool SPLIT = false>
+__device__ __forceinline__ void rows_tile(const RowsTile &ta, float *lds, RowsSplit sp = RowsSplit{nullptr, 0, 0})
 {
+    static_assert(!SPLIT || C::REG_TOTALS, "dz-split launches are for the register-total tiles");
     const RowsTile a = ta;  // registers: no reloads through a generic pointer in the loops
     constexpr int RP = C::RP, RS = C::RS, RZ = C::RZ, NT = C::NT;
     float *SX = lds;
@@ -516,7 +528,8 @@ __device__ __forceinline__ void rows_tile(const RowsTile &ta, float *lds)
                 Zt[k][c][o] = 0.0f;
                 Yt[k][c][o] = 0.0f;
             }
-    for (int dz = -RS; dz <= RS; ++dz) {
+    const int dzlo = SPLIT ? sp.dzlo : -RS, dzhi = SPLIT ? sp.dzhi : RS;
+    for (int dz = dzlo; dz <= dzhi; ++dz) {
         float Zd[C::P3R][V3][RZ], Yd[C::P3R][V3][RZ];
 #pragma unroll
         for (int k = 0; k < C::P3R; ++k)
@@ -599,8 +612,31 @@ __device__ __forceinline__ void rows_tile(const RowsTile &ta, float *lds)
             });
         }
         // close the dz block: totals += partials (old total + partial, in
-        // this order, in registers or in the output buffers)
-        if constexpr (C::REG_TOTALS) {
+        // this order, in registers or in the output buffers) -- or, split,
+        // the partials into their slot
+        if constexpr (SPLIT) {
+            const long long nvox = (long long)a.nx * a.ny * a.nz;
+            gfloat *ys = (gfloat *)sp.slots + 2LL * (dz + RS) * nvox, *zs = ys + nvox;
+#pragma unroll
+            for (int k = 0; k < C::P3R; ++k) {
+                const int it = tid + k * NT;
+                const int x = (it % NXG) * V3, y = (it / NXG) % C::TY, zr = it / (NXG * C::TY);
+                const int gy = a.y0 + y, gz = a.z0 + zr * RZ;
+                if (!p3ok[k] || it >= C::P3N || gy >= a.ny) continue;
+#pragma unroll
+                for (int c = 0; c < V3; ++c) {
+                    const int gx = a.x0 + x + c;
+                    if (gx >= a.nx) continue;
+#pragma unroll
+                    for (int o = 0; o < RZ; ++o) {
+                        if (gz + o >= a.nz) continue;
+                        const long long oi = gx + (long long)a.nx * (gy + (long long)a.ny * (gz + o));
+                        ys[oi] = Yd[k][c][o];
+                        zs[oi] = Zd[k][c][o];
+                    }
+                }
+            }
+        } else if constexpr (C::REG_TOTALS) {
 #pragma unroll
             for (int k = 0; k < C::P3R; ++k)
 #pragma unroll
@@ -614,7 +650,7 @@ __device__ __forceinline__ void rows_tile(const RowsTile &ta, float *lds)
             store(Zd, Yd, dz != -RS, dz == RS);
         }
     }
-    if constexpr (C::REG_TOTALS) store(Zt, Yt, false, true);
+    if constexpr (C::REG_TOTALS && !SPLIT) store(Zt, Yt, false, true);
 }
 
 }  // namespace gnlm

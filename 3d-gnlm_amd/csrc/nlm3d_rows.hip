@@ -20,6 +20,15 @@ __global__ __launch_bounds__(kRowsThreads) __attribute__((amdgpu_waves_per_eu(WA
     rows_kernel_body<RowsCfg<RP, RS, SHORT>>(a, lds);
 }
 
+// The dz-split form (small volumes; same tile, same bits).
+template <int RP, int RS, bool SHORT, int WAVES>
+__global__ __launch_bounds__(kRowsThreads) __attribute__((amdgpu_waves_per_eu(WAVES))) void nlm3d_fwd_rows_split(
+    RowsArgs a, RowsSplitArgs sa)
+{
+    extern __shared__ __align__(16) float lds[];
+    rows_kernel_body_split<RowsCfg<RP, RS, SHORT>>(a, sa, lds);
+}
+
 // Deep tiles (RowsCfg ZM = 2): 512 threads, two workgroups per CU at four
 // waves per SIMD (128 VGPRs).
 template <int RP, int RS, bool SHORT>
@@ -50,6 +59,30 @@ bool rows_rp1_four_per_cu(long long tiles)
     return r3 <= 4 && 4 * r4 < 3 * r3;
 }
 
+// Workgroups of a dz-split launch per tile (1: no split).  The split is for
+// volumes with fewer tiles than the GPU holds at once (the 96 x 96 x 48
+// configuration: 168 tiles for 768 workgroup slots): each tile is spread
+// over up to 2 r_s + 1 workgroups, as many as fill the slots in one round.
+// Not with a caller's workspace (it has no room for the slots), nor when the
+// slots would pass 256 MiB.
+std::atomic<bool> g_dz_split{true};  // gnlm_set_dz_split
+
+template <class C>
+int rows_split_parts(const InDesc &in, const Workspace *ws, int waves)
+{
+    if (ws || !g_dz_split.load(std::memory_order_relaxed)) return 1;
+    constexpr int ND = 2 * C::RS + 1;
+    const long long nv = (long long)in.nx * in.ny * in.nz;
+    if (8LL * ND * nv > (256LL << 20)) return 1;
+    const long long tiles = (long long)((in.nx + C::TX - 1) / C::TX) * ((in.ny + C::TY - 1) / C::TY) *
+                            ((in.nz + C::TZ - 1) / C::TZ);
+    const long long by_lds = (long long)kLdsLimitBytes / (long long)rows_lds_cfg<C>();
+    const long long per_cu = by_lds < waves ? by_lds : waves;
+    long long n = (long long)device_cus() * per_cu / tiles;
+    if (n > ND) n = ND;
+    return n < 2 ? 1 : (int)n;
+}
+
 // Launch shapes of one tile family: builds and tile depths that give the
 // same bits (the autotuner times them at a volume's own size).
 //   r_p = 1:        0 = 168 VGPRs, 3 tiles per CU; 1 = 128 VGPRs, 4 per CU;
@@ -58,15 +91,21 @@ bool rows_rp1_four_per_cu(long long tiles)
 //   r_p = 3:        0 = 128 VGPRs (4 per CU, spills outside the offset loop);
 //                   1 = 168 VGPRs (3 per CU, no spills)
 //   r_p = 4:        0 = 16 x 8 x 8, 4 per CU; 1 = deep 16 x 8 x 16, 2 per CU
+//   r_p 1-3, last:  the dz split of shape 0's build where rows_split_parts
+//                   allows one (small volumes), else shape 0 itself; the
+//                   default rule (no recorded shape) takes it wherever
+//                   rows_split_parts allows it: the split at least doubles
+//                   the workgroups of a launch that fills under half the GPU
 // (deep 16 x 10 x 20 tiles for r_p = 3 give the same bits too but ran 2-8%
 // slower than both builds at 96^3-256^3, so they are not a candidate)
 // (r_p = 4's big 16 x 24 x 24 tile is launched from the variant dispatch.)
 // shape < 0: the default rule below.
 int rows_num_shapes(int r_p, bool short_tiles)
 {
-    if (r_p == 1) return 3;
-    if (r_p == 2 && short_tiles) return 2;
-    if (r_p == 3) return 2;
+    if (r_p == 1) return 4;
+    if (r_p == 2 && short_tiles) return 3;
+    if (r_p == 2) return 2;
+    if (r_p == 3) return 3;
     if (r_p == 4) return 2;
     return 1;
 }
@@ -76,6 +115,15 @@ hipError_t rows_launch_t(const InDesc &in, float *out, float *Zout, float k, hip
                          int shape)
 {
     using C = RowsCfg<RP, RS, SHORT>;
+    if constexpr (RP <= 3) {
+        if (shape < 0 || shape == rows_num_shapes(RP, SHORT) - 1) {
+            constexpr int W0 = rows_waves<RP, SHORT>();
+            const int ns = rows_split_parts<C>(in, ws, W0);
+            if (ns >= 2)
+                return rows_launch_split_cfg<C>(in, out, Zout, k, s, nlm3d_fwd_rows_split<RP, RS, SHORT, W0>, ns);
+            if (shape >= 0) shape = 0;
+        }
+    }
     if constexpr (RP == 1) {
         if (shape < 0) {
             const long long tiles = (long long)((in.nx + C::TX - 1) / C::TX) * ((in.ny + C::TY - 1) / C::TY) *
@@ -145,6 +193,8 @@ size_t fwd3d_rows_lds_bytes(int r_p, int r_s, bool short_tiles)
 }
 
 int fwd3d_rows_num_shapes(int r_p, bool short_tiles) { return rows_num_shapes(r_p, short_tiles); }
+
+void fwd3d_set_dz_split(bool on) { g_dz_split.store(on, std::memory_order_relaxed); }
 
 hipError_t fwd3d_rows_launch(const InDesc &in, float *out, float *Zout, int r_p, int r_s, float k, bool short_tiles,
                              hipStream_t s, const Workspace *ws, int shape)

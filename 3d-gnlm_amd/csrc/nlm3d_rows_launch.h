@@ -42,6 +42,54 @@ __device__ __forceinline__ void rows_kernel_body(const RowsArgs &a, float *lds)
     rows_tile<C>(tl, lds);
 }
 
+// dz-split launch: workgroup b handles tile b % ntiles over the dz range of
+// part b / ntiles (nsplit near-equal ranges of the 2 r_s + 1 planes of
+// offsets), writing per-dz partial sums into slots (RowsSplit).
+struct RowsSplitArgs {
+    float *slots;
+    int nsplit;
+};
+
+template <class C>
+__device__ __forceinline__ void rows_kernel_body_split(const RowsArgs &a, const RowsSplitArgs &sa, float *lds)
+{
+    const int ntiles = a.ntx * a.nty * a.ntz;
+    const int part = blockIdx.x / ntiles, t = blockIdx.x % ntiles;
+    constexpr int ND = 2 * C::RS + 1;
+    RowsTile tl;
+    tl.v = a.v;
+    tl.out = a.out;
+    tl.Zacc = a.Zacc;
+    tl.nx = a.nx;
+    tl.ny = a.ny;
+    tl.nz = a.nz;
+    tl.k = a.k;
+    tl.x0 = (t % a.ntx) * C::TX;
+    tl.y0 = ((t / a.ntx) % a.nty) * C::TY;
+    tl.z0 = (t / (a.ntx * a.nty)) * C::TZ;
+    RowsSplit sp;
+    sp.slots = sa.slots;
+    sp.dzlo = -C::RS + part * ND / sa.nsplit;
+    sp.dzhi = -C::RS + (part + 1) * ND / sa.nsplit - 1;
+    rows_tile<C, true>(tl, lds, sp);
+}
+
+// The split launch's totals: for each voxel 0 + slot 0 + slot 1 + ... in dz
+// order (the additions of the single-workgroup tile), then y = Y / Z.
+__global__ void rows_split_combine(const float *slots, int nd, long long nvox, float *out, float *Zout)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nvox;
+         i += (long long)gridDim.x * blockDim.x) {
+        float Z = 0.0f, Y = 0.0f;
+        for (int j = 0; j < nd; ++j) {
+            Z = Z + slots[(2LL * j + 1) * nvox + i];
+            Y = Y + slots[2LL * j * nvox + i];
+        }
+        if (Zout) Zout[i] = Z;
+        out[i] = Y / Z;
+    }
+}
+
 // Fill the padded copy: reflect-101 samples on [-R, n + R) in every axis
 // (or the caller's own halo when the input arrives padded), 0 beyond.
 // Grid rows walk the planes, so the index arithmetic within a plane is
@@ -125,6 +173,68 @@ hipError_t rows_launch_cfg(const InDesc &in, float *out, float *Zout, float k, h
         }
     }
     const hipError_t e2 = gws ? hipSuccess : hipFreeAsync(ws, s);
+    return e != hipSuccess ? e : e2;
+}
+
+// The dz-split form of rows_launch_cfg (register-total tiles): padded copy
+// and 2 (2 r_s + 1) slots per voxel in one stream-ordered workspace, the
+// split kernel on ntiles x nsplit workgroups, then the combine.
+template <class C>
+hipError_t rows_launch_split_cfg(const InDesc &in, float *out, float *Zout, float k, hipStream_t s,
+                                 void (*kernel)(RowsArgs, RowsSplitArgs), int nsplit)
+{
+    static_assert(C::REG_TOTALS, "register-total tiles");
+    constexpr int ND = 2 * C::RS + 1;
+    if (nsplit < 2 || nsplit > ND) return hipErrorInvalidValue;
+    RowsArgs a;
+    a.nx = in.nx;
+    a.ny = in.ny;
+    a.nz = in.nz;
+    a.k = k;
+    a.out = out;
+    a.Zacc = Zout;
+    a.ntx = (in.nx + C::TX - 1) / C::TX;
+    a.nty = (in.ny + C::TY - 1) / C::TY;
+    a.ntz = (in.nz + C::TZ - 1) / C::TZ;
+    const int px = a.ntx * C::TX + 2 * C::LP;
+    const int py = a.nty * C::TY + 2 * C::R;
+    const int pz = a.ntz * C::TZ + 2 * C::R;
+    const long long npad = (long long)px * py * pz;
+    const long long nv = (long long)in.nx * in.ny * in.nz;
+    const long long ntiles = (long long)a.ntx * a.nty * a.ntz;
+    if (npad >= (1LL << 30) || nv >= (1LL << 31) || ntiles * nsplit >= (1LL << 31)) return hipErrorInvalidValue;
+    // the slots start on a 256-byte boundary after the padded copy
+    const long long soff = (npad + 63) / 64 * 64;
+    const size_t bytes = sizeof(float) * (size_t)(soff + 2LL * ND * nv);
+    float *ws = nullptr;
+    hipError_t e = hipMallocAsync((void **)&ws, bytes, s);
+    if (e != hipSuccess) return e;
+    a.v.p = ws;
+    a.v.px = px;
+    a.v.pxy = px * py;
+    a.v.bytes = (unsigned)(sizeof(float) * npad);
+    RowsSplitArgs sa;
+    sa.slots = ws + soff;
+    sa.nsplit = nsplit;
+    const int gx = (px * py + 255) / 256 < 1024 ? (px * py + 255) / 256 : 1024;
+    const int gy = pz < 65535 ? pz : 65535;
+    hipLaunchKernelGGL(rows_pad, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, s, in, ws, px, py, pz, C::LP, C::R);
+    e = hipGetLastError();
+    if (e == hipSuccess) {
+        const size_t lds = rows_lds_cfg<C>();
+        e = hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(kernel, dim3((unsigned)(ntiles * nsplit)), dim3(C::NT), lds, s, a, sa);
+            e = hipGetLastError();
+        }
+    }
+    if (e == hipSuccess) {
+        const long long g = (nv + 255) / 256 < 4096 ? (nv + 255) / 256 : 4096;
+        hipLaunchKernelGGL(rows_split_combine, dim3((unsigned)g), dim3(256), 0, s, (const float *)sa.slots, ND, nv, out,
+                           Zout);
+        e = hipGetLastError();
+    }
+    const hipError_t e2 = hipFreeAsync(ws, s);
     return e != hipSuccess ? e : e2;
 }
 

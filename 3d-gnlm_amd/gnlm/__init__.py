@@ -47,6 +47,8 @@ def lib():
     L.gnlm_u8_to_f32.argtypes = [vp, vp, ll, vp]
     L.gnlm3d_forward_u8.argtypes = [vp, i, ll, ll, vp, vp, i, i, i, i, i, f, vp]
     L.gnlm2d_forward_u8.argtypes = [vp, i, ll, vp, vp, i, i, i, i, f, vp]
+    L.gnlm_set_dz_split.argtypes = [i]
+    L.gnlm_set_dz_split.restype = i
     L.gnlm_nl_means_3d_u8.argtypes = [vp, vp, i, i, i, i, i, f]
     L.gnlm_nl_means_2d_u8.argtypes = [vp, vp, i, i, i, i, i, i, f]
     L.gnlm_phantom3d.argtypes = [vp, vp, i, i, i, i, f, f, f, f, ctypes.c_ulonglong]
@@ -235,6 +237,12 @@ def set_backward_parts(nparts):
     """Workgroups per tile of the symmetric adjoint: 0 = automatic, 1-4 forced
     (gnlm_set_backward_parts)."""
     check(lib().gnlm_set_backward_parts(int(nparts)), "gnlm_set_backward_parts")
+
+
+def set_dz_split(enable):
+    """dz-split forward of small volumes on (True, the default) or off
+    (gnlm_set_dz_split; same bits either way)."""
+    check(lib().gnlm_set_dz_split(1 if enable else 0), "gnlm_set_dz_split")
 
 
 def check(status, where=""):

@@ -162,7 +162,8 @@ def _median_time(fn, runs=3):
 def batch_summary(r):
     """The batched-workload object carried on the headline line."""
     return {"workload": "nlm3d_batch64", "ratio": r["value"], "plain_ms": r["plain_ms"],
-            "batched_ms": r["batched_ms"], "bit_identical": r["bit_identical"], "steps": r["steps"],
+            "batched_ms": r["batched_ms"], "ratio_vs_unsplit": r["ratio_vs_unsplit"],
+            "plain_unsplit_ms": r["plain_unsplit_ms"], "bit_identical": r["bit_identical"], "steps": r["steps"],
             "warmup": r["warmup"], "tiles": r["config"]["tiles"], "workgroups": r["config"]["workgroups"],
             "target": ">= 1.5x per-volume launches, identical outputs"}
 
@@ -224,13 +225,16 @@ def bench_batch(args):
     """64 volumes of uneven size (dims from {32, 48, ..., 192} per axis),
     ellipsoid phantoms with their own sigma ~ U[0.02, 0.10] and h = sigma,
     r_p in {1, 2}, r_s in {3, 5}: one graph-replayed persistent launch vs 64
-    per-volume launches; outputs must be bit-identical."""
+    per-volume launches; outputs must be bit-identical.  The per-volume
+    launches are timed twice: as the library runs them (small volumes take
+    the dz split, gnlm_set_dz_split) and with the split off (one workgroup per
+    tile, as the batched launch runs every tile)."""
     import torch
     import gnlm
     dev = torch.device("cuda:0")
     rng = np.random.default_rng(20261208)
     dims_set = np.arange(32, 193, 16)
-    vols, outs_a, outs_b, pairs = [], [], [], 0
+    vols, outs_a, outs_b, outs_c, pairs = [], [], [], [], 0
     for i in range(64):
         nx, ny, nz = (int(v) for v in rng.choice(dims_set, 3))
         sigma = float(rng.uniform(0.02, 0.10))
@@ -241,6 +245,7 @@ def bench_batch(args):
         vols.append((t, rp, rs, sigma))
         outs_a.append(torch.empty_like(t))
         outs_b.append(torch.empty_like(t))
+        outs_c.append(torch.empty_like(t))
         pairs += nx * ny * nz * (2 * rs + 1) ** 3
     plan = gnlm.BatchPlan([(u, o, rp, rs, h) for (u, rp, rs, h), o in zip(vols, outs_b)])
     ntiles, nblocks = plan.info()
@@ -249,20 +254,32 @@ def bench_batch(args):
         for (u, rp, rs, h), o in zip(vols, outs_a):
             gnlm.nlm3d(u, rp, rs, h, out=o)
 
+    def plain_unsplit():
+        for (u, rp, rs, h), o in zip(vols, outs_c):
+            gnlm.nlm3d(u, rp, rs, h, out=o)
+
     def batched():
         plan.run()
 
     res = {}
-    for name, fn in (("plain", plain), ("batched", batched)):
-        for _ in range(args.warmup):
-            fn()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            fn()
-        torch.cuda.synchronize()
-        res[name] = (time.perf_counter() - t0) / args.steps
-    identical = all(torch.equal(a, b) for a, b in zip(outs_a, outs_b))
+    try:
+        for name, fn in (("plain", plain), ("batched", batched), ("plain_unsplit", plain_unsplit)):
+            if name == "plain_unsplit":
+                # re-tune the launch shapes without the split
+                gnlm.set_dz_split(False)
+                gnlm.reset_tuning()
+            for _ in range(max(args.warmup, 1)):
+                fn()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                fn()
+            torch.cuda.synchronize()
+            res[name] = (time.perf_counter() - t0) / args.steps
+    finally:
+        gnlm.set_dz_split(True)
+        gnlm.reset_tuning()
+    identical = all(torch.equal(a, b) and torch.equal(a, c) for a, b, c in zip(outs_a, outs_b, outs_c))
     nvox = sum(u.numel() for u, *_ in vols)
     return {
         "metric": "batched persistent speedup vs 64 per-volume launches",
@@ -274,6 +291,8 @@ def bench_batch(args):
         "config": {"workload": "nlm3d_batch64", "volumes": 64, "voxels": nvox, "pairs": pairs, "tiles": ntiles,
                    "workgroups": nblocks},
         "plain_ms": round(res["plain"] * 1e3, 4), "batched_ms": round(res["batched"] * 1e3, 4),
+        "plain_unsplit_ms": round(res["plain_unsplit"] * 1e3, 4),
+        "ratio_vs_unsplit": round(res["plain_unsplit"] / res["batched"], 4),
         "batched_Mvox_s": round(nvox / res["batched"] / 1e6, 2), "bit_identical": identical,
     }
 

@@ -183,6 +183,17 @@ int gnlm_set_backward_form(int form);
  * 0-4. */
 int gnlm_set_backward_parts(int nparts);
 
+/* dz-split forward of small volumes (no counterpart in the reference: its
+ * launcher runs one thread block per tile of the volume, 3DNLM/main.cu:15-21),
+ * process-wide.  With 1 (default), a rows-tile forward (r_p 1-3) whose tiles
+ * fill less than half of the GPU's workgroup slots spreads each tile over
+ * several workgroups, one per range of dz, which write per-dz partial sums
+ * into the stream-ordered workspace (8 (2 r_s + 1) bytes per voxel, at most
+ * 256 MiB); a combine pass adds them in dz order, the additions one workgroup
+ * makes, so the output bits are those of the unsplit launch.  0 turns it off
+ * (the unsplit launch).  GNLM_EINVAL for other values. */
+int gnlm_set_dz_split(int enable);
+
 /* ---- batched persistent forward --------------------------------------
  * No counterpart in the reference (its launcher processes one compile-time
  * volume per call, 3DNLM/main.cu:7607-7680).  Many volumes of uneven size,

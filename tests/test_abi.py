@@ -66,6 +66,9 @@ def test_invalid_arguments_rejected_without_gpu():
     out = (ctypes.c_float * 16)()
     assert L.gnlm_nl_means_3d_u8(buf, out, 4, 4, 4, 4, 3, 10.0) == gnlm.EINVAL
     assert L.gnlm_nl_means_2d_u8(buf, out, 4, 4, 3, 4, 2, 2, 10.0) == gnlm.EINVAL
+    # the dz-split switch takes 0 or 1
+    assert L.gnlm_set_dz_split(2) == gnlm.EINVAL and L.gnlm_set_dz_split(-1) == gnlm.EINVAL
+    assert L.gnlm_set_dz_split(1) == gnlm.OK
     # the 8-bit 2D entry: NULL input or output, bad h, padding smaller than the halo
     assert L.gnlm2d_forward_u8(None, 0, 0, fake, None, 8, 8, 1, 2, 10.0, None) == gnlm.EINVAL
     assert L.gnlm2d_forward_u8(fake, 0, 0, None, None, 8, 8, 1, 2, 10.0, None) == gnlm.EINVAL

@@ -100,3 +100,32 @@ def test_batch_plan_tuning_keeps_bits(dev):
             plan.close()
     finally:
         gnlm.set_tuning(gnlm.TUNE_LAZY)
+
+
+@pytest.mark.parametrize("shape3,rp,rs", [((48, 96, 96), 1, 3), ((64, 64, 64), 2, 5), ((17, 30, 23), 3, 4),
+                                          ((40, 40, 40), 2, 3), ((9, 70, 21), 1, 2)])
+def test_dz_split_same_bits(dev, shape3, rp, rs):
+    """The dz split of small volumes (several workgroups per tile, each
+    writing per-dz partial sums; a combine pass adds them in dz order) gives
+    the unsplit launch's bits, y and Z, under the default rule and tuned."""
+    u = np.random.default_rng(sum(shape3) * 3 + rp).random(shape3, dtype=np.float32)
+    ut = torch.from_numpy(u).to(dev)
+    outs = []
+    try:
+        for split in (True, False):
+            gnlm.set_dz_split(split)
+            for mode in (gnlm.TUNE_OFF, gnlm.TUNE_LAZY):
+                gnlm.reset_tuning()
+                gnlm.set_tuning(mode)
+                print(f"case dz split={split} tuning={mode} dims={shape3} rp={rp} rs={rs}", flush=True)
+                y, Z = gnlm.nlm3d(ut, rp, rs, 0.15, want_Z=True)
+                y2 = gnlm.nlm3d(ut, rp, rs, 0.15)
+                outs.append((y.cpu(), Z.cpu(), y2.cpu()))
+    finally:
+        gnlm.set_dz_split(True)
+        gnlm.set_tuning(gnlm.TUNE_LAZY)
+        gnlm.reset_tuning()
+    for y, Z, y2 in outs:
+        assert torch.equal(y, outs[0][0]) and torch.equal(Z, outs[0][1]) and torch.equal(y2, outs[0][0])
+    ref64 = oracle.nlm3d_brute(u.astype(np.float64), rp, rs, 0.15)
+    assert max_rel(outs[0][0].numpy(), ref64) <= 1e-5, report(outs[0][0].numpy(), ref64)
