@@ -34,6 +34,10 @@ hipError_t fwd3d_tiled_launch_variant(const InDesc &in, float *out, float *Zout,
 // Launch shapes of a variant for (r_p, r_s): the bit-identical alternatives the
 // autotuner chooses between (1 where there is no choice).
 int fwd3d_num_shapes(int r_p, int r_s, int variant);
+// Whether shape runs anything other than shape 0 on this volume (the dz-split
+// shape falls back to shape 0 where the split does not apply): the tuner
+// times only the shapes that differ.
+bool fwd3d_shape_distinct(const InDesc &in, int r_p, int r_s, int variant, int shape);
 // Whether a rows-family variant's padded copy of an nx*ny*nz volume fits its
 // 32-bit offsets.
 bool fwd3d_rows_fits(int nx, int ny, int nz, int r_p, int r_s, int variant);
@@ -60,6 +64,8 @@ hipError_t fwd3d_rows_launch(const InDesc &in, float *out, float *Zout, int r_p,
 // Launch shapes of the rows tiles for (r_p, short): bit-identical builds and
 // tile depths (nlm3d_rows.hip lists them); shape -1 = the default rule.
 int fwd3d_rows_num_shapes(int r_p, bool short_tiles);
+// The dz split would run for this volume (no caller workspace).
+bool fwd3d_rows_split_applies(const InDesc &in, int r_p, int r_s, bool short_tiles);
 
 // ---- autotuner (gnlm_tune.cpp) ---------------------------------------------
 // Variant for (r_p, r_s): the tuned one if gnlm_autotune ran, otherwise the

@@ -119,6 +119,7 @@ int fwd3d_pick_shape(const InDesc &in, float *out, float *Zout, int r_p, int r_s
     int best = -1;
     float best_ms = 0.0f;
     for (int c = 0; c < n; ++c) {
+        if (!fwd3d_shape_distinct(in, r_p, r_s, variant, c)) continue;  // runs shape 0 here
         hipError_t e = fwd3d_tiled_launch_variant(in, out, Zout, r_p, r_s, k, variant, s, nullptr, c);
         if (e == hipSuccess) e = hipEventRecord(e0, s);
         for (int r = 0; r < 2 && e == hipSuccess; ++r)

@@ -168,6 +168,15 @@ int fwd3d_num_shapes(int r_p, int r_s, int variant)
     return 1;
 }
 
+bool fwd3d_shape_distinct(const InDesc &in, int r_p, int r_s, int variant, int shape)
+{
+    // the last shape of the r_p 1-3 rows families is the dz split
+    if (shape > 0 && fwd3_is_rows(variant) && r_p >= 1 && r_p <= 3 &&
+        shape == fwd3d_rows_num_shapes(r_p, variant == kVRowsShort) - 1)
+        return fwd3d_rows_split_applies(in, r_p, r_s, variant == kVRowsShort);
+    return true;
+}
+
 hipError_t fwd3d_tiled_launch_variant(const InDesc &in, float *out, float *Zout, int r_p, int r_s, float k,
                                       int variant, hipStream_t s, const Workspace *ws, int shape)
 {

@@ -196,6 +196,18 @@ int fwd3d_rows_num_shapes(int r_p, bool short_tiles) { return rows_num_shapes(r_
 
 void fwd3d_set_dz_split(bool on) { g_dz_split.store(on, std::memory_order_relaxed); }
 
+bool fwd3d_rows_split_applies(const InDesc &in, int r_p, int r_s, bool short_tiles)
+{
+#define GNLM_ROWS_SPLIT(RP, RS, SH)                                                              \
+    if (r_p == RP && r_s == RS && short_tiles == SH) {                                             \
+        if constexpr (RP <= 3) return rows_split_parts<RowsCfg<RP, RS, SH>>(in, nullptr, rows_waves<RP, SH>()) >= 2; \
+        return false;                                                                            \
+    }
+    GNLM_ROWS_CASES(GNLM_ROWS_SPLIT)
+#undef GNLM_ROWS_SPLIT
+    return false;
+}
+
 hipError_t fwd3d_rows_launch(const InDesc &in, float *out, float *Zout, int r_p, int r_s, float k, bool short_tiles,
                              hipStream_t s, const Workspace *ws, int shape)
 {
