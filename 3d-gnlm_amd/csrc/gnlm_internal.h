@@ -84,6 +84,7 @@ void fwd3d_set_dz_split(bool on);  // gnlm_set_dz_split
 bool fwd2d_tiled_supported(int r_p, int r_s);
 hipError_t fwd2d_tiled_launch(const InDesc &in, float *out, float *Zout, int r_p, int r_s, float k,
                               hipStream_t s);
+void fwd2d_set_dy_split(bool on);  // gnlm_set_dz_split (the 2D dy split)
 // the 8-bit build: samples from in8, laid out as in describes (in.data unused)
 hipError_t fwd2d_tiled_u8_launch(const InDesc &in, const unsigned char *in8, float *out, float *Zout, int r_p,
                                  int r_s, float k, hipStream_t s);

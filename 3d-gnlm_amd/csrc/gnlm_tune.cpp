@@ -234,6 +234,7 @@ int gnlm_set_dz_split(int enable)
 {
     if (enable != 0 && enable != 1) return GNLM_EINVAL;
     fwd3d_set_dz_split(enable == 1);
+    fwd2d_set_dy_split(enable == 1);
     return GNLM_OK;
 }
 

@@ -28,11 +28,18 @@
 // uint8 images of the reference) every S is an exact integer in FP32 and the
 // results are identical bit for bit.
 //
+// dy split (SPLIT, images with fewer tiles than half the workgroup slots):
+// workgroup b computes tile b % ntiles over the dy range of part b / ntiles
+// and writes each dy's partial sums (Y, Z) into slot dy + r_s; a combine
+// pass forms 0 + partial(-r_s) + ... + partial(r_s) in dy order and y = Y/Z,
+// the additions the unsplit tile makes in registers: the same bits.
+//
 // 8-bit build (INT8): the tile is staged from the caller's 8-bit image, and
 // since every partial patch sum of squared 8-bit differences is an integer
 // below 2^24 (11 x 11 x 255^2 < 2^23), both box sums run as sliding windows
 // (nlm_box.h:box_run*), 2 operations per output instead of about 3: the same
 // S, so the same bits as the FP32 build on the converted image.
+#include <atomic>
 #include <type_traits>
 #include "gnlm_internal.h"
 #include "nlm_box.h"
@@ -64,6 +71,8 @@ struct Fwd2Args {
     int rs;
     float k;
     int ntx, nty;
+    float *slots;  // dy split: 2 (2 r_s + 1) floats per pixel (slot j: Y at 2 j npix, Z at (2 j + 1) npix)
+    int nsplit;    // dy split: parts per tile
 };
 
 __host__ __device__ constexpr int lds2_x(int TX, int R) { return (TX + 2 * R) | 1; }
@@ -129,7 +138,7 @@ __device__ __forceinline__ void static_for2(F &&f)
 template <int RP, int RSC>
 constexpr int waves2() { return RSC == 0 ? 1 : (RP >= 4 ? 3 : 4); }
 
-template <int RP, int RSC, bool INT8>
+template <int RP, int RSC, bool INT8, bool SPLIT>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(waves2<RP, RSC>()))) void nlm2d_fwd_tiled(
     Fwd2Args a)
 {
@@ -145,7 +154,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(waves2<RP, R
     float *U = lds;
     float *SXB = lds + LX * LY;
 
-    const int t = xcd_tile(blockIdx.x, a.ntx * a.nty);
+    const int ntiles = a.ntx * a.nty;
+    const int t = SPLIT ? (int)(blockIdx.x % ntiles) : xcd_tile(blockIdx.x, ntiles);
     const int x0 = (t % a.ntx) * G::TX;
     const int y0 = (t / a.ntx) * G::TY;
     const int tid = threadIdx.x;
@@ -194,8 +204,16 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(waves2<RP, R
     const int candbase = (R + cy0) * LX + (R + cx);
 
     const int rs = RSC > 0 ? RSC : a.rs;
+    int dylo = -rs, dyhi = rs;
+    if constexpr (SPLIT) {
+        const int part = blockIdx.x / ntiles, nd = 2 * rs + 1;
+        dylo = -rs + part * nd / a.nsplit;
+        dyhi = -rs + (part + 1) * nd / a.nsplit - 1;
+    }
+    const int gx = x0 + cx;
+    const long long npix = (long long)a.in.nx * a.in.ny;
     int it = 0;
-    for (int dy = -rs; dy <= rs; ++dy) {
+    for (int dy = dylo; dy <= dyhi; ++dy) {
         float Zd[G::YR], Yd[G::YR];  // per-dy partial sums (two-level accumulation)
 #pragma unroll
         for (int m = 0; m < G::YR; ++m) {
@@ -283,15 +301,28 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(waves2<RP, R
             for (int dx = -rs; dx <= rs; ++dx)
                 offset(dx, ((dx + rs) & 1) != 0, [&](int q, int i) { return U[base[q] + dy * LX + dx + i]; });
         }
+        if constexpr (SPLIT) {
+            if (gx < a.in.nx) {
+                float *ys = a.slots + 2LL * (dy + rs) * npix, *zs = ys + npix;
 #pragma unroll
-        for (int m = 0; m < G::YR; ++m) {
-            Zacc[m] = Zacc[m] + Zd[m];
-            Yacc[m] = Yacc[m] + Yd[m];
+                for (int m = 0; m < G::YR; ++m) {
+                    const int gy = y0 + cy0 + m;
+                    if (gy < a.in.ny) {
+                        ys[gx + (long long)a.in.nx * gy] = Yd[m];
+                        zs[gx + (long long)a.in.nx * gy] = Zd[m];
+                    }
+                }
+            }
+        } else {
+#pragma unroll
+            for (int m = 0; m < G::YR; ++m) {
+                Zacc[m] = Zacc[m] + Zd[m];
+                Yacc[m] = Yacc[m] + Yd[m];
+            }
         }
     }
 
-    const int gx = x0 + cx;
-    if (gx < a.in.nx) {
+    if (!SPLIT && gx < a.in.nx) {
 #pragma unroll
         for (int m = 0; m < G::YR; ++m) {
             const int gy = y0 + cy0 + m;
@@ -302,6 +333,44 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(waves2<RP, R
             }
         }
     }
+}
+
+// the dy split's totals: 0 + slot 0 + slot 1 + ... in dy order, then y = Y / Z
+__global__ void nlm2d_split_combine(const float *slots, int nd, long long npix, float *out, float *Zout)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < npix;
+         i += (long long)gridDim.x * blockDim.x) {
+        float Z = 0.0f, Y = 0.0f;
+        for (int j = 0; j < nd; ++j) {
+            Z = Z + slots[(2LL * j + 1) * npix + i];
+            Y = Y + slots[2LL * j * npix + i];
+        }
+        if (Zout) Zout[i] = Z;
+        out[i] = Y / Z;
+    }
+}
+
+std::atomic<bool> g_dy_split{true};  // gnlm_set_dz_split covers the 2D dy split too
+
+// Parts per tile of the dy split (1: none): as many parts (up to 2 r_s + 1)
+// as fill the workgroup slots in one round, for images whose tiles fill at
+// most a quarter of them, at search windows of 21 x 21 or more; the slots
+// stay within 256 MiB.  The split costs an allocation, a combine launch and
+// 16 (2 r_s + 1) bytes of traffic per pixel, so it pays only where a round of
+// tiles is long: at patch 11 / search 21, 256^2 0.088 against 0.238 ms and
+// 512^2 0.18 against 0.24 ms, but 768^2 (two parts) 0.46 against 0.30 ms;
+// at patch 5 / search 11 it was never faster (512^2 0.091 against 0.073 ms).
+template <int RP, int RSC>
+int split2_parts(long long tiles, long long npix, int rs)
+{
+    if (!g_dy_split.load(std::memory_order_relaxed)) return 1;
+    const long long nd = 2LL * rs + 1;
+    if (nd * nd < 441 || 8 * nd * npix > (256LL << 20)) return 1;
+    const long long by_lds = (long long)kLdsLimitBytes / (long long)fwd2_lds_bytes<RP>(rs);
+    const long long per_cu = by_lds < waves2<RP, RSC>() ? by_lds : waves2<RP, RSC>();
+    long long n = (long long)device_cus() * per_cu / tiles;
+    if (n > nd) n = nd;
+    return n < 4 ? 1 : (int)n;
 }
 
 template <int RP, int RSC, bool INT8>
@@ -318,12 +387,37 @@ hipError_t launch2_k(const InDesc &in, const unsigned char *in8, float *out, flo
     a.k = k;
     a.ntx = (in.nx + G::TX - 1) / G::TX;
     a.nty = (in.ny + G::TY - 1) / G::TY;
+    a.slots = nullptr;
+    a.nsplit = 1;
+    const long long tiles = (long long)a.ntx * a.nty;
+    const long long npix = (long long)in.nx * in.ny;
     const size_t lds = fwd2_lds_bytes<RP>(rs);
-    hipError_t e = hipFuncSetAttribute((const void *)nlm2d_fwd_tiled<RP, RSC, INT8>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    const int ns = split2_parts<RP, RSC>(tiles, npix, rs);
+    if (ns < 2) {
+        hipError_t e = hipFuncSetAttribute((const void *)nlm2d_fwd_tiled<RP, RSC, INT8, false>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((nlm2d_fwd_tiled<RP, RSC, INT8, false>), dim3((unsigned)tiles), dim3(NT), lds, s, a);
+        return hipGetLastError();
+    }
+    const int nd = 2 * rs + 1;
+    hipError_t e = hipMallocAsync((void **)&a.slots, sizeof(float) * 2 * (size_t)nd * (size_t)npix, s);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((nlm2d_fwd_tiled<RP, RSC, INT8>), dim3(a.ntx * a.nty), dim3(NT), lds, s, a);
-    return hipGetLastError();
+    a.nsplit = ns;
+    e = hipFuncSetAttribute((const void *)nlm2d_fwd_tiled<RP, RSC, INT8, true>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL((nlm2d_fwd_tiled<RP, RSC, INT8, true>), dim3((unsigned)(tiles * ns)), dim3(NT), lds, s, a);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+        const long long g = (npix + 255) / 256 < 4096 ? (npix + 255) / 256 : 4096;
+        hipLaunchKernelGGL(nlm2d_split_combine, dim3((unsigned)g), dim3(256), 0, s, (const float *)a.slots, nd, npix,
+                           out, Zout);
+        e = hipGetLastError();
+    }
+    const hipError_t e2 = hipFreeAsync(a.slots, s);
+    return e != hipSuccess ? e : e2;
 }
 
 // Search radii with a register-window build (the reference's own 2D geometry,
@@ -385,5 +479,7 @@ hipError_t fwd2d_tiled_u8_launch(const InDesc &in, const unsigned char *in8, flo
 {
     return launch2<true>(in, in8, out, Zout, r_p, r_s, k, s);
 }
+
+void fwd2d_set_dy_split(bool on) { g_dy_split.store(on, std::memory_order_relaxed); }
 
 }  // namespace gnlm

@@ -191,7 +191,9 @@ int gnlm_set_backward_parts(int nparts);
  * into the stream-ordered workspace (8 (2 r_s + 1) bytes per voxel, at most
  * 256 MiB); a combine pass adds them in dz order, the additions one workgroup
  * makes, so the output bits are those of the unsplit launch.  0 turns it off
- * (the unsplit launch).  GNLM_EINVAL for other values. */
+ * (the unsplit launch).  The same switch covers the 2D tiled forward, whose
+ * small images split each tile's dy range the same way (8 (2 r_s + 1) bytes
+ * per pixel).  GNLM_EINVAL for other values. */
 int gnlm_set_dz_split(int enable);
 
 /* ---- batched persistent forward --------------------------------------

@@ -135,3 +135,32 @@ def test_u8_2d_reference_geometry_full_size(dev):
                          underflow=underflow_bound(21 * 21, 255.0, ref64, Z64), B64=B64)
     print("u8 2d 2048", rep, flush=True)
     assert rep["ok"], rep
+
+
+@pytest.mark.parametrize("shape,rp,rs,h", [((512, 512), 5, 10, 10.0), ((128, 128), 5, 10, 10.0), ((300, 170), 2, 11, 12.0),
+                                           ((64, 200), 3, 12, 9.0), ((97, 33), 1, 10, 15.0), ((300, 170), 2, 5, 12.0)])
+def test_2d_dy_split_same_bits(dev, shape, rp, rs, h):
+    """The dy split of small images at search windows of 21 x 21 and more
+    (several workgroups per tile, each writing per-dy partial sums; a combine
+    pass adds them in dy order) gives the unsplit launch's bits, y and Z, on
+    the 8-bit and the FP32 builds (the register-window build of patch 11 /
+    search 21 and the run-time-radius builds; the last case takes no split)."""
+    img = _img(shape, sum(shape) + 5 * rp + rs)
+    t8 = torch.from_numpy(img).to(dev)
+    tf = t8.float()
+    outs = []
+    try:
+        for split in (True, False):
+            gnlm.set_dz_split(split)
+            print(f"case 2d dy split={split} shape={shape} rp={rp} rs={rs}", flush=True)
+            y8, Z8 = gnlm.nlm2d_u8(t8, rp, rs, h, want_Z=True)
+            yf, Zf = gnlm.nlm2d(tf, rp, rs, h, want_Z=True)
+            y1 = gnlm.nlm2d_u8(t8, rp, rs, h)
+            torch.cuda.synchronize()
+            outs.append((y8.cpu(), Z8.cpu(), yf.cpu(), Zf.cpu(), y1.cpu()))
+    finally:
+        gnlm.set_dz_split(True)
+    for o in outs:
+        for a, b in zip(o, outs[0]):
+            assert torch.equal(a, b)
+    assert torch.equal(outs[0][0], outs[0][2]) and torch.equal(outs[0][4], outs[0][0])
