@@ -94,8 +94,9 @@ int rows_split_parts(const InDesc &in, const Workspace *ws, int waves)
 //   r_p 1-3, last:  the dz split of shape 0's build where rows_split_parts
 //                   allows one (small volumes), else shape 0 itself; the
 //                   default rule (no recorded shape) takes it wherever
-//                   rows_split_parts allows it: the split at least doubles
-//                   the workgroups of a launch that fills under half the GPU
+//                   rows_split_parts allows it and r_s >= 3: the split at
+//                   least doubles the workgroups of a launch that fills
+//                   under half the GPU
 // (deep 16 x 10 x 20 tiles for r_p = 3 give the same bits too but ran 2-8%
 // slower than both builds at 96^3-256^3, so they are not a candidate)
 // (r_p = 4's big 16 x 24 x 24 tile is launched from the variant dispatch.)
@@ -116,7 +117,10 @@ hipError_t rows_launch_t(const InDesc &in, float *out, float *Zout, float k, hip
 {
     using C = RowsCfg<RP, RS, SHORT>;
     if constexpr (RP <= 3) {
-        if (shape < 0 || shape == rows_num_shapes(RP, SHORT) - 1) {
+        // the fixed rule takes the split from r_s = 3 on (343 offsets and
+        // more per voxel: the slots' traffic and the combine stay small
+        // against the work); the tuner times it wherever it applies
+        if ((shape < 0 && RS >= 3) || shape == rows_num_shapes(RP, SHORT) - 1) {
             constexpr int W0 = rows_waves<RP, SHORT>();
             const int ns = rows_split_parts<C>(in, ws, W0);
             if (ns >= 2)
