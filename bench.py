@@ -685,8 +685,8 @@ def main():
         "parity": parity,
     }
     if args.workload == "nlm3d_256_p5_s11" and not args.no_extras:
-        # the north star's other two targets, on the same line: the batched
-        # persistent launch (64 uneven volumes) and fwd+bwd vs fwd (8 x 128^3)
+        # the two other targets, on the same line: the batched persistent
+        # launch (64 uneven volumes) and fwd+bwd vs fwd (8 x 128^3)
         sub = argparse.Namespace(**vars(args))
         sub.steps, sub.warmup = 3, 1
         line["batched"] = batch_summary(bench_batch(sub))
